@@ -1,0 +1,35 @@
+/*
+ * qd_test.h - diagnostic entry points of libqdamd, used by the test suite
+ * to check single building blocks of the propagator on the device.
+ * Each replaces one vendor routine the reference calls on its hot path.
+ */
+#ifndef QD_TEST_H
+#define QD_TEST_H
+
+#include <stdint.h>
+#include "qd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Real GEMM C (rows x n) = A (rows x k) . B (k x n), all column-major, run
+ * with the Legendre-transform MFMA kernel.  Replaces the cublasZgemm calls
+ * with a zero-imaginary B (omegawavepacket.cu:212-218, :233-239). n <= 128. */
+qd_status qd_probe_gemm(const double *A, int32_t rows, int32_t k, const double *B, int32_t n, double *C);
+
+/* In-place unnormalised DFT of `count` contiguous complex pencils of length n
+ * with the LDS Stockham kernel (forward exp(-2 pi i jk/n), inverse +).
+ * Replaces cufftExecZ2Z (omegawavepacket.cu:242-269). */
+qd_status qd_probe_fft(double *data, int32_t n, int32_t count, int32_t inverse);
+
+/* Eigendecomposition of the n x n Coriolis matrix C_l over
+ * omega = omega_min..omega_min+n-1 (host solver; replaces LAPACK dstev,
+ * CoriolisMatrix.C:63-99).  evecs[alpha*n + i] = component i of vector alpha. */
+qd_status qd_probe_coriolis(int32_t J, int32_t l, int32_t omega_min, int32_t n, double *evals, double *evecs);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QD_TEST_H */

@@ -1,0 +1,164 @@
+"""Binding to libqdamd.so (the HIP propagator behind include/qd.h).
+
+The library is the only compute path: if it is not built, `lib()` raises.
+There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import LIB_PATH
+from . import abi
+
+_lib = None
+
+_DECLS = {
+    "qd_evolve": (C.c_int, [C.c_void_p]),
+    "qd_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "qd_run": (C.c_int, [C.c_void_p, C.c_int32]),
+    "qd_synchronize": (C.c_int, [C.c_void_p]),
+    "qd_copy_wavepackets": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "qd_get_observables": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]),
+    "qd_get_autocorrelation": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]),
+    "qd_get_crp": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "qd_steps_done": (C.c_int32, [C.c_void_p]),
+    "qd_destroy": (None, [C.c_void_p]),
+    "qd_last_error": (C.c_char_p, [C.c_void_p]),
+    "qd_last_global_error": (C.c_char_p, []),
+    "qd_profile_steps": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    "qd_version": (C.c_char_p, []),
+    "qd_probe_gemm": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]),
+    "qd_probe_fft": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
+    "qd_probe_coriolis": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
+}
+
+
+class QDError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"qd error {code}: {msg}")
+        self.code = code
+
+
+def lib(path: Optional[str] = None):
+    """Load libqdamd.so; raises FileNotFoundError if it has not been built."""
+    global _lib
+    if _lib is None:
+        p = path or LIB_PATH
+        if not os.path.exists(p):
+            raise FileNotFoundError(f"{p} is not built (run __graft_entry__.build())")
+        _lib = C.CDLL(p)
+        for name, (res, args) in _DECLS.items():
+            f = getattr(_lib, name)
+            f.restype = res
+            f.argtypes = args
+    return _lib
+
+
+def _check(rc: int, h=None):
+    if rc != abi.QD_OK:
+        L = lib()
+        msg = (L.qd_last_error(h) if h else L.qd_last_global_error()) or b""
+        raise QDError(rc, msg.decode())
+
+
+def evolve(inputs: abi.Inputs) -> None:
+    """qd_evolve: the one-shot drop-in (create, run total_steps, copy back, destroy)."""
+    _check(lib().qd_evolve(C.addressof(inputs.struct)))
+
+
+class Propagator:
+    """A resident job (qd_create / qd_run / ... / qd_destroy)."""
+
+    def __init__(self, inputs: abi.Inputs):
+        self.inputs = inputs
+        h = C.c_void_p()
+        _check(lib().qd_create(C.addressof(inputs.struct), C.byref(h)))
+        self.h = h
+
+    def run(self, n_steps: int) -> None:
+        _check(lib().qd_run(self.h, n_steps), self.h)
+
+    def synchronize(self) -> None:
+        _check(lib().qd_synchronize(self.h), self.h)
+
+    @property
+    def steps_done(self) -> int:
+        return lib().qd_steps_done(self.h)
+
+    def wavepackets(self) -> np.ndarray:
+        psi = np.zeros_like(self.inputs.psi)
+        _check(lib().qd_copy_wavepackets(self.h, psi.ctypes.data), self.h)
+        return psi
+
+    def observables(self, first: int = 0, count: Optional[int] = None) -> np.ndarray:
+        n = self.steps_done - first if count is None else count
+        nO = len(self.inputs.omegas)
+        out = np.zeros((n, nO, abi.QD_OBS_FIELDS))
+        _check(lib().qd_get_observables(self.h, first, n, out.ctypes.data), self.h)
+        return out
+
+    def autocorrelation(self, first: int = 0, count: Optional[int] = None) -> np.ndarray:
+        n = self.steps_done - first if count is None else count
+        out = np.zeros(n, dtype=np.complex128)
+        _check(lib().qd_get_autocorrelation(self.h, first, n, out.ctypes.data), self.h)
+        return out
+
+    def crp(self) -> np.ndarray:
+        out = np.zeros(len(self.inputs.energies))
+        _check(lib().qd_get_crp(self.h, out.ctypes.data), self.h)
+        return out
+
+    def profile(self, n_steps: int) -> dict:
+        us = np.zeros(abi.QD_KERNEL_KINDS)
+        _check(lib().qd_profile_steps(self.h, n_steps, us.ctypes.data), self.h)
+        return dict(zip(abi.KERNEL_NAMES, us.tolist()))
+
+    def close(self) -> None:
+        if self.h:
+            lib().qd_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# probes (include/qd_test.h)
+# ---------------------------------------------------------------------------
+
+def probe_gemm(A: np.ndarray, B: np.ndarray) -> np.ndarray:
+    A = np.asfortranarray(A, dtype=np.float64)
+    B = np.asfortranarray(B, dtype=np.float64)
+    rows, k = A.shape
+    n = B.shape[1]
+    Cm = np.zeros((rows, n), order="F")
+    _check(lib().qd_probe_gemm(A.ctypes.data, rows, k, B.ctypes.data, n, Cm.ctypes.data))
+    return Cm
+
+
+def probe_fft(x: np.ndarray, inverse: bool = False) -> np.ndarray:
+    """x: (count, n) complex; returns the unnormalised DFT of each row."""
+    a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    count, n = a.shape
+    _check(lib().qd_probe_fft(a.ctypes.data, n, count, int(inverse)))
+    return a
+
+
+def probe_coriolis(J: int, l: int, omega_min: int, n: int):
+    e = np.zeros(n)
+    v = np.zeros((n, n), order="F")
+    _check(lib().qd_probe_coriolis(J, l, omega_min, n, e.ctypes.data, v.ctypes.data))
+    return e, v

@@ -1,0 +1,114 @@
+// host_internal.h - host-side classes of libqdamd: validated job setup,
+// device residency of one partial-wave job, and the step engine that launches
+// the HIP kernels for one or many resident jobs.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <limits>
+#include <cstdlib>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "qd.h"
+#include "qd_internal.h"
+
+namespace qd {
+
+struct Error : std::runtime_error {
+  qd_status code;
+  Error(qd_status c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define QD_HIP(x)                                                                              \
+  do {                                                                                         \
+    hipError_t qd_e_ = (x);                                                                    \
+    if (qd_e_ != hipSuccess)                                                                   \
+      throw ::qd::Error(QD_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(qd_e_));     \
+  } while (0)
+
+#define QD_REQUIRE(cond, msg)                                   \
+  do {                                                          \
+    if (!(cond)) throw ::qd::Error(QD_ERR_INVALID, (msg));      \
+  } while (0)
+
+// coriolis.cpp
+void coriolis_tridiagonal(int J, int l, int omega_min, int n, double *diag, double *off);
+void tridiag_eigen(int n, double *d, double *e, double *z);
+
+// Host description of one job after validation (shapes + borrowed pointers).
+struct JobShape {
+  int n1, n2, nt, M, L, lmax, nO, omin, kmax, J, parity;
+  double dt, dr1, dr2, left1, left2, m1, m2;
+  bool dump, crp, acf, mixed;
+  int steps0;           // initial value of time.steps
+  int legendre_n_max;
+};
+
+// Throws Error(QD_ERR_INVALID) on any inconsistency; reads only host memory.
+JobShape validate_inputs(const qd_inputs *in);
+
+// Device memory owned by one job.
+class DeviceJob {
+ public:
+  DeviceJob(const qd_inputs *in, const JobShape &shape, hipStream_t stream);
+  ~DeviceJob();
+  DeviceJob(const DeviceJob &) = delete;
+  DeviceJob &operator=(const DeviceJob &) = delete;
+
+  JobShape shape;
+  JobDev dev{};          // descriptor (device pointers), uploaded by the engine
+  size_t footprint = 0;  // bytes of device memory held
+  int records_done = 0;  // records drained to the host
+  int steps = 0;         // host mirror of the device step counter
+  std::vector<double> obs;      // [records][nO][5]
+  std::vector<double> acf;      // [records][2]
+
+  // copy the ring records not yet drained (requires the stream to be idle)
+  void drain(int records_total);
+  void copy_psi(double *dst, hipStream_t s) const;
+  void flux_to_crp(const qd_inputs *in, double *crp) const;
+
+  static size_t estimate_footprint(const JobShape &s, int n_energies);
+
+ private:
+  std::vector<void *> allocs_;
+  void *alloc(size_t bytes);
+  template <class T> T *upload(const std::vector<T> &v, hipStream_t s);
+};
+
+// Launches the step kernels for a set of resident jobs.
+class Engine {
+ public:
+  explicit Engine(hipStream_t s);
+  ~Engine();
+  void set_jobs(const std::vector<DeviceJob *> &jobs);
+  // enqueue one step for every job; with ev != nullptr records
+  // QD_KERNEL_KINDS+1 events around the launches
+  void step(hipEvent_t *ev = nullptr);
+  const std::vector<DeviceJob *> &jobs() const { return jobs_; }
+  hipStream_t stream() const { return stream_; }
+  // graph capture of one step (after set_jobs); replays on the stream
+  void step_graph();
+  void drop_graph();
+
+ private:
+  hipStream_t stream_;
+  std::vector<DeviceJob *> jobs_;
+  JobDev *d_jobs_ = nullptr;
+  int *d_bases_ = nullptr;      // [KK_COUNT][n_jobs + 1]
+  int n_alloc_jobs_ = 0;
+  int total_[KK_COUNT] = {};
+  size_t lds_fftR_ = 0, lds_fftr_ = 0;
+  int fwd_tiles_ = 0, bwd_tiles_ = 0, fwd_kp_ = 0, bwd_kp_ = 0, nb_max_ = 1;
+  int mixed_ = 0, any_flux_ = 0;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t graph_exec_ = nullptr;
+  StepArgs args(int kind) const;
+  void enqueue(hipEvent_t *ev);
+};
+
+void init_kernel_attributes();
+
+}  // namespace qd

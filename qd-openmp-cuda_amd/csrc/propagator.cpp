@@ -1,0 +1,503 @@
+// propagator.cpp - input validation, device residency of a partial-wave job
+// and the step engine.
+//
+// Reference counterparts:
+//   validate_inputs ... the insist() checks spread over omegawavepacket.cu:46,
+//                       :142-143, wavepackets1device.cu:276-284, CoriolisMatrix.C:65-71,
+//                       cudaMath.h:119 (odd n), matlabStructures.C:19-22 (unchecked
+//                       classes) and the norm check of cudaOpenMP2.cu:437-440
+//   DeviceJob ......... OmegaWavepacketsOnSingleDevice::setup_data_on_device
+//                       (wavepackets1device.cu:58-150) + OmegaWavepacket::setup_device_data
+//                       (omegawavepacket.cu:66-198) + setup_coriolis_matrices_and_copy_to_device
+//                       (cudaOpenMP2.cu:158-209) + setup_constant_memory_on_device
+//                       (wavepackets1device.cu:265-287)
+//   Engine::step ...... one iteration of CudaOpenMPQMMD::test_multiple_cards
+//                       (cudaOpenMP2.cu:338-387)
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <numeric>
+
+#include "host_internal.h"
+
+namespace qd {
+
+namespace {
+
+constexpr size_t kLdsLimit = 160 * 1024;
+constexpr int kRing = 1024;
+
+bool factors_235(int n) {
+  if (n < 2) return false;
+  for (int p : {2, 3, 5})
+    while (n % p == 0) n /= p;
+  return n == 1;
+}
+
+int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+int gemm_lds_stride(int nt_tiles) { return nt_tiles * 16 + ((nt_tiles & 1) ? 0 : 16); }
+
+size_t gemm_lds_bytes(int kp, int np, bool mixed) {
+  const int nt = np / 16;
+  return (size_t)kp * gemm_lds_stride(nt) * (mixed ? 2 * sizeof(float) : sizeof(double));
+}
+
+int pick_divisor(int n, const int *cands, int ncand, int limit_elems, int other) {
+  for (int q = 0; q < ncand; q++)
+    if (n % cands[q] == 0 && cands[q] * other <= limit_elems) return cands[q];
+  return 1;
+}
+
+}  // namespace
+
+void init_attrs_fft();
+void init_attrs_gemm();
+
+void init_kernel_attributes() {
+  static bool done = false;
+  if (done) return;
+  init_attrs_fft();
+  init_attrs_gemm();
+  (void)hipGetLastError();   // an unsupported attribute must not leak into later error checks
+  done = true;
+}
+
+JobShape validate_inputs(const qd_inputs *in) {
+  QD_REQUIRE(in != nullptr, "inputs pointer is NULL");
+  JobShape s{};
+  const qd_omega_states &os = in->omega_states;
+  s.n1 = in->r1.n; s.n2 = in->r2.n; s.nt = in->theta.n;
+  QD_REQUIRE(s.n1 > 0 && s.n2 > 0 && s.nt > 0, "grid sizes must be positive");
+  QD_REQUIRE(s.n1 % 2 == 0 && s.n2 % 2 == 0,
+             "r1.n and r2.n must be even (the kinetic table is undefined for odd n, cudaMath.h:119)");
+  QD_REQUIRE(factors_235(s.n1) && factors_235(s.n2) && s.n1 <= kFftMaxN && s.n2 <= kFftMaxN,
+             "r1.n and r2.n must factor into 2, 3, 5 and not exceed 512");
+  QD_REQUIRE(std::isfinite(in->r1.dr) && in->r1.dr > 0 && std::isfinite(in->r2.dr) && in->r2.dr > 0,
+             "r1.dr and r2.dr must be positive");
+  QD_REQUIRE(in->r1.mass > 0 && in->r2.mass > 0, "masses must be positive");
+  QD_REQUIRE(in->r1.left > 0 && in->r2.left > 0, "radial grids must start at r > 0 (1/r^2 terms)");
+  QD_REQUIRE(in->theta.x && in->theta.w, "theta.x and theta.w are required");
+  QD_REQUIRE(in->pot, "pot is required");
+  QD_REQUIRE(in->time.steps, "time.steps is required (in/out)");
+  QD_REQUIRE(in->time.total_steps >= 0, "time.total_steps must be >= 0");
+  QD_REQUIRE(std::isfinite(in->time.time_step), "time.time_step must be finite");
+  QD_REQUIRE(*in->time.steps >= 0, "time.steps must be >= 0");
+  s.lmax = os.l_max; s.L = os.l_max + 1;
+  QD_REQUIRE(os.l_max >= 0, "l_max must be >= 0");
+  QD_REQUIRE(s.nt > s.lmax + 1, "theta.n must exceed l_max + 1 (omegawavepacket.cu:46)");
+  s.nO = os.n_omegas;
+  QD_REQUIRE(s.nO >= 1 && os.omegas, "at least one omega block is required");
+  QD_REQUIRE(s.nO <= kMaxOmega, "at most 24 omega blocks per job");
+  QD_REQUIRE(os.n_omegas_max >= s.nO, "n_omegas_max < number of omegas (matlabStructures.C:108)");
+  s.J = os.J; s.parity = os.parity;
+  QD_REQUIRE(s.J >= 0, "J must be >= 0");
+  QD_REQUIRE(s.J != 0 || s.parity == 0, "J = 0 requires parity 0 (CoriolisMatrix.C:65)");
+  QD_REQUIRE(s.parity == 0 || s.parity == 1, "parity must be 0 or 1");
+  const int omin_rule = (s.J + s.parity) % 2 == 0 ? 0 : 1;
+  s.omin = os.omegas[0];
+  QD_REQUIRE(s.omin == (s.J == 0 ? 0 : omin_rule),
+             "omegas[0] must be 0 when J + p is even, 1 when odd (OmegaList.m:4-16)");
+  for (int b = 1; b < s.nO; b++)
+    QD_REQUIRE(os.omegas[b] == s.omin + b, "omegas must be consecutive integers (omegawavepacket.cu:142)");
+  const int kcap = std::min(s.J, s.lmax);
+  s.kmax = in->k_max >= 0 ? std::min(kcap, (int)in->k_max) : kcap;
+  QD_REQUIRE(s.omin + s.nO - 1 == s.kmax,
+             "the omega list must run to min(J, l_max, k_max): the Coriolis exponential mixes all of them");
+  QD_REQUIRE(os.legendre_n_max == s.lmax - s.omin + 1,
+             "associated_legendres must be (theta.n, l_max - omegas[0] + 1, n_omegas)");
+  QD_REQUIRE(os.associated_legendres, "associated_legendres is required");
+  QD_REQUIRE(os.wave_packets, "wave_packets is required");
+  s.legendre_n_max = os.legendre_n_max;
+  s.M = s.n1 * s.n2;
+  s.dt = in->time.time_step;
+  s.dr1 = in->r1.dr; s.dr2 = in->r2.dr; s.left1 = in->r1.left; s.left2 = in->r2.left;
+  s.m1 = in->r1.mass; s.m2 = in->r2.mass;
+  s.dump = in->dump1.dump && in->dump2.dump;
+  s.mixed = in->precision == QD_PREC_MIXED;
+  QD_REQUIRE(in->precision == QD_PREC_FP64 || in->precision == QD_PREC_MIXED, "unknown precision");
+  s.steps0 = *in->time.steps;
+  s.acf = true;
+  const qd_crp &c = in->crp;
+  s.crp = c.calculate_CRP != 0 && c.n_energies > 0;
+  if (s.crp) {
+    QD_REQUIRE(c.energies && c.eta_sq && c.CRP, "CRP.energies, eta_sq and CRP are required");
+    QD_REQUIRE(c.n_gradient_points >= 3 && c.n_gradient_points % 2 == 1,
+               "CRP.n_gradient_points must be odd and >= 3");
+    const int h = c.n_gradient_points / 2;
+    QD_REQUIRE(c.n_dividing_surface - h >= 0 && c.n_dividing_surface + h < s.n2,
+               "CRP dividing surface stencil leaves the r grid");
+  }
+  const bool copy = in->options.wave_to_matlab || in->options.wave_callback;
+  QD_REQUIRE(!copy || in->options.steps_to_copy_psi_from_device_to_host > 0,
+             "steps_to_copy_psi_from_device_to_host must be > 0 when copy-back is enabled");
+  // LDS limits of the Legendre GEMMs (B operand staged whole)
+  const int nl_max = s.lmax - s.omin + 1;
+  QD_REQUIRE(gemm_lds_bytes(round_up(s.nt, 4), round_up(nl_max, 16), s.mixed) <= kLdsLimit &&
+                 gemm_lds_bytes(round_up(nl_max, 4), round_up(s.nt, 16), s.mixed) <= kLdsLimit,
+             "theta.n / l_max too large for the LDS-resident Legendre tables");
+  // initial norm (cudaOpenMP2.cu:437-440 would abort at the first step)
+  const long G = (long)s.M * s.nt;
+  double norm = 0.0;
+  for (int b = 0; b < s.nO; b++)
+    for (int k = 0; k < s.nt; k++) {
+      const double *p = os.wave_packets + 2 * ((long)b * G + (long)k * s.M);
+      double a = 0.0;
+      for (long m = 0; m < 2L * s.M; m++) a += p[m] * p[m];
+      norm += in->theta.w[k] * a;
+    }
+  norm *= s.dr1 * s.dr2;
+  QD_REQUIRE(std::isfinite(norm), "initial wavepacket is not finite");
+  QD_REQUIRE(norm <= 1.0 + 1e-10, "initial wavepacket norm exceeds 1");
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// DeviceJob
+// ---------------------------------------------------------------------------
+void *DeviceJob::alloc(size_t bytes) {
+  void *p = nullptr;
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) throw Error(QD_ERR_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  allocs_.push_back(p);
+  footprint += bytes;
+  return p;
+}
+
+template <class T>
+T *DeviceJob::upload(const std::vector<T> &v, hipStream_t s) {
+  T *p = (T *)alloc(v.size() * sizeof(T));
+  if (!v.empty()) QD_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  return p;
+}
+
+DeviceJob::~DeviceJob() {
+  for (void *p : allocs_) (void)hipFree(p);
+}
+
+size_t DeviceJob::estimate_footprint(const JobShape &s, int n_energies) {
+  const size_t G = (size_t)s.M * s.nt;
+  size_t b = 0;
+  b += 16 * G * s.nO * 2;                 // psi + psi0
+  b += 16 * (size_t)s.M * s.L * s.nO;     // fbr
+  b += 24 * G;                            // vphase + pot
+  b += 16 * (size_t)s.L * s.n1 * s.nO * s.nO;
+  b += 16 * (size_t)n_energies * s.nO * s.nt * s.n1 * 2;
+  b += 4 << 20;                           // tables, partials, ring
+  return b;
+}
+
+DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : shape(sh) {
+  const int n1 = sh.n1, n2 = sh.n2, nt = sh.nt, M = sh.M, L = sh.L, nO = sh.nO;
+  const long G = (long)M * nt;
+  const double dt = sh.dt;
+  JobDev &d = dev;
+  d.n1 = n1; d.n2 = n2; d.nt = nt; d.M = M; d.L = L; d.lmax = sh.lmax; d.nO = nO;
+  d.omin = sh.omin; d.kmax = sh.kmax; d.J = sh.J; d.mixed = sh.mixed ? 1 : 0;
+  d.dt = dt; d.dr1 = sh.dr1; d.dr2 = sh.dr2;
+  steps = sh.steps0;
+
+  // ---- work decomposition
+  const int nl_max = sh.lmax - sh.omin + 1;
+  d.fwd_kp = round_up(nt, 4);
+  d.fwd_np = round_up(nl_max, 16);
+  d.bwd_kp = round_up(nl_max, 4);
+  d.bwd_np = round_up(nt, 16);
+  const long strips = (2L * M + 15) / 16;
+  d.gemm_strips = (int)std::max(1L, strips * nO / (4L * 1536));
+  d.gemm_fwd_groups = (int)((strips + 4L * d.gemm_strips - 1) / (4L * d.gemm_strips));
+  d.gemm_bwd_groups = d.gemm_fwd_groups;
+  static const int rows_c[] = {8, 4, 2, 1};
+  static const int cols_c[] = {16, 8, 4, 2, 1};
+  d.fft_rows = pick_divisor(n2, rows_c, 4, 2048, n1);
+  d.fft_cols = pick_divisor(n1, cols_c, 5, 2048, n2);
+  d.obs_chunks = std::max(1, (M + 2047) / 2048);
+  d.fbr_chunks = (M + kBlock - 1) / kBlock;
+  long slabs = 0;
+  for (int b = 0; b < nO; b++) slabs += L - (sh.omin + b);
+  d.units[KK_DVR] = (int)((G + kBlock * 4 - 1) / (kBlock * 4));
+  d.units[KK_LEG_FWD] = nO * d.gemm_fwd_groups;
+  d.units[KK_LEG_BWD] = nO * d.gemm_bwd_groups;
+  d.units[KK_FBR] = (L - sh.omin) * d.fbr_chunks;
+  d.units[KK_FFT_R] = (int)(slabs * (n2 / d.fft_rows));
+  d.units[KK_FFT_r] = (int)(slabs * (n1 / d.fft_cols));
+  d.units[KK_OBS] = nO * nt * d.obs_chunks;
+  d.units[KK_REDUCE] = 1;
+  d.units[KK_FLUX] = 0;
+
+  // ---- host tables
+  std::vector<double> r1(n1), r2(n2), I1(n1), I2(n2), kinE1(n1), kinE2(n2);
+  for (int i = 0; i < n1; i++) {
+    r1[i] = sh.left1 + i * sh.dr1;
+    I1[i] = 1.0 / (2 * sh.m1 * r1[i] * r1[i]);                 // cudaMath.h:150-157
+  }
+  for (int j = 0; j < n2; j++) {
+    r2[j] = sh.left2 + j * sh.dr2;
+    I2[j] = 1.0 / (2 * sh.m2 * r2[j] * r2[j]);
+  }
+  auto kin_table = [](int n, double dr, double mass, std::vector<double> &out) {   // cudaMath.h:117-130
+    const double two_pi_xl = 2 * M_PI / (n * dr);
+    for (int i = 0; i < n; i++) {
+      const double k = (i <= n / 2) ? two_pi_xl * i : two_pi_xl * (-n + i);
+      out[i] = k * k / (mass + mass);
+    }
+  };
+  kin_table(n1, sh.dr1, sh.m1, kinE1);
+  kin_table(n2, sh.dr2, sh.m2, kinE2);
+  auto phase = [](double x) { return make_double2(std::cos(x), std::sin(x)); };
+  std::vector<double2> kin1(n1), kin2(n2);
+  for (int i = 0; i < n1; i++) kin1[i] = phase(-dt * kinE1[i]);
+  for (int j = 0; j < n2; j++) kin2[j] = phase(-dt * kinE2[j]);
+  auto twiddles = [](int n) {
+    std::vector<double2> w(n);
+    for (int e = 0; e < n; e++) {
+      const long double a = -2.0L * 3.141592653589793238462643383279502884L * e / n;
+      w[e] = make_double2((double)cosl(a), (double)sinl(a));
+    }
+    return w;
+  };
+  // potential phase with the absorber folded in (evolutionCUDAaux.cu:5-11, :243-252)
+  std::vector<double2> vph(G);
+  std::vector<double> pot(in->pot, in->pot + G);
+  for (long idx = 0; idx < G; idx++) {
+    const int i = idx % n1, j = (idx / n1) % n2;
+    const double dd = sh.dump ? in->dump1.dump[i] * in->dump2.dump[j] : 1.0;
+    const double2 p = phase(-dt * pot[idx]);
+    vph[idx] = make_double2(dd * p.x, dd * p.y);
+  }
+  // Legendre tables (omegawavepacket.cu:134-181), padded row-major [k][c]
+  std::vector<double> wP((size_t)nO * d.fwd_kp * d.fwd_np, 0.0), Pt((size_t)nO * d.bwd_kp * d.bwd_np, 0.0);
+  const double *Pin = in->omega_states.associated_legendres;
+  for (int b = 0; b < nO; b++) {
+    const int nl = sh.lmax - (sh.omin + b) + 1;
+    const double *pb = Pin + (size_t)b * nt * sh.legendre_n_max;
+    for (int c = 0; c < nl; c++)
+      for (int k = 0; k < nt; k++) {
+        const double p = pb[(size_t)c * nt + k];
+        wP[((size_t)b * d.fwd_kp + k) * d.fwd_np + c] = in->theta.w[k] * p;
+        Pt[((size_t)b * d.bwd_kp + c) * d.bwd_np + k] = p;
+      }
+  }
+  // Coriolis exponentials with the R part of the rotational phase:
+  //   corF(l, i)[b][b1] = exp(-i dt/2 l(l+1) I1_i) sum_a exp(-i dt/2 e_a / (2 m1 R_i^2)) v_ab v_ab1
+  std::vector<double2> corF((size_t)L * n1 * nO * nO, make_double2(0.0, 0.0));
+  std::vector<double2> cmat((size_t)L * nO, make_double2(0.0, 0.0));
+  std::vector<double2> rot2((size_t)L * n2);
+  const double tau = dt / 2;
+  for (int l = 0; l < L; l++)
+    for (int j = 0; j < n2; j++) rot2[(size_t)l * n2 + j] = phase(-tau * l * (l + 1) * I2[j]);
+  for (int l = sh.omin; l < L; l++) {
+    const int n = std::min(sh.kmax, l) - sh.omin + 1;
+    std::vector<double> diag(n), off(std::max(1, n - 1)), z(n * n);
+    coriolis_tridiagonal(sh.J, l, sh.omin, n, diag.data(), off.data());
+    for (int b = 0; b < n; b++)
+      cmat[(size_t)l * nO + b] = make_double2(diag[b], b + 1 < n ? off[b] : 0.0);
+    tridiag_eigen(n, diag.data(), off.data(), z.data());
+    for (int i = 0; i < n1; i++) {
+      const double2 pr = phase(-tau * l * (l + 1) * I1[i]);
+      std::vector<double2> ea(n);
+      for (int a = 0; a < n; a++) ea[a] = phase(-tau * I1[i] * diag[a]);
+      double2 *F = corF.data() + ((size_t)l * n1 + i) * nO * nO;
+      for (int b = 0; b < n; b++)
+        for (int b1 = 0; b1 < n; b1++) {
+          double re = 0.0, im = 0.0;
+          for (int a = 0; a < n; a++) {
+            const double vv = z[a * n + b] * z[a * n + b1];
+            re += ea[a].x * vv;
+            im += ea[a].y * vv;
+          }
+          F[b * nO + b1] = make_double2(pr.x * re - pr.y * im, pr.x * im + pr.y * re);
+        }
+    }
+  }
+
+  // ---- device state
+  d.psi = (double2 *)alloc(16 * G * nO);
+  QD_HIP(hipMemcpyAsync(d.psi, in->omega_states.wave_packets, 16 * G * nO, hipMemcpyHostToDevice, st));
+  d.fbr = (double2 *)alloc(16 * (size_t)M * L * nO);
+  QD_HIP(hipMemsetAsync(d.fbr, 0, 16 * (size_t)M * L * nO, st));
+  d.psi0 = (double2 *)alloc(16 * G * nO);
+  QD_HIP(hipMemcpyAsync(d.psi0, in->omega_states.wave_packets, 16 * G * nO, hipMemcpyHostToDevice, st));
+  d.vphase = upload(vph, st);
+  d.pot = upload(pot, st);
+  d.w = upload(std::vector<double>(in->theta.w, in->theta.w + nt), st);
+  d.wP = upload(wP, st);
+  d.Pt = upload(Pt, st);
+  if (sh.mixed) {
+    std::vector<float> h(wP.size()), lo(wP.size()), h2(Pt.size()), lo2(Pt.size());
+    for (size_t q = 0; q < wP.size(); q++) { h[q] = (float)wP[q]; lo[q] = (float)(wP[q] - (double)h[q]); }
+    for (size_t q = 0; q < Pt.size(); q++) { h2[q] = (float)Pt[q]; lo2[q] = (float)(Pt[q] - (double)h2[q]); }
+    d.wP_hi = upload(h, st); d.wP_lo = upload(lo, st);
+    d.Pt_hi = upload(h2, st); d.Pt_lo = upload(lo2, st);
+  }
+  d.corF = upload(corF, st);
+  d.rot2 = upload(rot2, st);
+  d.kin1 = upload(kin1, st);
+  d.kin2 = upload(kin2, st);
+  d.kinE1 = upload(kinE1, st);
+  d.kinE2 = upload(kinE2, st);
+  d.I1 = upload(I1, st);
+  d.I2 = upload(I2, st);
+  d.cmat = upload(cmat, st);
+  d.twR = upload(twiddles(n1), st);
+  d.twr = upload(twiddles(n2), st);
+  d.part_fbr = (double *)alloc(sizeof(double) * (size_t)d.units[KK_FBR] * nO * 2);
+  d.part_ke = (double *)alloc(sizeof(double) * (size_t)d.units[KK_FFT_r]);
+  d.part_obs = (double *)alloc(sizeof(double) * (size_t)d.units[KK_OBS] * 4);
+  d.ring = kRing;
+  d.obs_ring = (double *)alloc(sizeof(double) * (size_t)kRing * nO * 5);
+  d.acf_ring = (double2 *)alloc(sizeof(double2) * kRing);
+  d.counters = (int *)alloc(2 * sizeof(int));
+  const int cnt[2] = {sh.steps0, 0};
+  QD_HIP(hipMemcpyAsync(d.counters, cnt, sizeof(cnt), hipMemcpyHostToDevice, st));
+  d.flux_on = 0;
+  QD_HIP(hipStreamSynchronize(st));   // host tables go out of scope
+}
+
+void DeviceJob::drain(int records_total) {
+  const int nO = shape.nO;
+  if (records_total <= records_done) return;
+  QD_REQUIRE(records_total - records_done <= dev.ring, "observable ring overflow");
+  obs.resize((size_t)records_total * nO * 5);
+  acf.resize((size_t)records_total * 2);
+  for (int r = records_done; r < records_total; r++) {
+    const int slot = r % dev.ring;
+    QD_HIP(hipMemcpy(obs.data() + (size_t)r * nO * 5, dev.obs_ring + (size_t)slot * nO * 5,
+                     sizeof(double) * nO * 5, hipMemcpyDeviceToHost));
+    QD_HIP(hipMemcpy(acf.data() + (size_t)r * 2, dev.acf_ring + slot, sizeof(double2), hipMemcpyDeviceToHost));
+  }
+  records_done = records_total;
+}
+
+void DeviceJob::copy_psi(double *dst, hipStream_t s) const {
+  const size_t bytes = 16 * (size_t)shape.M * shape.nt * shape.nO;
+  QD_HIP(hipMemcpyAsync(dst, dev.psi, bytes, hipMemcpyDeviceToHost, s));
+  QD_HIP(hipStreamSynchronize(s));
+}
+
+void DeviceJob::flux_to_crp(const qd_inputs *, double *) const {}
+
+// ---------------------------------------------------------------------------
+// Engine
+// ---------------------------------------------------------------------------
+Engine::Engine(hipStream_t s) : stream_(s) {}
+
+Engine::~Engine() {
+  drop_graph();
+  if (d_jobs_) (void)hipFree(d_jobs_);
+  if (d_bases_) (void)hipFree(d_bases_);
+}
+
+void Engine::drop_graph() {
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  graph_exec_ = nullptr;
+  graph_ = nullptr;
+}
+
+void Engine::set_jobs(const std::vector<DeviceJob *> &jobs) {
+  drop_graph();
+  jobs_ = jobs;
+  const int n = (int)jobs.size();
+  if (n > n_alloc_jobs_) {
+    if (d_jobs_) QD_HIP(hipFree(d_jobs_));
+    if (d_bases_) QD_HIP(hipFree(d_bases_));
+    d_jobs_ = nullptr;
+    d_bases_ = nullptr;
+    const int cap = std::max(n, 8);
+    QD_HIP(hipMalloc(&d_jobs_, sizeof(JobDev) * cap));
+    QD_HIP(hipMalloc(&d_bases_, sizeof(int) * KK_COUNT * (cap + 1)));
+    n_alloc_jobs_ = cap;
+  }
+  if (n == 0) return;
+  std::vector<JobDev> descs(n);
+  std::vector<int> bases((size_t)KK_COUNT * (n + 1), 0);
+  lds_fftR_ = lds_fftr_ = 0;
+  fwd_tiles_ = bwd_tiles_ = fwd_kp_ = bwd_kp_ = 0;
+  nb_max_ = 1;
+  mixed_ = jobs[0]->dev.mixed;
+  any_flux_ = 0;
+  for (int q = 0; q < n; q++) {
+    const JobDev &d = jobs[q]->dev;
+    QD_REQUIRE(d.mixed == mixed_, "all jobs of one engine must share a precision");
+    descs[q] = d;
+    for (int k = 0; k < KK_COUNT; k++) bases[(size_t)k * (n + 1) + q + 1] = bases[(size_t)k * (n + 1) + q] + d.units[k];
+    lds_fftR_ = std::max(lds_fftR_, (size_t)(2 * d.fft_rows * d.n1 + d.n1) * 16);
+    lds_fftr_ = std::max(lds_fftr_, (size_t)(2 * d.fft_cols * d.n2 + d.n2) * 16);
+    fwd_tiles_ = std::max(fwd_tiles_, d.fwd_np / 16);
+    bwd_tiles_ = std::max(bwd_tiles_, d.bwd_np / 16);
+    fwd_kp_ = std::max(fwd_kp_, d.fwd_kp);
+    bwd_kp_ = std::max(bwd_kp_, d.bwd_kp);
+    nb_max_ = std::max(nb_max_, d.nO);
+    any_flux_ |= d.flux_on;
+  }
+  for (int k = 0; k < KK_COUNT; k++) total_[k] = bases[(size_t)k * (n + 1) + n];
+  QD_HIP(hipMemcpyAsync(d_jobs_, descs.data(), sizeof(JobDev) * n, hipMemcpyHostToDevice, stream_));
+  QD_HIP(hipMemcpyAsync(d_bases_, bases.data(), sizeof(int) * bases.size(), hipMemcpyHostToDevice, stream_));
+  QD_HIP(hipStreamSynchronize(stream_));
+}
+
+StepArgs Engine::args(int kind) const {
+  StepArgs a;
+  a.jobs = d_jobs_;
+  a.unit_base = d_bases_ + (size_t)kind * (jobs_.size() + 1);
+  a.n_jobs = (int)jobs_.size();
+  return a;
+}
+
+void Engine::enqueue(hipEvent_t *ev) {
+  hipStream_t s = stream_;
+  auto mark = [&](int q) {
+    if (ev) QD_HIP(hipEventRecord(ev[q], s));
+  };
+  mark(0);
+  launch_dvr(args(KK_DVR), total_[KK_DVR], s);
+  mark(1);
+  launch_leg_fwd(args(KK_LEG_FWD), total_[KK_LEG_FWD], fwd_tiles_, fwd_kp_, mixed_, s);
+  mark(2);
+  launch_fbr(args(KK_FBR), total_[KK_FBR], 0, nb_max_, s);
+  mark(3);
+  launch_fft_R(args(KK_FFT_R), total_[KK_FFT_R], 0, lds_fftR_, s);
+  mark(4);
+  launch_fft_r(args(KK_FFT_r), total_[KK_FFT_r], lds_fftr_, s);
+  mark(5);
+  launch_fft_R(args(KK_FFT_R), total_[KK_FFT_R], 1, lds_fftR_, s);
+  mark(6);
+  launch_fbr(args(KK_FBR), total_[KK_FBR], 1, nb_max_, s);
+  mark(7);
+  launch_leg_bwd(args(KK_LEG_BWD), total_[KK_LEG_BWD], bwd_tiles_, bwd_kp_, mixed_, s);
+  mark(8);
+  launch_obs(args(KK_OBS), total_[KK_OBS], s);
+  if (any_flux_) launch_flux(args(KK_FLUX), total_[KK_FLUX], s);
+  mark(9);
+  launch_reduce(args(KK_REDUCE), (int)jobs_.size(), s);
+  mark(10);
+  QD_HIP(hipGetLastError());
+}
+
+void Engine::step(hipEvent_t *ev) {
+  if (jobs_.empty()) return;
+  enqueue(ev);
+}
+
+void Engine::step_graph() {
+  if (jobs_.empty()) return;
+  if (!graph_exec_) {
+    QD_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue(nullptr);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(stream_, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    QD_HIP(hipStreamEndCapture(stream_, &graph_));
+    QD_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+  }
+  QD_HIP(hipGraphLaunch(graph_exec_, stream_));
+}
+
+}  // namespace qd

@@ -1,0 +1,326 @@
+// qd_abi.cpp - the extern "C" entry points of include/qd.h.
+//
+// qd_evolve is the drop-in for mexFunction (cudaOpenMPEvolution.C:15-79):
+// it validates the input struct, makes every omega block resident on the
+// current device, runs time.total_steps steps with the reference's step
+// sequence (cudaOpenMP2.cu:338-387), writes the step counter, the wavepackets
+// and the observables back into caller memory, and frees the device.
+// No C++ exception or HIP error crosses this boundary: every failure becomes a
+// qd_status with a message.
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+
+#include "host_internal.h"
+#include "qd_test.h"
+
+using namespace qd;
+
+struct qd_handle {
+  hipStream_t stream = nullptr;
+  std::unique_ptr<DeviceJob> job;
+  std::unique_ptr<Engine> engine;
+  const qd_inputs *in = nullptr;   // borrowed, must outlive the handle
+  std::string err;
+  int records_enqueued = 0;
+  int last_checked = 0;
+  bool use_graph = true;
+};
+
+static thread_local std::string g_last_error;
+
+static qd_status fail(qd_handle *h, qd_status code, const std::string &msg) {
+  if (h) h->err = msg;
+  g_last_error = msg;
+  return code;
+}
+
+#define QD_GUARD(h, ...)                                                 \
+  try {                                                                  \
+    __VA_ARGS__                                                          \
+  } catch (const Error &e) {                                             \
+    return fail((h), e.code, e.what());                                  \
+  } catch (const std::bad_alloc &) {                                     \
+    return fail((h), QD_ERR_NOMEM, "host allocation failed");            \
+  } catch (const std::exception &e) {                                    \
+    return fail((h), QD_ERR_DEVICE, e.what());                           \
+  }
+
+namespace {
+
+void print_records(qd_handle *h, int from, int to) {
+  const int nO = h->job->shape.nO;
+  const int *om = h->in->omega_states.omegas;
+  for (int r = from; r < to; r++) {
+    const double *o = h->job->obs.data() + (size_t)r * nO * 5;
+    double tn = 0.0, te = 0.0;
+    std::printf("\n Step: %d\n", h->job->shape.steps0 + r);
+    for (int b = 0; b < nO; b++) {   // wavepackets1device.cu:443-451
+      const double *x = o + b * 5;
+      const double e = x[1] + x[2] + x[3] + x[4];
+      std::printf(" %d %.12f %.12f %.12f %.12f %.12f %.12f\n", om[b], x[0], x[1], x[2], x[3], x[4], e);
+      tn += x[0];
+      te += e;
+    }
+    std::printf(" Total: %.12f %.12f\n", tn, te);   // cudaOpenMP2.cu:442
+  }
+  std::fflush(stdout);
+}
+
+// drain the ring, check the norm of each new record, publish the step counter
+qd_status sync_and_check(qd_handle *h) {
+  QD_HIP(hipStreamSynchronize(h->stream));
+  DeviceJob &j = *h->job;
+  j.drain(h->records_enqueued);
+  const int nO = j.shape.nO;
+  for (int r = h->last_checked; r < j.records_done; r++) {
+    double tn = 0.0;
+    for (int b = 0; b < nO; b++) tn += j.obs[(size_t)r * nO * 5 + b * 5];
+    if (!(tn <= 1.0 + 1e-10)) {
+      if (h->in->print_energies) print_records(h, h->last_checked, r + 1);
+      h->last_checked = r + 1;
+      *h->in->time.steps = j.shape.steps0 + r + 1;
+      char buf[160];
+      std::snprintf(buf, sizeof buf, "total norm %.17g exceeds 1 at step %d (cudaOpenMP2.cu:437-440)", tn,
+                    j.shape.steps0 + r);
+      return fail(h, QD_ERR_NORM, buf);
+    }
+  }
+  if (h->in->print_energies) print_records(h, h->last_checked, j.records_done);
+  h->last_checked = j.records_done;
+  *h->in->time.steps = j.steps;
+  return QD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *qd_version(void) { return "qdamd 0.1 (gfx950)"; }
+
+const char *qd_last_global_error(void) { return g_last_error.c_str(); }
+
+const char *qd_last_error(const qd_handle *h) { return h ? h->err.c_str() : g_last_error.c_str(); }
+
+qd_status qd_create(const qd_inputs *in, qd_handle **out) {
+  if (!out) return fail(nullptr, QD_ERR_INVALID, "out pointer is NULL");
+  *out = nullptr;
+  std::unique_ptr<qd_handle> h(new qd_handle());
+  QD_GUARD(nullptr, {
+    const JobShape shape = validate_inputs(in);   // before any device work
+    init_kernel_attributes();
+    QD_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    h->in = in;
+    h->job.reset(new DeviceJob(in, shape, h->stream));
+    h->engine.reset(new Engine(h->stream));
+    h->engine->set_jobs({h->job.get()});
+    const char *g = std::getenv("QD_NO_GRAPH");
+    h->use_graph = !(g && g[0] == '1');
+    *out = h.release();
+    return QD_OK;
+  })
+}
+
+void qd_destroy(qd_handle *h) {
+  if (!h) return;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  h->engine.reset();
+  h->job.reset();
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+qd_status qd_run(qd_handle *h, int32_t n_steps) {
+  if (!h) return fail(nullptr, QD_ERR_INVALID, "handle is NULL");
+  if (n_steps < 0) return fail(h, QD_ERR_INVALID, "n_steps must be >= 0");
+  QD_GUARD(h, {
+    DeviceJob &j = *h->job;
+    const qd_options &op = h->in->options;
+    const bool copy = (op.wave_to_matlab || op.wave_callback) && op.steps_to_copy_psi_from_device_to_host > 0;
+    for (int s = 0; s < n_steps; s++) {
+      const int steps_before = j.steps;
+      // the first step (steps == 0) carries the V(-dt/2) phase: run it eagerly,
+      // every later step replays the captured graph
+      if (h->use_graph && steps_before != 0) h->engine->step_graph();
+      else h->engine->step();
+      h->records_enqueued++;
+      j.steps++;
+      const bool ring_full = h->records_enqueued - j.records_done >= j.dev.ring;
+      const bool copy_now = copy && steps_before % op.steps_to_copy_psi_from_device_to_host == 0;
+      if (ring_full || copy_now || s == n_steps - 1) {
+        qd_status st = sync_and_check(h);
+        if (st != QD_OK) return st;
+      }
+      if (copy_now) {   // cudaOpenMP2.cu:380-384, before steps++ (:386)
+        j.copy_psi(h->in->omega_states.wave_packets, h->stream);
+        *h->in->time.steps = steps_before;
+        if (op.wave_callback) op.wave_callback(op.callback_user, steps_before);
+        *h->in->time.steps = j.steps;
+      }
+    }
+    return QD_OK;
+  })
+}
+
+qd_status qd_synchronize(qd_handle *h) {
+  if (!h) return fail(nullptr, QD_ERR_INVALID, "handle is NULL");
+  QD_GUARD(h, { return sync_and_check(h); })
+}
+
+int32_t qd_steps_done(const qd_handle *h) { return h ? h->records_enqueued : 0; }
+
+qd_status qd_copy_wavepackets(qd_handle *h, double *dst) {
+  if (!h || !dst) return fail(h, QD_ERR_INVALID, "NULL argument");
+  QD_GUARD(h, {
+    h->job->copy_psi(dst, h->stream);
+    return QD_OK;
+  })
+}
+
+qd_status qd_get_observables(qd_handle *h, int32_t first, int32_t count, double *dst) {
+  if (!h || !dst) return fail(h, QD_ERR_INVALID, "NULL argument");
+  QD_GUARD(h, {
+    qd_status st = sync_and_check(h);
+    if (st != QD_OK) return st;
+    DeviceJob &j = *h->job;
+    if (first < 0 || count < 0 || first + count > j.records_done)
+      return fail(h, QD_ERR_INVALID, "observable records out of range");
+    const size_t rec = (size_t)j.shape.nO * 5;
+    std::memcpy(dst, j.obs.data() + (size_t)first * rec, sizeof(double) * rec * count);
+    return QD_OK;
+  })
+}
+
+qd_status qd_get_autocorrelation(qd_handle *h, int32_t first, int32_t count, double *dst) {
+  if (!h || !dst) return fail(h, QD_ERR_INVALID, "NULL argument");
+  QD_GUARD(h, {
+    qd_status st = sync_and_check(h);
+    if (st != QD_OK) return st;
+    DeviceJob &j = *h->job;
+    if (first < 0 || count < 0 || first + count > j.records_done)
+      return fail(h, QD_ERR_INVALID, "autocorrelation records out of range");
+    std::memcpy(dst, j.acf.data() + (size_t)first * 2, sizeof(double) * 2 * count);
+    return QD_OK;
+  })
+}
+
+qd_status qd_get_crp(qd_handle *h, double *crp) {
+  if (!h || !crp) return fail(h, QD_ERR_INVALID, "NULL argument");
+  QD_GUARD(h, {
+    if (!h->job->shape.crp) return fail(h, QD_ERR_STATE, "CRP was not requested (calculate_CRP == 0)");
+    QD_HIP(hipStreamSynchronize(h->stream));
+    h->job->flux_to_crp(h->in, crp);
+    return QD_OK;
+  })
+}
+
+qd_status qd_profile_steps(qd_handle *h, int32_t n_steps, double *us) {
+  if (!h || !us || n_steps <= 0) return fail(h, QD_ERR_INVALID, "bad arguments");
+  QD_GUARD(h, {
+    hipEvent_t ev[QD_KERNEL_KINDS + 1];
+    for (auto &e : ev) QD_HIP(hipEventCreate(&e));
+    double acc[QD_KERNEL_KINDS] = {};
+    qd_status st = QD_OK;
+    for (int s = 0; s < n_steps && st == QD_OK; s++) {
+      h->engine->step(ev);
+      h->records_enqueued++;
+      h->job->steps++;
+      QD_HIP(hipEventSynchronize(ev[QD_KERNEL_KINDS]));
+      for (int k = 0; k < QD_KERNEL_KINDS; k++) {
+        float ms = 0.f;
+        QD_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+        acc[k] += ms * 1000.0;
+      }
+      if (h->records_enqueued - h->job->records_done >= h->job->dev.ring) st = sync_and_check(h);
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    if (st == QD_OK) st = sync_and_check(h);
+    // FFT_R appears twice and FBR twice per step; report per launch
+    for (int k = 0; k < QD_KERNEL_KINDS; k++) us[k] = acc[k] / n_steps;
+    return st;
+  })
+}
+
+qd_status qd_evolve(qd_inputs *in) {
+  qd_handle *h = nullptr;
+  qd_status st = qd_create(in, &h);
+  if (st != QD_OK) return st;
+  const int total = in->time.total_steps;
+  st = qd_run(h, total);
+  if (st == QD_OK && in->observables) st = qd_get_observables(h, 0, total, in->observables);
+  if (st == QD_OK && in->autocorrelation) st = qd_get_autocorrelation(h, 0, total, in->autocorrelation);
+  if (st == QD_OK && in->final_copy) st = qd_copy_wavepackets(h, in->omega_states.wave_packets);
+  if (st == QD_OK && h->job->shape.crp) st = qd_get_crp(h, in->crp.CRP);
+  if (st != QD_OK) g_last_error = h->err;
+  qd_destroy(h);
+  return st;
+}
+
+// ---------------------------------------------------------------------------
+// probes (include/qd_test.h)
+// ---------------------------------------------------------------------------
+qd_status qd_probe_gemm(const double *A, int32_t rows, int32_t k, const double *B, int32_t n, double *C) {
+  QD_GUARD(nullptr, {
+    init_kernel_attributes();
+    QD_REQUIRE(A && B && C && rows > 0 && k > 0 && n > 0 && n <= 128, "bad probe arguments");
+    const int kp = (k + 3) / 4 * 4, np = (n + 15) / 16 * 16;
+    std::vector<double> Bp((size_t)kp * np, 0.0);
+    for (int kk = 0; kk < k; kk++)
+      for (int c = 0; c < n; c++) Bp[(size_t)kk * np + c] = B[(size_t)c * k + kk];   // B is (k x n) col-major
+    double *dA, *dB, *dC;
+    QD_HIP(hipMalloc(&dA, sizeof(double) * rows * k));
+    QD_HIP(hipMalloc(&dB, sizeof(double) * Bp.size()));
+    QD_HIP(hipMalloc(&dC, sizeof(double) * rows * n));
+    QD_HIP(hipMemcpy(dA, A, sizeof(double) * rows * k, hipMemcpyHostToDevice));
+    QD_HIP(hipMemcpy(dB, Bp.data(), sizeof(double) * Bp.size(), hipMemcpyHostToDevice));
+    launch_gemm_probe(dA, rows, dB, kp, np, dC, rows, rows, k, n, nullptr);
+    QD_HIP(hipGetLastError());
+    QD_HIP(hipMemcpy(C, dC, sizeof(double) * rows * n, hipMemcpyDeviceToHost));
+    QD_HIP(hipFree(dA));
+    QD_HIP(hipFree(dB));
+    QD_HIP(hipFree(dC));
+    return QD_OK;
+  })
+}
+
+qd_status qd_probe_fft(double *data, int32_t n, int32_t count, int32_t inverse) {
+  QD_GUARD(nullptr, {
+    init_kernel_attributes();
+    QD_REQUIRE(data && n >= 2 && n <= kFftMaxN && count > 0, "bad probe arguments");
+    int m = n;
+    for (int p : {2, 3, 5})
+      while (m % p == 0) m /= p;
+    QD_REQUIRE(m == 1, "n must factor into 2, 3, 5");
+    std::vector<double2> tw(n);
+    for (int e = 0; e < n; e++) {
+      const long double a = -2.0L * 3.141592653589793238462643383279502884L * e / n;
+      tw[e] = make_double2((double)cosl(a), (double)sinl(a));
+    }
+    double2 *dd, *dt;
+    QD_HIP(hipMalloc(&dd, sizeof(double2) * n * count));
+    QD_HIP(hipMalloc(&dt, sizeof(double2) * n));
+    QD_HIP(hipMemcpy(dd, data, sizeof(double2) * n * count, hipMemcpyHostToDevice));
+    QD_HIP(hipMemcpy(dt, tw.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
+    launch_fft_probe(dd, dt, n, count, inverse, nullptr);
+    QD_HIP(hipGetLastError());
+    QD_HIP(hipMemcpy(data, dd, sizeof(double2) * n * count, hipMemcpyDeviceToHost));
+    QD_HIP(hipFree(dd));
+    QD_HIP(hipFree(dt));
+    return QD_OK;
+  })
+}
+
+qd_status qd_probe_coriolis(int32_t J, int32_t l, int32_t omega_min, int32_t n, double *evals, double *evecs) {
+  QD_GUARD(nullptr, {
+    QD_REQUIRE(evals && evecs && n >= 1 && n <= 64, "bad probe arguments");
+    std::vector<double> diag(n), off(std::max(1, n - 1));
+    coriolis_tridiagonal(J, l, omega_min, n, diag.data(), off.data());
+    tridiag_eigen(n, diag.data(), off.data(), evecs);
+    for (int a = 0; a < n; a++) evals[a] = diag[a];
+    return QD_OK;
+  })
+}
+
+}  // extern "C"

@@ -1,0 +1,125 @@
+// qd_internal.h - device-side job descriptor and kernel entry points shared by
+// the HIP translation units of libqdamd.
+//
+// One JobDev describes one partial-wave job (all of its omega blocks).  Every
+// step kernel takes an array of JobDev plus a per-kernel prefix table that
+// maps blockIdx.x to (job, local work unit), so a single launch can advance
+// one job (qd_run) or every admitted job of the server at once.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace qd {
+
+constexpr int kMaxOmega = 24;     // omega blocks per job (K_max <= 23)
+constexpr int kFftMaxN = 512;     // longest FFT pencil
+constexpr int kBlock = 256;       // threads per workgroup for the step kernels
+
+// per-kernel work-unit counts (index into JobDev::units / LaunchTables)
+enum KernelKind {
+  KK_DVR = 0,
+  KK_LEG_FWD,
+  KK_FBR,
+  KK_FFT_R,
+  KK_FFT_r,
+  KK_LEG_BWD,
+  KK_OBS,
+  KK_REDUCE,
+  KK_FLUX,
+  KK_COUNT
+};
+
+struct JobDev {
+  // shape
+  int n1, n2, nt, M, L, lmax, nO, omin, kmax;
+  int J;
+  int mixed;                 // 1: mixed-precision Legendre GEMMs
+  // work decomposition
+  int gemm_strips;           // 16-row strips per wave in the Legendre GEMMs
+  int gemm_fwd_groups;       // workgroups per omega block (forward GEMM)
+  int gemm_bwd_groups;       // workgroups per omega block (backward GEMM)
+  int fwd_kp, fwd_np;        // padded K (mult of 4) and N (mult of 16), forward
+  int bwd_kp, bwd_np;        // padded K and N, backward
+  int fft_rows;              // pencils per workgroup along R
+  int fft_cols;              // pencils per workgroup along r
+  int obs_chunks;            // workgroups per (block, theta slice) in the DVR observables
+  int fbr_chunks;            // workgroups per slab in the FBR pass
+  int units[KK_COUNT];       // work units (workgroups) per kernel for this job
+  // physics
+  double dt, dr1, dr2;
+  // state
+  double2 *psi;              // [nO][nt][M] DVR wavepackets
+  double2 *fbr;              // [nO][L][M]  FBR wavepackets (slabs l < omega unused)
+  double2 *psi0;             // [nO][nt][M] initial state (autocorrelation), may be null
+  // tables
+  const double2 *vphase;     // [nt][M]  d1[i] d2[j] exp(-i dt V)
+  const double *pot;         // [nt][M]
+  const double *w;           // [nt]
+  const double *wP;          // [nO][fwd_kp][fwd_np] row-major (c fastest): w_k P_l(x_k)
+  const double *Pt;          // [nO][bwd_kp][bwd_np] row-major: P_l(x_k), k = l - omega
+  const float *wP_hi, *wP_lo;// split FP32 copies for the mixed form
+  const float *Pt_hi, *Pt_lo;
+  const double2 *corF;       // [L][n1][nO][nO] rotational(dt/2 on R) x Coriolis exponential
+  const double2 *rot2;       // [L][n2] exp(-i dt/2 l(l+1) / (2 m2 r_j^2))
+  const double2 *kin1;       // [n1] exp(-i dt k_i^2 / (2 m1))
+  const double2 *kin2;       // [n2] exp(-i dt k_j^2 / (2 m2))
+  const double *kinE1;       // [n1] k_i^2/(2 m1)
+  const double *kinE2;       // [n2]
+  const double *I1;          // [n1] 1/(2 m1 R_i^2)
+  const double *I2;          // [n2] 1/(2 m2 r_j^2)
+  const double2 *cmat;       // [L][nO] (C_l(omega,omega), C_l(omega,omega+1)) exact entries
+  const double2 *twR;        // [n1] exp(-2 pi i e / n1)
+  const double2 *twr;        // [n2] exp(-2 pi i e / n2)
+  // reduction scratch (per-workgroup partial sums)
+  double *part_fbr;          // [units FBR][nO][2]  (Coriolis energy, rotational energy)
+  double *part_ke;           // [units FFT_r]       kinetic energy of one (block, slab, column group)
+  double *part_obs;          // [nO][nt][obs_chunks][4] (norm, potential, acf re, acf im)
+  double *obs_ring;          // [ring][nO][5] per-step observables
+  double2 *acf_ring;         // [ring] autocorrelation
+  int ring;                  // ring capacity (records)
+  int *counters;             // device: [0] step counter (the reference's time.steps),
+                             //         [1] records written; advanced by the reduce kernel
+  // flux
+  int flux_on, flux_ns, flux_ng, flux_ne;
+  const double *flux_coef;   // [ng] derivative weights
+  const double *flux_E;      // [ne]
+  double2 *flux_A;           // [ne][nO][nt][n1]
+  double2 *flux_B;           // [ne][nO][nt][n1]
+};
+
+// Per-launch arguments common to all step kernels.
+struct StepArgs {
+  const JobDev *jobs;
+  const int *unit_base;      // [n_jobs + 1] prefix of units for this kernel kind
+  int n_jobs;
+};
+
+// Locate the job owning workgroup `u` (binary search over the prefix table).
+__device__ __forceinline__ int locate_job(const int *base, int n_jobs, int u) {
+  int lo = 0, hi = n_jobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (base[mid] <= u) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Host-side launchers (each enqueues one kernel on `stream`).
+void launch_dvr(const StepArgs &a, int total_units, hipStream_t stream);
+void launch_leg_fwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, hipStream_t stream);
+void launch_leg_bwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, hipStream_t stream);
+void launch_fbr(const StepArgs &a, int total_units, int with_energy, int nb_max, hipStream_t stream);
+void launch_fft_R(const StepArgs &a, int total_units, int inverse, size_t lds, hipStream_t stream);
+void launch_fft_r(const StepArgs &a, int total_units, size_t lds, hipStream_t stream);
+// standalone batched 1-D FFT of `count` contiguous pencils (test probe)
+void launch_fft_probe(double2 *data, const double2 *tw, int n, int count, int inverse, hipStream_t stream);
+void launch_obs(const StepArgs &a, int total_units, hipStream_t stream);
+void launch_reduce(const StepArgs &a, int n_jobs, hipStream_t stream);
+void launch_flux(const StepArgs &a, int total_units, hipStream_t stream);
+
+// GEMM probe for tests: C (rows x n) = A (rows x k, lda) * B ([kp][np] row-major)
+void launch_gemm_probe(const double *A, int lda, const double *B, int kp, int np, double *C, int ldc,
+                       int rows, int k, int n, hipStream_t stream);
+
+}  // namespace qd

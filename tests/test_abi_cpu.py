@@ -1,0 +1,129 @@
+"""C-ABI checks that need no GPU: the library loads, exports every symbol
+declared in include/*.h, rejects invalid inputs before touching the device
+(the reference's insist() conditions, SURVEY of cudaMath.h:119,
+omegawavepacket.cu:46/:142, CoriolisMatrix.C:65, cudaOpenMP2.cu:437-440),
+and its host-side Coriolis eigensolver agrees with the oracle's."""
+import ctypes as C
+import dataclasses
+import os
+import re
+
+import numpy as np
+import pytest
+
+from qd_openmp_cuda_amd import abi, api, inputs
+from oracle import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = []
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if not h.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\*?\s*(qd_[a-z0-9_]+)\s*\(", text, flags=re.M):
+            names.append(m.group(1))
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    L = api.lib()
+    names = declared_functions()
+    assert "qd_evolve" in names and "qd_create" in names and "qd_probe_gemm" in names
+    for n in names:
+        assert hasattr(L, n), n
+    assert b"gfx950" in L.qd_version()
+
+
+def test_struct_layout_matches_header():
+    # field offsets as a C compiler lays out include/qd.h
+    assert C.sizeof(abi.qd_radial) == 32
+    assert abi.qd_inputs.r2.offset == 32
+    assert abi.qd_inputs.theta.offset == 64
+
+
+def small_job(**kw):
+    base = dict(n1=32, n2=24, ntheta=12, R_range=(2.0, 10.0), r_range=(0.6, 4.0), E_coll=0.01)
+    base.update(kw)
+    return inputs.build_job(inputs.JobSpec(**base))
+
+
+def expect_invalid(inp, pattern):
+    with pytest.raises(api.QDError) as ei:
+        api.evolve(inp)
+    assert ei.value.code == abi.QD_ERR_INVALID, str(ei.value)
+    assert re.search(pattern, str(ei.value)), str(ei.value)
+
+
+def test_rejects_odd_grid():
+    job = small_job()
+    inp = abi.Inputs(job, total_steps=1)
+    inp.struct.r1.n = 31
+    expect_invalid(inp, "even")
+
+
+def test_rejects_theta_not_exceeding_lmax():
+    job = small_job()
+    inp = abi.Inputs(job, total_steps=1)
+    inp.struct.omega_states.l_max = 11
+    expect_invalid(inp, "l_max")
+
+
+def test_rejects_nonconsecutive_omegas():
+    job = small_job(J=3, parity=1, k_max=-1, omega0=1, j0=1)
+    inp = abi.Inputs(job, total_steps=1)
+    inp.omegas[1] = 5
+    expect_invalid(inp, "consecutive")
+
+
+def test_rejects_parity_for_j0():
+    job = small_job()
+    inp = abi.Inputs(job, total_steps=1)
+    inp.struct.omega_states.parity = 1
+    expect_invalid(inp, "parity")
+
+
+def test_rejects_incomplete_omega_list():
+    job = small_job(J=3, parity=1, k_max=2, omega0=1, j0=1)      # omegas 1..2
+    inp = abi.Inputs(job, total_steps=1)
+    inp.struct.k_max = -1                                         # claims untruncated J=3
+    expect_invalid(inp, "min\\(J, l_max, k_max\\)")
+
+
+def test_rejects_norm_above_one():
+    job = small_job()
+    job["omega_states"] = dict(job["omega_states"], wave_packets=job["omega_states"]["wave_packets"] * 1.001)
+    expect_invalid(abi.Inputs(job, total_steps=1), "norm")
+
+
+def test_rejects_bad_flux_surface():
+    job = small_job(n_energies=4, flux_r=0.7)
+    expect_invalid(abi.Inputs(job, total_steps=1), "stencil")
+
+
+def test_rejects_missing_copy_interval():
+    job = small_job()
+    inp = abi.Inputs(job, total_steps=1, callback=lambda u, s: None, copy_every=0)
+    expect_invalid(inp, "steps_to_copy")
+
+
+def test_null_inputs():
+    L = api.lib()
+    assert L.qd_evolve(None) == abi.QD_ERR_INVALID
+    h = C.c_void_p()
+    assert L.qd_create(None, C.byref(h)) == abi.QD_ERR_INVALID
+    assert L.qd_run(None, 1) == abi.QD_ERR_INVALID
+
+
+@pytest.mark.parametrize("J,l,om,n", [(1, 1, 0, 2), (20, 30, 0, 5), (8, 40, 1, 8), (40, 50, 1, 8), (23, 23, 0, 24)])
+def test_host_eigensolver_matches_oracle(J, l, om, n):
+    e, v = api.probe_coriolis(J, l, om, n)
+    c = oracle.coriolis_dense(J, l, om, n)
+    eo, _ = oracle.sym_eig(c)
+    scale = max(1.0, np.abs(c).max())
+    assert np.allclose(np.sort(e), np.sort(eo), atol=1e-11 * scale)
+    assert np.allclose(v @ np.diag(e) @ v.T, c, atol=1e-11 * scale)
+    assert np.allclose(v.T @ v, np.eye(n), atol=1e-13)

@@ -1,0 +1,157 @@
+"""Parity of the HIP propagator (libqdamd.so, through the C ABI) with the FP64
+CPU oracle on the same seeded inputs.
+
+Bars (FP64 form): wavepacket within 1e-10 relative (L2 over all blocks) after
+every checked step count, per-step observables (norm, potential, kinetic,
+rotational, Coriolis energy) within 1e-9 relative to the total energy scale,
+autocorrelation within 1e-10 absolute.
+"""
+import dataclasses
+import sys
+
+import numpy as np
+import pytest
+
+from qd_openmp_cuda_amd import abi, api, inputs
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+PSI_TOL = 1e-10
+OBS_TOL = 1e-9
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def run_both(spec, steps, precision=abi.QD_PREC_FP64, start_steps=0):
+    job = inputs.build_job(spec)
+    a = abi.Inputs(job, total_steps=steps, steps=start_steps)
+    b = abi.Inputs(job, total_steps=steps, steps=start_steps, precision=precision)
+    _say(f"case {spec.n1}x{spec.n2}x{spec.ntheta} J={spec.J} p={spec.parity} k_max={spec.k_max} "
+         f"steps={steps} prec={precision}")
+    oracle.evolve(a, n_threads=8)
+    api.evolve(b)
+    return a, b
+
+
+def rel_l2(x, y):
+    return float(np.linalg.norm((x - y).ravel()) / np.linalg.norm(y.ravel()))
+
+
+def check_parity(a, b, psi_tol=PSI_TOL, obs_tol=OBS_TOL):
+    e_psi = rel_l2(b.psi, a.psi)
+    scale = max(1.0, float(np.abs(a.obs).max()))
+    e_obs = float(np.abs(b.obs - a.obs).max()) / scale
+    e_acf = float(np.abs(b.acf - a.acf).max())
+    _say(f"  psi rel {e_psi:.3e}  obs {e_obs:.3e}  acf {e_acf:.3e}")
+    assert b.steps[0] == a.steps[0]
+    assert e_psi <= psi_tol
+    assert e_obs <= obs_tol
+    assert e_acf <= psi_tol
+
+
+def test_gemm_probe_matches_numpy():
+    rng = np.random.default_rng(1)
+    for rows, k, n in [(64, 4, 16), (96, 12, 20), (1024, 48, 47), (640, 63, 64), (2048, 96, 95), (100, 7, 3)]:
+        A = rng.standard_normal((rows, k))
+        B = rng.standard_normal((k, n))
+        Cg = api.probe_gemm(A, B)
+        Cr = A @ B
+        err = np.abs(Cg - Cr).max() / np.abs(Cr).max()
+        _say(f"gemm {rows}x{k}x{n}: {err:.2e}")
+        assert err < 1e-14
+
+
+@pytest.mark.parametrize("n", [24, 48, 64, 96, 128, 160, 192, 256, 384, 512])
+def test_fft_probe_matches_numpy(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((3, n)) + 1j * rng.standard_normal((3, n))
+    f = api.probe_fft(x, inverse=False)
+    g = api.probe_fft(x, inverse=True)
+    ef = np.abs(f - np.fft.fft(x, axis=1)).max() / np.abs(np.fft.fft(x, axis=1)).max()
+    eg = np.abs(g - n * np.fft.ifft(x, axis=1)).max() / np.abs(np.fft.fft(x, axis=1)).max()
+    _say(f"fft n={n}: fwd {ef:.2e} inv {eg:.2e}")
+    assert ef < 1e-14 and eg < 1e-14
+
+
+def test_parity_j0_small():
+    spec = inputs.JobSpec(n1=64, n2=32, ntheta=24, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
+                          E_coll=0.02, absorber=True, absorber_width=1.5, absorber_strength=0.1)
+    a, b = run_both(spec, 6)
+    check_parity(a, b)
+
+
+def test_parity_coriolis_truncated():
+    spec = inputs.JobSpec(n1=64, n2=32, ntheta=24, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
+                          J=5, parity=1, k_max=3, omega0=1, j0=2, E_coll=0.02)
+    a, b = run_both(spec, 5)
+    assert len(a.omegas) == 3
+    check_parity(a, b)
+
+
+def test_parity_coriolis_untruncated_even():
+    spec = inputs.JobSpec(n1=48, n2=32, ntheta=20, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
+                          J=4, parity=0, k_max=-1, omega0=0, j0=1, E_coll=0.02)
+    a, b = run_both(spec, 5)
+    assert list(a.omegas) == [0, 1, 2, 3, 4]
+    check_parity(a, b)
+
+
+def test_parity_c1_shape():
+    spec = inputs.config_spec("c1")
+    a, b = run_both(spec, 4)
+    check_parity(a, b)
+
+
+def test_parity_c3_blocks_reduced_grid():
+    # the c3 Coriolis structure (J=20, K_max=4, 5 blocks) on a reduced grid
+    spec = dataclasses.replace(inputs.config_spec("c3"), n1=96, n2=64, ntheta=32)
+    a, b = run_both(spec, 3)
+    assert len(a.omegas) == 5
+    check_parity(a, b)
+
+
+def test_resume_equals_single_run():
+    """N steps, then M more from the stored state and step counter == N+M in one run
+    (cudaOpenMP2.cu:344: V(-dt/2) only when steps == 0)."""
+    spec = inputs.JobSpec(n1=64, n2=32, ntheta=24, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
+                          J=3, parity=1, k_max=-1, omega0=1, j0=1, E_coll=0.02)
+    job = inputs.build_job(spec)
+    one = abi.Inputs(job, total_steps=7)
+    api.evolve(one)
+    first = abi.Inputs(job, total_steps=3)
+    api.evolve(first)
+    job2 = dict(job)
+    job2["omega_states"] = dict(job["omega_states"], wave_packets=first.psi.copy())
+    second = abi.Inputs(job2, total_steps=4, steps=int(first.steps[0]))
+    api.evolve(second)
+    assert second.steps[0] == 7
+    e = rel_l2(second.psi, one.psi)
+    _say(f"resume: {e:.3e}")
+    assert e <= 1e-13
+
+
+def test_fp64_bitwise_reproducible():
+    spec = inputs.JobSpec(n1=64, n2=32, ntheta=24, J=2, parity=0, k_max=-1, j0=1, E_coll=0.02,
+                          R_range=(2.0, 12.0), r_range=(0.5, 5.0))
+    job = inputs.build_job(spec)
+    x = abi.Inputs(job, total_steps=4)
+    y = abi.Inputs(job, total_steps=4)
+    api.evolve(x)
+    api.evolve(y)
+    assert np.array_equal(x.psi, y.psi)
+    assert np.array_equal(x.obs, y.obs)
+
+
+def test_mixed_precision_close_to_fp64():
+    spec = inputs.JobSpec(n1=64, n2=32, ntheta=24, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
+                          J=5, parity=1, k_max=3, omega0=1, j0=2, E_coll=0.02)
+    a, b = run_both(spec, 5, precision=abi.QD_PREC_MIXED)
+    e_psi = rel_l2(b.psi, a.psi)
+    e_norm = float(np.abs(b.obs[..., 0].sum(1) - a.obs[..., 0].sum(1)).max())
+    _say(f"  mixed: psi rel {e_psi:.3e} norm {e_norm:.3e}")
+    assert e_psi < 1e-5
+    assert e_norm < 1e-6
