@@ -1,0 +1,201 @@
+"""Benchmark: grid-point-steps/s of the split-operator wavepacket step on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): J = 20,
+K_max = 4 (5 omega blocks), each block 192 R x 128 r x 64 theta, complex128,
+LEPS surface, dt = 10 au.  A step is one full split-operator step over all
+omega blocks with inputs resident in HBM.  value = total DVR grid points
+(sum over blocks of n1 n2 ntheta) x steps / seconds, FP64 form; the mixed
+(FP32 hi/lo Legendre) form is measured on the same workload and reported
+beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-mixed]
+"""
+import argparse
+import dataclasses
+import importlib.util
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "qd-openmp-cuda_amd")
+
+HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFLOPS = 78.6
+FP32_MFMA_PEAK_TFLOPS = 157.3
+
+
+def _load_package():
+    if "qd_openmp_cuda_amd" in sys.modules:
+        return sys.modules["qd_openmp_cuda_amd"]
+    spec = importlib.util.spec_from_file_location(
+        "qd_openmp_cuda_amd", os.path.join(PKG_DIR, "__init__.py"), submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["qd_openmp_cuda_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def kernel_model(job):
+    """Algorithmic bytes and flops per launch of each kernel kind for one step.
+
+    G = n1 n2 ntheta DVR points per block, F_b = n1 n2 (l_max - omega_b + 1)
+    FBR points of block b; every complex value is 16 B."""
+    r1, r2, th, os_ = job["r1"], job["r2"], job["theta"], job["omega_states"]
+    n1, n2, nt = r1["n"], r2["n"], th["n"]
+    M = n1 * n2
+    G = M * nt
+    omegas = [int(o) for o in os_["omegas"]]
+    nO = len(omegas)
+    lmax = os_["l_max"]
+    F = [M * (lmax - o + 1) for o in omegas]
+    Ft = sum(F)
+    gemm_flops = sum(2 * (2 * M) * nt * (lmax - o + 1) for o in omegas)
+    return {
+        "dvr": {"bytes": G * (16 + 32 * nO), "flops": 0, "bound": "hbm"},
+        "leg_fwd": {"bytes": nO * 16 * G + 16 * Ft, "flops": gemm_flops, "bound": "mfma"},
+        "fbr1": {"bytes": 32 * Ft, "flops": 0, "bound": "hbm"},
+        "fft_R_fwd": {"bytes": 32 * Ft, "flops": 0, "bound": "hbm"},
+        "fft_r": {"bytes": 32 * Ft, "flops": 0, "bound": "hbm"},
+        "fft_R_inv": {"bytes": 32 * Ft, "flops": 0, "bound": "hbm"},
+        "fbr2": {"bytes": 32 * Ft, "flops": 0, "bound": "hbm"},
+        "leg_bwd": {"bytes": nO * 16 * G + 16 * Ft, "flops": gemm_flops, "bound": "mfma"},
+        "obs": {"bytes": nO * 32 * G + 8 * G, "flops": 0, "bound": "hbm"},
+        "reduce": {"bytes": 0, "flops": 0, "bound": "hbm"},
+    }, G * nO
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
+    p = os.path.join(ROOT, "profiles", "pmc_traffic_c3.txt")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        parts = line.split()
+        if len(parts) >= 2 and parts[0] == kernel:
+            try:
+                return float(parts[1])
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(job, abi, n_steps=2):
+    """FP64 oracle on the host cores over a bounded sample of the same workload."""
+    if ROOT not in sys.path:
+        sys.path.insert(0, ROOT)
+    from oracle import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    inp = abi.Inputs(job, total_steps=1, want_acf=False)
+    oracle.evolve(inp, threads)           # warm-up step (page-in, thread pool)
+    inp = abi.Inputs(job, total_steps=n_steps, want_acf=False)
+    t0 = time.perf_counter()
+    oracle.evolve(inp, threads)
+    dt = time.perf_counter() - t0
+    pts = job["r1"]["n"] * job["r2"]["n"] * job["theta"]["n"] * len(job["omega_states"]["omegas"])
+    return {"value": pts * n_steps / dt, "unit": "grid-point-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n_steps} steps of the same workload after 1 warm-up step, FP64 oracle "
+                      f"(oracle/oracle.cpp, OpenMP), {dt:.2f} s"}
+
+
+def measure(api, abi, job, precision, steps, warmup):
+    inp = abi.Inputs(job, total_steps=steps + warmup, precision=precision, want_acf=False,
+                     want_observables=False, final_copy=False)
+    with api.Propagator(inp) as p:
+        p.run(warmup)
+        p.synchronize()
+        t0 = time.perf_counter()
+        p.run(steps)
+        p.synchronize()
+        elapsed = time.perf_counter() - t0
+        prof = p.profile(20)
+    return elapsed, prof
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs one partial-wave job set on one GPU; "
+                                   "--gpus N > 1 is not supported (jobs shard across GPUs by job, "
+                                   "see DESIGN.md)"}))
+        sys.exit(2)
+    _load_package()
+    from qd_openmp_cuda_amd import abi, api, inputs
+
+    spec = inputs.config_spec("c3")
+    job = inputs.build_job(spec)
+    model, pts = kernel_model(job)
+    print(f"# workload c3: {spec.n1}x{spec.n2}x{spec.ntheta}, J={spec.J}, K_max={spec.k_max}, "
+          f"{len(job['omega_states']['omegas'])} blocks; steps={args.steps} warmup={args.warmup}",
+          file=sys.stderr, flush=True)
+
+    el64, prof64 = measure(api, abi, job, abi.QD_PREC_FP64, args.steps, args.warmup)
+    value = pts * args.steps / el64
+    print(f"# fp64: {el64 / args.steps * 1e3:.3f} ms/step, {value:.4g} gps/s", file=sys.stderr, flush=True)
+
+    mixed = None
+    if not args.no_mixed:
+        elm, profm = measure(api, abi, job, abi.QD_PREC_MIXED, args.steps, args.warmup)
+        mixed = {"value": pts * args.steps / elm, "ms_per_step": elm / args.steps * 1e3,
+                 "ratio_vs_fp64": el64 / elm, "kernels_us": {k: round(v, 2) for k, v in profm.items()}}
+        print(f"# mixed: {elm / args.steps * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
+
+    # roofline of the dominant kernel (largest time per step)
+    dom = max(prof64, key=lambda k: prof64[k])
+    m = model[dom]
+    us = prof64[dom]
+    if m["bound"] == "mfma":
+        achieved = m["flops"] / (us * 1e-6) / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / FP64_MFMA_PEAK_TFLOPS}
+    else:
+        achieved = m["bytes"] / (us * 1e-6) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS}
+    roof["kernel"] = dom
+    roof["traffic"] = read_traffic(dom)
+    per_kernel = {}
+    for k, us_k in prof64.items():
+        mk = model[k]
+        per_kernel[k] = {"us": round(us_k, 2),
+                         "GB/s": round(mk["bytes"] / (us_k * 1e-6) / 1e9, 1) if us_k > 0 else None,
+                         "TFLOP/s": round(mk["flops"] / (us_k * 1e-6) / 1e12, 2) if mk["flops"] else None}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(job, abi, args.cpu_steps)
+
+    out = {
+        "metric": "grid-point-steps/s, FP64 vs mixed, J=20 K=4; p50/p99 partial-wave latency",
+        "value": value,
+        "unit": "grid-point-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el64 / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "J=20, K_max=4, 5 omega blocks of 192x128x64 (R x r x theta), complex128, "
+                               "LEPS surface, dt=10 au",
+                   "grid_points_per_step": pts, "parallelism": "single GPU, all omega blocks resident"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "mixed": mixed,
+        "kernels": per_kernel,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

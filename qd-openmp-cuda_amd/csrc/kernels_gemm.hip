@@ -61,12 +61,14 @@ __device__ __forceinline__ void strip_f64(const double *__restrict__ A, long lda
       }
     }
   }
+  // v_mfma_f64_16x16x4f64 accumulator layout: item it of lane (lr, lk) holds
+  // D[lk + 4 it][lr], i.e. C[r0 + lr][t*16 + lk + 4 it]
   if (rok) {
 #pragma unroll
     for (int t = 0; t < NT; t++) {
 #pragma unroll
       for (int it = 0; it < 4; it++) {
-        const int c = t * 16 + 4 * lk + it;
+        const int c = t * 16 + lk + 4 * it;
         if (c < N) C[(long)c * ldc + r] = acc[t][it];
       }
     }
@@ -124,6 +126,8 @@ __device__ __forceinline__ void strip_mixed(const double *__restrict__ A, long l
       for (int it = 0; it < 4; it++) acc[t][it] += (double)hh[t][it] + (double)hl[t][it];
     }
   }
+  // v_mfma_f32_16x16x4f32 accumulator layout: item it of lane (lr, lk) holds
+  // D[4 lk + it][lr], i.e. C[r0 + lr][t*16 + 4 lk + it]
   if (rok) {
 #pragma unroll
     for (int t = 0; t < NT; t++) {

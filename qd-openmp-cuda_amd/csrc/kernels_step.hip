@@ -222,7 +222,7 @@ __device__ double block_sum_strided(const double *src, long count, long stride, 
 }
 
 __global__ __launch_bounds__(kBlock) void k_reduce(StepArgs a) {
-  __shared__ double red[8];
+  __shared__ double red[16];   // block_sum<4> needs 4 waves x 4 values
   __shared__ double acc[2];
   const int job = blockIdx.x;   // one unit per job
   const JobDev &jb = a.jobs[job];

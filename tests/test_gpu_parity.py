@@ -88,7 +88,7 @@ def test_parity_coriolis_truncated():
     spec = inputs.JobSpec(n1=64, n2=32, ntheta=24, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
                           J=5, parity=1, k_max=3, omega0=1, j0=2, E_coll=0.02)
     a, b = run_both(spec, 5)
-    assert len(a.omegas) == 3
+    assert list(a.omegas) == [0, 1, 2, 3]        # J + p even: omega_min = 0, cut at k_max = 3
     check_parity(a, b)
 
 
