@@ -28,6 +28,11 @@ qd_status qd_probe_fft(double *data, int32_t n, int32_t count, int32_t inverse);
  * CoriolisMatrix.C:63-99).  evecs[alpha*n + i] = component i of vector alpha. */
 qd_status qd_probe_coriolis(int32_t J, int32_t l, int32_t omega_min, int32_t n, double *evals, double *evecs);
 
+/* Plain device-to-device copy of `bytes` bytes, `repeats` times, timed with
+ * HIP events; calibrates the HBM byte counters (FETCH_SIZE / WRITE_SIZE)
+ * against a kernel of known traffic.  No counterpart in the reference. */
+qd_status qd_probe_copy(int64_t bytes, int32_t repeats, double *us_per_copy);
+
 #ifdef __cplusplus
 }
 #endif

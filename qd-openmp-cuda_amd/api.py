@@ -34,6 +34,7 @@ _DECLS = {
     "qd_probe_gemm": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]),
     "qd_probe_fft": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     "qd_probe_coriolis": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
+    "qd_probe_copy": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
 }
 
 
@@ -162,3 +163,10 @@ def probe_coriolis(J: int, l: int, omega_min: int, n: int):
     v = np.zeros((n, n), order="F")
     _check(lib().qd_probe_coriolis(J, l, omega_min, n, e.ctypes.data, v.ctypes.data))
     return e, v
+
+
+def probe_copy(nbytes: int, repeats: int = 10) -> float:
+    """Microseconds per plain device copy of nbytes (read nbytes + write nbytes)."""
+    us = C.c_double()
+    _check(lib().qd_probe_copy(nbytes, repeats, C.byref(us)))
+    return us.value

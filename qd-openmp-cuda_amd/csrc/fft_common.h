@@ -1,0 +1,162 @@
+// fft_common.h - device helpers shared by the FFT passes: small DFT
+// butterflies, the slab work-unit decoder and the per-point
+// rotational . Coriolis . rotational mix.
+#pragma once
+
+#include "qd_complex.h"
+#include "qd_internal.h"
+
+namespace qd {
+
+// Radix of the next stage when m = n / Ns points remain: 4s first, then 2,
+// then 3s, then 5s.
+__host__ __device__ constexpr int next_radix(int m) {
+  return (m % 4 == 0) ? 4 : (m % 2 == 0) ? 2 : (m % 3 == 0) ? 3 : 5;
+}
+
+__host__ __device__ constexpr int fft_stage_count(int n) {
+  int s = 0;
+  for (int ns = 1; ns < n; ns *= next_radix(n / ns)) s++;
+  return s;
+}
+
+template <bool INV>
+__device__ __forceinline__ double2 rot_i(double2 a) {   // multiply by -i (forward) / +i (inverse)
+  return INV ? cmul_pi(a) : cmul_mi(a);
+}
+
+// In-register DFT of R points, forward exp(-2 pi i qs/R) or inverse.
+template <bool INV, int R>
+__device__ __forceinline__ void dft_small(const double2 (&v)[R], double2 (&X)[R]) {
+  if constexpr (R == 2) {
+    X[0] = cadd(v[0], v[1]);
+    X[1] = csub(v[0], v[1]);
+  } else if constexpr (R == 4) {
+    const double2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+    const double2 a2 = cadd(v[1], v[3]), a3 = rot_i<INV>(csub(v[1], v[3]));
+    X[0] = cadd(a0, a2);
+    X[2] = csub(a0, a2);
+    X[1] = cadd(a1, a3);
+    X[3] = csub(a1, a3);
+  } else if constexpr (R == 3) {
+    const double c3 = -0.5, s3 = 0.86602540378443864676;
+    const double2 s = cadd(v[1], v[2]);
+    const double2 d = rot_i<INV>(csub(v[1], v[2]));
+    const double2 t = make_double2(v[0].x + c3 * s.x, v[0].y + c3 * s.y);
+    X[0] = cadd(v[0], s);
+    X[1] = make_double2(t.x + s3 * d.x, t.y + s3 * d.y);
+    X[2] = make_double2(t.x - s3 * d.x, t.y - s3 * d.y);
+  } else {
+    static_assert(R == 5, "radix");
+    const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
+    const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;
+    const double2 a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+    const double2 a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+    X[0] = cadd(v[0], cadd(a1, a2));
+    const double2 t1 = make_double2(v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y);
+    const double2 t2 = make_double2(v[0].x + c2 * a1.x + c1 * a2.x, v[0].y + c2 * a1.y + c1 * a2.y);
+    const double2 u1 = rot_i<INV>(make_double2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y));
+    const double2 u2 = rot_i<INV>(make_double2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y));
+    X[1] = cadd(t1, u1);
+    X[4] = csub(t1, u1);
+    X[2] = cadd(t2, u2);
+    X[3] = csub(t2, u2);
+  }
+}
+
+// unit -> (block b, slab l, group g), units ordered by block, slab (l >= omega_b), group
+__device__ __forceinline__ void decode_slab_unit(const JobDev &jb, int u, int groups, int &b, int &l, int &g) {
+  b = 0;
+  for (;;) {
+    const int ub = (jb.L - (jb.omin + b)) * groups;
+    if (u < ub || b == jb.nO - 1) break;
+    u -= ub;
+    b++;
+  }
+  l = jb.omin + b + u / groups;
+  g = u % groups;
+}
+
+// rotational . Coriolis . rotational at one grid point (i, j) of slab l for
+// the nb blocks present at l:
+//   y_b = rot2(l, j) sum_a V_l[a][b] phi(l, i, a) sum_b1 V_l[a][b1] x_b1
+// = exp(-i dt/4 l(l+1)/(2 m r^2)) exp(-i dt/2 C_l/(2 m1 R^2)) exp(-i dt/4 ...) x
+// with C_l = V_l^T diag(e) V_l; evolutionCUDAaux.cu:178-209 applies the same
+// exponential as a dense sum over the eigenpairs.  With ENERGY, accumulates the
+// Coriolis energy of x with the exact tridiagonal entries of C_l
+// (evolutionCUDAaux.cu:211-241; the scalar rotational phase cancels in
+// <x_b|C|x_b1>) and the rotational energy l(l+1)(I1 + I2)|y|^2 of the output.
+template <bool ENERGY, int NB>
+__device__ __forceinline__ void coriolis_point(const JobDev &jb, double2 (&x)[NB], int nb, int l, int i, int j,
+                                               const double *__restrict__ V, double (&ce)[NB],
+                                               double (&re)[NB]) {
+  const int nO = jb.nO;
+  if (ENERGY) {
+    const double I1 = jb.I1[i];
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      if (b < nb) {
+        const double2 c = jb.cmat[l * nO + b];
+        double en = c.x * cnorm(x[b]);
+        if (b + 1 < nb) en += 2.0 * c.y * cdot_re(x[b], x[b + 1]);
+        ce[b] += en * I1;
+      }
+    }
+  }
+  const double2 *phi = jb.corPhi + ((long)l * jb.n1 + i) * nO;
+  double2 t[NB];
+#pragma unroll
+  for (int al = 0; al < NB; al++) {
+    if (al < nb) {
+      double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int b = 0; b < NB; b++)
+        if (b < nb) {
+          const double v = V[al * nO + b];
+          acc.x = fma(v, x[b].x, acc.x);
+          acc.y = fma(v, x[b].y, acc.y);
+        }
+      t[al] = cmul(phi[al], acc);
+    }
+  }
+  const double2 r2 = jb.rot2[(long)l * jb.n2 + j];
+  const double wrot = (double)l * (l + 1) * (jb.I1[i] + jb.I2[j]);
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+    if (b < nb) {
+      double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int al = 0; al < NB; al++)
+        if (al < nb) {
+          const double v = V[al * nO + b];
+          acc.x = fma(v, t[al].x, acc.x);
+          acc.y = fma(v, t[al].y, acc.y);
+        }
+      x[b] = cmul(r2, acc);
+      if (ENERGY) re[b] += wrot * cnorm(x[b]);
+    }
+  }
+}
+
+// Sums 2*NB per-thread energy values over the block and stores them to
+// part_cor[(2b + f) * units + u]; red must hold 4 * 2 * NB doubles.
+template <int NB>
+__device__ __forceinline__ void store_energy_partials(const JobDev &jb, int u, const double (&ce)[NB],
+                                                      const double (&re)[NB], double *red) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 2 * NB; q++) {
+    double x = (q & 1) ? re[q >> 1] : ce[q >> 1];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    if (lane == 0) red[wave * 2 * NB + q] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 * jb.nO) {
+    const int q = threadIdx.x;   // q = 2 b + f
+    const double sum = (q < 2 * NB) ? (red[q] + red[2 * NB + q]) + (red[4 * NB + q] + red[6 * NB + q]) : 0.0;
+    jb.part_cor[(long)q * jb.units[KK_FFT_R] + u] = sum;
+  }
+}
+
+}  // namespace qd

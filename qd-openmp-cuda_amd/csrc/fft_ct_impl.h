@@ -1,0 +1,237 @@
+// fft_ct_impl.h - compile-time-length FFT passes for the common grid
+// sizes (32, 48, 64, 96, 128, 192, 256 points along R or r).
+//
+// Same operators as kernels_fft.hip (see there for the reference
+// counterparts).  With the length, the pencils per workgroup and the radix
+// plan known at compile time, every index computation divides by constants
+// and the stages unroll.  Each stage is in-place on one LDS tile: a thread
+// reads the inputs of its butterflies into registers, the workgroup
+// synchronises, and the outputs are written back, so a tile needs no
+// ping-pong buffer.  In the R pass the tiles of all omega blocks present at
+// slab l stay in registers (the Coriolis mix couples blocks point by point)
+// and only one block at a time passes through LDS.
+#pragma once
+#include "fft_common.h"
+
+namespace qd {
+
+// Stockham stages NS..N of an in-place transform of NPEN pencils; pencil p
+// element e is A[p * PS + e * ES].  PF maps consecutive threads to
+// consecutive pencils.  Entry and exit are synchronised.
+template <bool INV, int N, int NS, int NPEN, int PS, int ES, bool PF>
+__device__ __forceinline__ void stages_ip(double2 *A, const double2 *tw) {
+  if constexpr (NS < N) {
+    constexpr int R = next_radix(N / NS);
+    constexpr int NB = N / R;
+    constexpr int TOTAL = NPEN * NB;
+    constexpr int MB = (TOTAL + kBlock - 1) / kBlock;
+    constexpr int TWS = N / (NS * R);
+    double2 X[MB][R];
+#pragma unroll
+    for (int m = 0; m < MB; m++) {
+      const int t = threadIdx.x + m * kBlock;
+      if (TOTAL % kBlock == 0 || t < TOTAL) {
+        const int p = PF ? t % NPEN : t / NB;
+        const int idx = PF ? t / NPEN : t % NB;
+        double2 v[R];
+#pragma unroll
+        for (int q = 0; q < R; q++) v[q] = A[p * PS + (idx + q * NB) * ES];
+        if constexpr (NS > 1) {
+          const int k = idx % NS;
+#pragma unroll
+          for (int q = 1; q < R; q++) {
+            double2 w = tw[q * k * TWS];
+            if (INV) w.y = -w.y;
+            v[q] = cmul(v[q], w);
+          }
+        }
+        dft_small<INV, R>(v, X[m]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < MB; m++) {
+      const int t = threadIdx.x + m * kBlock;
+      if (TOTAL % kBlock == 0 || t < TOTAL) {
+        const int p = PF ? t % NPEN : t / NB;
+        const int idx = PF ? t / NPEN : t % NB;
+        const int d = (idx / NS) * NS * R + idx % NS;
+#pragma unroll
+        for (int q = 0; q < R; q++) A[p * PS + (d + q * NS) * ES] = X[m][q];
+      }
+    }
+    __syncthreads();
+    stages_ip<INV, N, NS * R, NPEN, PS, ES, PF>(A, tw);
+  }
+}
+
+// rows per workgroup of the R pass and columns per workgroup of the r pass
+__host__ __device__ constexpr int ct_rows(int n1) { return n1 <= 128 ? 8 : 4; }
+__host__ __device__ constexpr int ct_cols(int n2) { return n2 <= 64 ? 16 : 8; }
+
+// ---------------------------------------------------------------------------
+// R pass with the Coriolis mix (forward: mix then DFT; inverse: DFT, scale,
+// energies, mix)
+// ---------------------------------------------------------------------------
+template <bool INV, int NB, int N1>
+__global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
+  constexpr int P = ct_rows(N1);
+  constexpr int PN = P * N1;
+  constexpr int PPT = (PN + kBlock - 1) / kBlock;
+  __shared__ double2 A[PN];
+  __shared__ double2 tw[N1];
+  __shared__ double red[4 * 2 * NB];
+  const int job = locate_job(a.unit_base, a.n_jobs, blockIdx.x);
+  const JobDev &jb = a.jobs[job];
+  const int u = blockIdx.x - a.unit_base[job];
+  const int M = jb.M;
+  const long LM = (long)jb.L * M;
+  const int groups = jb.n2 / P;
+  const int l = jb.omin + u / groups, g = u % groups;
+  const int nb = min(jb.kmax, l) - jb.omin + 1;
+  const double *V = jb.corV + (long)l * jb.nO * jb.nO;
+  double2 *base = jb.fbr + (long)l * M + (long)g * PN;
+  for (int e = threadIdx.x; e < N1; e += kBlock) tw[e] = jb.twR[e];
+  double2 x[NB][PPT];
+  double ce[NB], re[NB];
+#pragma unroll
+  for (int b = 0; b < NB; b++) { ce[b] = 0.0; re[b] = 0.0; }
+  if (!INV) {
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+#pragma unroll
+      for (int k = 0; k < PPT; k++) {
+        const int e = threadIdx.x + k * kBlock;
+        x[b][k] = (b < nb && (PN % kBlock == 0 || e < PN)) ? base[b * LM + e] : make_double2(0.0, 0.0);
+      }
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+      const int e = threadIdx.x + k * kBlock;
+      if (PN % kBlock == 0 || e < PN) {
+        double2 xp[NB];
+#pragma unroll
+        for (int b = 0; b < NB; b++) xp[b] = x[b][k];
+        coriolis_point<false, NB>(jb, xp, nb, l, e % N1, g * P + e / N1, V, ce, re);
+#pragma unroll
+        for (int b = 0; b < NB; b++) x[b][k] = xp[b];
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      if (b < nb) {
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+          const int e = threadIdx.x + k * kBlock;
+          if (PN % kBlock == 0 || e < PN) A[e] = x[b][k];
+        }
+        __syncthreads();
+        stages_ip<false, N1, 1, P, N1, 1, false>(A, tw);
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+          const int e = threadIdx.x + k * kBlock;
+          if (PN % kBlock == 0 || e < PN) base[b * LM + e] = A[e];
+        }
+      }
+    }
+  } else {
+    const double s = 1.0 / (double)M;
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      if (b < nb) {
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+          const int e = threadIdx.x + k * kBlock;
+          if (PN % kBlock == 0 || e < PN) A[e] = base[b * LM + e];
+        }
+        __syncthreads();
+        stages_ip<true, N1, 1, P, N1, 1, false>(A, tw);
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+          const int e = threadIdx.x + k * kBlock;
+          x[b][k] = (PN % kBlock == 0 || e < PN) ? cscale(A[e], s) : make_double2(0.0, 0.0);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < PPT; k++) x[b][k] = make_double2(0.0, 0.0);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+      const int e = threadIdx.x + k * kBlock;
+      if (PN % kBlock == 0 || e < PN) {
+        double2 xp[NB];
+#pragma unroll
+        for (int b = 0; b < NB; b++) xp[b] = x[b][k];
+        coriolis_point<true, NB>(jb, xp, nb, l, e % N1, g * P + e / N1, V, ce, re);
+#pragma unroll
+        for (int b = 0; b < NB; b++)
+          if (b < nb) base[b * LM + e] = xp[b];
+      }
+    }
+    store_energy_partials<NB>(jb, u, ce, re, red);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// r pass: forward DFT along r, kinetic phase and energy, inverse DFT
+// ---------------------------------------------------------------------------
+template <int N2>
+__global__ __launch_bounds__(kBlock) void k_fft_r_ct(StepArgs a) {
+  constexpr int Q = ct_cols(N2);
+  constexpr int QN = Q * N2;
+  __shared__ double2 A[QN];
+  __shared__ double2 tw[N2];
+  __shared__ double red[4];
+  const int job = locate_job(a.unit_base, a.n_jobs, blockIdx.x);
+  const JobDev &jb = a.jobs[job];
+  const int u = blockIdx.x - a.unit_base[job];
+  const int n1 = jb.n1;
+  int b, l, g;
+  decode_slab_unit(jb, u, n1 / Q, b, l, g);
+  double2 *base = jb.fbr + ((long)b * jb.L + l) * jb.M + (long)g * Q;
+  for (int e = threadIdx.x; e < N2; e += kBlock) tw[e] = jb.twr[e];
+  for (int e = threadIdx.x; e < QN; e += kBlock) A[e] = base[(long)(e / Q) * n1 + (e % Q)];
+  __syncthreads();
+  stages_ip<false, N2, 1, Q, 1, Q, true>(A, tw);
+  double ke = 0.0;
+  for (int e = threadIdx.x; e < QN; e += kBlock) {
+    const int i = g * Q + (e % Q), j = e / Q;
+    const double2 x = cmul(A[e], cmul(jb.kin1[i], jb.kin2[j]));
+    A[e] = x;
+    ke += cnorm(x) * (jb.kinE1[i] + jb.kinE2[j]);
+  }
+  __syncthreads();
+  stages_ip<true, N2, 1, Q, 1, Q, true>(A, tw);
+  for (int e = threadIdx.x; e < QN; e += kBlock) base[(long)(e / Q) * n1 + (e % Q)] = A[e];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) ke += __shfl_xor(ke, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ke;
+  __syncthreads();
+  if (threadIdx.x == 0) jb.part_ke[u] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// ---------------------------------------------------------------------------
+// launchers for one compile-time R length (instantiated in kernels_fft_ct*.hip)
+// ---------------------------------------------------------------------------
+template <bool INV, int N1>
+void launch_R_nb(const StepArgs &a, int units, int nb_max, hipStream_t s) {
+  const dim3 g(units), t(kBlock);
+  if (nb_max <= 1) hipLaunchKernelGGL((k_fft_R_ct<INV, 1, N1>), g, t, 0, s, a);
+  else if (nb_max <= 2) hipLaunchKernelGGL((k_fft_R_ct<INV, 2, N1>), g, t, 0, s, a);
+  else if (nb_max <= 4) hipLaunchKernelGGL((k_fft_R_ct<INV, 4, N1>), g, t, 0, s, a);
+  else if (nb_max <= 6) hipLaunchKernelGGL((k_fft_R_ct<INV, 6, N1>), g, t, 0, s, a);
+  else if (nb_max <= 9) hipLaunchKernelGGL((k_fft_R_ct<INV, 9, N1>), g, t, 0, s, a);
+  else if (nb_max <= 16) hipLaunchKernelGGL((k_fft_R_ct<INV, 16, N1>), g, t, 0, s, a);
+  else hipLaunchKernelGGL((k_fft_R_ct<INV, 24, N1>), g, t, 0, s, a);
+}
+
+#define QD_FFT_CT_DEFINE(N1)                                                                   \
+  void launch_fft_R_ct_##N1(const StepArgs &a, int units, int inverse, int nb_max, hipStream_t s) { \
+    if (inverse) launch_R_nb<true, N1>(a, units, nb_max, s);                                   \
+    else launch_R_nb<false, N1>(a, units, nb_max, s);                                          \
+  }                                                                                            \
+  void launch_fft_r_ct_##N1(const StepArgs &a, int units, hipStream_t s) {                     \
+    hipLaunchKernelGGL(k_fft_r_ct<N1>, dim3(units), dim3(kBlock), 0, s, a);                    \
+  }
+
+}  // namespace qd

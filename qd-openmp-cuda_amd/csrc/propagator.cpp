@@ -211,8 +211,12 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   d.gemm_bwd_groups = d.gemm_fwd_groups;
   static const int rows_c[] = {8, 4, 2, 1};
   static const int cols_c[] = {16, 8, 4, 2, 1};
-  d.fft_rows = pick_divisor(n2, rows_c, 4, 2048, n1);
-  d.fft_cols = pick_divisor(n1, cols_c, 5, 2048, n2);
+  // compile-time FFT kernels where the lengths allow; otherwise the runtime-size
+  // R pass keeps nb(l) <= nO blocks of fft_rows rows plus one scratch tile in LDS
+  if (fft_ct_supported(n1) && n2 % fft_ct_rows(n1) == 0) d.fft_rows = fft_ct_rows(n1);
+  else d.fft_rows = pick_divisor(n2, rows_c, 4, 5120 / (nO + 1), n1);
+  if (fft_ct_supported(n2) && n1 % fft_ct_cols(n2) == 0) d.fft_cols = fft_ct_cols(n2);
+  else d.fft_cols = pick_divisor(n1, cols_c, 5, 2048, n2);
   d.obs_chunks = std::max(1, (M + 2047) / 2048);
   d.fbr_chunks = (M + kBlock - 1) / kBlock;
   long slabs = 0;
@@ -220,11 +224,11 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   d.units[KK_DVR] = (int)((G + kBlock * 4 - 1) / (kBlock * 4));
   d.units[KK_LEG_FWD] = nO * d.gemm_fwd_groups;
   d.units[KK_LEG_BWD] = nO * d.gemm_bwd_groups;
-  d.units[KK_FBR] = (L - sh.omin) * d.fbr_chunks;
-  d.units[KK_FFT_R] = (int)(slabs * (n2 / d.fft_rows));
+  d.units[KK_FBR] = 0;
+  d.units[KK_FFT_R] = (L - sh.omin) * (n2 / d.fft_rows);
   d.units[KK_FFT_r] = (int)(slabs * (n1 / d.fft_cols));
   d.units[KK_OBS] = nO * nt * d.obs_chunks;
-  d.units[KK_REDUCE] = 1;
+  d.units[KK_REDUCE] = nO;
   d.units[KK_FLUX] = 0;
 
   // ---- host tables
@@ -280,9 +284,11 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
         Pt[((size_t)b * d.bwd_kp + c) * d.bwd_np + k] = p;
       }
   }
-  // Coriolis exponentials with the R part of the rotational phase:
-  //   corF(l, i)[b][b1] = exp(-i dt/2 l(l+1) I1_i) sum_a exp(-i dt/2 e_a / (2 m1 R_i^2)) v_ab v_ab1
-  std::vector<double2> corF((size_t)L * n1 * nO * nO, make_double2(0.0, 0.0));
+  // Coriolis eigenpairs and phases: exp(-i dt/2 C_l/(2 m1 R_i^2)) = V^T diag(exp(-i dt/2 e_a I1_i)) V,
+  // with the R part of the two rotational quarter steps folded into the phases:
+  //   corPhi(l, i, a) = exp(-i dt/2 l(l+1) I1_i) exp(-i dt/2 e_a I1_i)
+  std::vector<double> corV((size_t)L * nO * nO, 0.0);
+  std::vector<double2> corPhi((size_t)L * n1 * nO, make_double2(0.0, 0.0));
   std::vector<double2> cmat((size_t)L * nO, make_double2(0.0, 0.0));
   std::vector<double2> rot2((size_t)L * n2);
   const double tau = dt / 2;
@@ -295,21 +301,11 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
     for (int b = 0; b < n; b++)
       cmat[(size_t)l * nO + b] = make_double2(diag[b], b + 1 < n ? off[b] : 0.0);
     tridiag_eigen(n, diag.data(), off.data(), z.data());
+    for (int a = 0; a < n; a++)
+      for (int b = 0; b < n; b++) corV[((size_t)l * nO + a) * nO + b] = z[a * n + b];
     for (int i = 0; i < n1; i++) {
-      const double2 pr = phase(-tau * l * (l + 1) * I1[i]);
-      std::vector<double2> ea(n);
-      for (int a = 0; a < n; a++) ea[a] = phase(-tau * I1[i] * diag[a]);
-      double2 *F = corF.data() + ((size_t)l * n1 + i) * nO * nO;
-      for (int b = 0; b < n; b++)
-        for (int b1 = 0; b1 < n; b1++) {
-          double re = 0.0, im = 0.0;
-          for (int a = 0; a < n; a++) {
-            const double vv = z[a * n + b] * z[a * n + b1];
-            re += ea[a].x * vv;
-            im += ea[a].y * vv;
-          }
-          F[b * nO + b1] = make_double2(pr.x * re - pr.y * im, pr.x * im + pr.y * re);
-        }
+      for (int a = 0; a < n; a++)
+        corPhi[((size_t)l * n1 + i) * nO + a] = phase(-tau * l * (l + 1) * I1[i] - tau * I1[i] * diag[a]);
     }
   }
 
@@ -332,7 +328,8 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
     d.wP_hi = upload(h, st); d.wP_lo = upload(lo, st);
     d.Pt_hi = upload(h2, st); d.Pt_lo = upload(lo2, st);
   }
-  d.corF = upload(corF, st);
+  d.corV = upload(corV, st);
+  d.corPhi = upload(corPhi, st);
   d.rot2 = upload(rot2, st);
   d.kin1 = upload(kin1, st);
   d.kin2 = upload(kin2, st);
@@ -343,14 +340,14 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   d.cmat = upload(cmat, st);
   d.twR = upload(twiddles(n1), st);
   d.twr = upload(twiddles(n2), st);
-  d.part_fbr = (double *)alloc(sizeof(double) * (size_t)d.units[KK_FBR] * nO * 2);
+  d.part_cor = (double *)alloc(sizeof(double) * (size_t)d.units[KK_FFT_R] * nO * 2);
   d.part_ke = (double *)alloc(sizeof(double) * (size_t)d.units[KK_FFT_r]);
   d.part_obs = (double *)alloc(sizeof(double) * (size_t)d.units[KK_OBS] * 4);
   d.ring = kRing;
   d.obs_ring = (double *)alloc(sizeof(double) * (size_t)kRing * nO * 5);
-  d.acf_ring = (double2 *)alloc(sizeof(double2) * kRing);
-  d.counters = (int *)alloc(2 * sizeof(int));
-  const int cnt[2] = {sh.steps0, 0};
+  d.acf_ring = (double2 *)alloc(sizeof(double2) * (size_t)kRing * nO);
+  d.counters = (int *)alloc(3 * sizeof(int));
+  const int cnt[3] = {sh.steps0, 0, 0};
   QD_HIP(hipMemcpyAsync(d.counters, cnt, sizeof(cnt), hipMemcpyHostToDevice, st));
   d.flux_on = 0;
   QD_HIP(hipStreamSynchronize(st));   // host tables go out of scope
@@ -366,7 +363,12 @@ void DeviceJob::drain(int records_total) {
     const int slot = r % dev.ring;
     QD_HIP(hipMemcpy(obs.data() + (size_t)r * nO * 5, dev.obs_ring + (size_t)slot * nO * 5,
                      sizeof(double) * nO * 5, hipMemcpyDeviceToHost));
-    QD_HIP(hipMemcpy(acf.data() + (size_t)r * 2, dev.acf_ring + slot, sizeof(double2), hipMemcpyDeviceToHost));
+    std::vector<double2> part(nO);
+    QD_HIP(hipMemcpy(part.data(), dev.acf_ring + (size_t)slot * nO, sizeof(double2) * nO, hipMemcpyDeviceToHost));
+    double re = 0.0, im = 0.0;
+    for (int b = 0; b < nO; b++) { re += part[b].x; im += part[b].y; }
+    acf[(size_t)r * 2] = re;
+    acf[(size_t)r * 2 + 1] = im;
   }
   records_done = records_total;
 }
@@ -415,6 +417,8 @@ void Engine::set_jobs(const std::vector<DeviceJob *> &jobs) {
   std::vector<JobDev> descs(n);
   std::vector<int> bases((size_t)KK_COUNT * (n + 1), 0);
   lds_fftR_ = lds_fftr_ = 0;
+  ct_n1_ = jobs[0]->dev.n1;
+  ct_n2_ = jobs[0]->dev.n2;
   fwd_tiles_ = bwd_tiles_ = fwd_kp_ = bwd_kp_ = 0;
   nb_max_ = 1;
   mixed_ = jobs[0]->dev.mixed;
@@ -424,8 +428,11 @@ void Engine::set_jobs(const std::vector<DeviceJob *> &jobs) {
     QD_REQUIRE(d.mixed == mixed_, "all jobs of one engine must share a precision");
     descs[q] = d;
     for (int k = 0; k < KK_COUNT; k++) bases[(size_t)k * (n + 1) + q + 1] = bases[(size_t)k * (n + 1) + q] + d.units[k];
-    lds_fftR_ = std::max(lds_fftR_, (size_t)(2 * d.fft_rows * d.n1 + d.n1) * 16);
+    lds_fftR_ = std::max(lds_fftR_, (size_t)((d.nO + 1) * d.fft_rows * d.n1 + d.n1) * 16);
     lds_fftr_ = std::max(lds_fftr_, (size_t)(2 * d.fft_cols * d.n2 + d.n2) * 16);
+    // compile-time kernels only when every job shares the length and tiling
+    if (d.n1 != ct_n1_ || !fft_ct_supported(d.n1) || d.fft_rows != fft_ct_rows(d.n1)) ct_n1_ = 0;
+    if (d.n2 != ct_n2_ || !fft_ct_supported(d.n2) || d.fft_cols != fft_ct_cols(d.n2)) ct_n2_ = 0;
     fwd_tiles_ = std::max(fwd_tiles_, d.fwd_np / 16);
     bwd_tiles_ = std::max(bwd_tiles_, d.bwd_np / 16);
     fwd_kp_ = std::max(fwd_kp_, d.fwd_kp);
@@ -457,22 +464,23 @@ void Engine::enqueue(hipEvent_t *ev) {
   mark(1);
   launch_leg_fwd(args(KK_LEG_FWD), total_[KK_LEG_FWD], fwd_tiles_, fwd_kp_, mixed_, s);
   mark(2);
-  launch_fbr(args(KK_FBR), total_[KK_FBR], 0, nb_max_, s);
   mark(3);
-  launch_fft_R(args(KK_FFT_R), total_[KK_FFT_R], 0, lds_fftR_, s);
+  if (ct_n1_) launch_fft_R_ct(args(KK_FFT_R), total_[KK_FFT_R], 0, ct_n1_, nb_max_, s);
+  else launch_fft_R_rt(args(KK_FFT_R), total_[KK_FFT_R], 0, nb_max_, lds_fftR_, s);
   mark(4);
-  launch_fft_r(args(KK_FFT_r), total_[KK_FFT_r], lds_fftr_, s);
+  if (ct_n2_) launch_fft_r_ct(args(KK_FFT_r), total_[KK_FFT_r], ct_n2_, s);
+  else launch_fft_r_rt(args(KK_FFT_r), total_[KK_FFT_r], lds_fftr_, s);
   mark(5);
-  launch_fft_R(args(KK_FFT_R), total_[KK_FFT_R], 1, lds_fftR_, s);
+  if (ct_n1_) launch_fft_R_ct(args(KK_FFT_R), total_[KK_FFT_R], 1, ct_n1_, nb_max_, s);
+  else launch_fft_R_rt(args(KK_FFT_R), total_[KK_FFT_R], 1, nb_max_, lds_fftR_, s);
   mark(6);
-  launch_fbr(args(KK_FBR), total_[KK_FBR], 1, nb_max_, s);
   mark(7);
   launch_leg_bwd(args(KK_LEG_BWD), total_[KK_LEG_BWD], bwd_tiles_, bwd_kp_, mixed_, s);
   mark(8);
   launch_obs(args(KK_OBS), total_[KK_OBS], s);
   if (any_flux_) launch_flux(args(KK_FLUX), total_[KK_FLUX], s);
   mark(9);
-  launch_reduce(args(KK_REDUCE), (int)jobs_.size(), s);
+  launch_reduce(args(KK_REDUCE), total_[KK_REDUCE], s);
   mark(10);
   QD_HIP(hipGetLastError());
 }

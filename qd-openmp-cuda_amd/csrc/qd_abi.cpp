@@ -315,6 +315,33 @@ qd_status qd_probe_fft(double *data, int32_t n, int32_t count, int32_t inverse) 
   })
 }
 
+qd_status qd_probe_copy(int64_t bytes, int32_t repeats, double *us_per_copy) {
+  QD_GUARD(nullptr, {
+    QD_REQUIRE(bytes >= 16 && repeats > 0 && us_per_copy, "bad probe arguments");
+    const long n = bytes / 16;
+    double2 *a, *b;
+    QD_HIP(hipMalloc(&a, 16 * n));
+    QD_HIP(hipMalloc(&b, 16 * n));
+    QD_HIP(hipMemset(a, 0, 16 * n));
+    hipEvent_t e0, e1;
+    QD_HIP(hipEventCreate(&e0));
+    QD_HIP(hipEventCreate(&e1));
+    launch_copy(a, b, n, nullptr);
+    QD_HIP(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < repeats; r++) launch_copy(r % 2 ? b : a, r % 2 ? a : b, n, nullptr);
+    QD_HIP(hipEventRecord(e1, nullptr));
+    QD_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    QD_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *us_per_copy = ms * 1000.0 / repeats;
+    QD_HIP(hipEventDestroy(e0));
+    QD_HIP(hipEventDestroy(e1));
+    QD_HIP(hipFree(a));
+    QD_HIP(hipFree(b));
+    return QD_OK;
+  })
+}
+
 qd_status qd_probe_coriolis(int32_t J, int32_t l, int32_t omega_min, int32_t n, double *evals, double *evecs) {
   QD_GUARD(nullptr, {
     QD_REQUIRE(evals && evecs && n >= 1 && n <= 64, "bad probe arguments");

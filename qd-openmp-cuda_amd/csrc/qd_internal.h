@@ -60,7 +60,8 @@ struct JobDev {
   const double *Pt;          // [nO][bwd_kp][bwd_np] row-major: P_l(x_k), k = l - omega
   const float *wP_hi, *wP_lo;// split FP32 copies for the mixed form
   const float *Pt_hi, *Pt_lo;
-  const double2 *corF;       // [L][n1][nO][nO] rotational(dt/2 on R) x Coriolis exponential
+  const double *corV;        // [L][nO][nO] eigenvectors of C_l: corV[(l nO + a) nO + b] = v_a(omega_b)
+  const double2 *corPhi;     // [L][n1][nO] exp(-i dt/2 l(l+1) I1_i) exp(-i dt/2 e_a I1_i)
   const double2 *rot2;       // [L][n2] exp(-i dt/2 l(l+1) / (2 m2 r_j^2))
   const double2 *kin1;       // [n1] exp(-i dt k_i^2 / (2 m1))
   const double2 *kin2;       // [n2] exp(-i dt k_j^2 / (2 m2))
@@ -72,14 +73,15 @@ struct JobDev {
   const double2 *twR;        // [n1] exp(-2 pi i e / n1)
   const double2 *twr;        // [n2] exp(-2 pi i e / n2)
   // reduction scratch (per-workgroup partial sums)
-  double *part_fbr;          // [units FBR][nO][2]  (Coriolis energy, rotational energy)
+  double *part_cor;          // [nO][2][units FFT_R] Coriolis and rotational energy partials
   double *part_ke;           // [units FFT_r]       kinetic energy of one (block, slab, column group)
   double *part_obs;          // [nO][nt][obs_chunks][4] (norm, potential, acf re, acf im)
   double *obs_ring;          // [ring][nO][5] per-step observables
-  double2 *acf_ring;         // [ring] autocorrelation
+  double2 *acf_ring;         // [ring][nO] autocorrelation contribution of each block
   int ring;                  // ring capacity (records)
   int *counters;             // device: [0] step counter (the reference's time.steps),
-                             //         [1] records written; advanced by the reduce kernel
+                             //         [1] records written, [2] reduce ticket;
+                             //         advanced by the last reduce workgroup of the step
   // flux
   int flux_on, flux_ns, flux_ng, flux_ne;
   const double *flux_coef;   // [ng] derivative weights
@@ -109,14 +111,26 @@ __device__ __forceinline__ int locate_job(const int *base, int n_jobs, int u) {
 void launch_dvr(const StepArgs &a, int total_units, hipStream_t stream);
 void launch_leg_fwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, hipStream_t stream);
 void launch_leg_bwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, hipStream_t stream);
-void launch_fbr(const StepArgs &a, int total_units, int with_energy, int nb_max, hipStream_t stream);
-void launch_fft_R(const StepArgs &a, int total_units, int inverse, size_t lds, hipStream_t stream);
-void launch_fft_r(const StepArgs &a, int total_units, size_t lds, hipStream_t stream);
+// FFT along R fused with rotational . Coriolis . rotational (before the forward
+// transform, after the inverse one; the inverse pass also forms the energies),
+// and the r pass with the kinetic phase.  _rt: any length factoring into
+// 2, 3, 5 (dynamic LDS); _ct: compile-time lengths (fft_ct_supported), with
+// fft_ct_rows(n1) rows / fft_ct_cols(n2) columns per workgroup.
+void launch_fft_R_rt(const StepArgs &a, int total_units, int inverse, int nb_max, size_t lds, hipStream_t stream);
+void launch_fft_r_rt(const StepArgs &a, int total_units, size_t lds, hipStream_t stream);
+bool fft_ct_supported(int n);
+int fft_ct_rows(int n1);
+int fft_ct_cols(int n2);
+void launch_fft_R_ct(const StepArgs &a, int total_units, int inverse, int n1, int nb_max, hipStream_t stream);
+void launch_fft_r_ct(const StepArgs &a, int total_units, int n2, hipStream_t stream);
 // standalone batched 1-D FFT of `count` contiguous pencils (test probe)
 void launch_fft_probe(double2 *data, const double2 *tw, int n, int count, int inverse, hipStream_t stream);
 void launch_obs(const StepArgs &a, int total_units, hipStream_t stream);
-void launch_reduce(const StepArgs &a, int n_jobs, hipStream_t stream);
+void launch_reduce(const StepArgs &a, int total_units, hipStream_t stream);
 void launch_flux(const StepArgs &a, int total_units, hipStream_t stream);
+
+// plain copy kernel (traffic calibration)
+void launch_copy(const double2 *src, double2 *dst, long n, hipStream_t stream);
 
 // GEMM probe for tests: C (rows x n) = A (rows x k, lda) * B ([kp][np] row-major)
 void launch_gemm_probe(const double *A, int lda, const double *B, int kp, int np, double *C, int ldc,

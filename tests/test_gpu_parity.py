@@ -100,6 +100,14 @@ def test_parity_coriolis_untruncated_even():
     check_parity(a, b)
 
 
+def test_parity_runtime_fft_lengths():
+    # 80 = 2^4 * 5 and 40 are not compile-time lengths: exercises the runtime-size FFT path
+    spec = inputs.JobSpec(n1=80, n2=40, ntheta=20, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
+                          J=2, parity=0, k_max=-1, omega0=0, j0=1, E_coll=0.02)
+    a, b = run_both(spec, 4)
+    check_parity(a, b)
+
+
 def test_parity_c1_shape():
     spec = inputs.config_spec("c1")
     a, b = run_both(spec, 4)
