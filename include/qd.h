@@ -167,7 +167,9 @@ typedef struct {
   int32_t print_energies; /* print per-step lines in the reference's format */
   int32_t final_copy;     /* qd_evolve: copy the final state into wave_packets */
   double *observables;    /* qd_evolve: optional out [total_steps][n_omegas][QD_OBS_FIELDS] */
-  double *autocorrelation;/* qd_evolve: optional out, complex <psi(0)|psi(t_n)> per step */
+  double *autocorrelation;/* optional out, complex <psi(0)|psi(t_n)> per step; when NULL at
+                             create the autocorrelation is not computed (saves one read
+                             of the initial state per step) */
 } qd_inputs;
 
 typedef struct qd_handle qd_handle;

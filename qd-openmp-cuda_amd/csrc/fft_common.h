@@ -77,6 +77,21 @@ __device__ __forceinline__ void decode_slab_unit(const JobDev &jb, int u, int gr
   g = u % groups;
 }
 
+// Per-thread sums of the second FBR half step, per block: Coriolis energy of
+// the pre-mix state, and of the output y the rotational energy, |y|^2 and
+// conj(c0) y (c0: FBR coefficients of the initial state).
+template <int NB, int NF>
+struct FbrSums {
+  static constexpr int F = NF;  // ce, re, norm [, acf re, acf im]
+  double v[NB][F];
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+#pragma unroll
+      for (int f = 0; f < F; f++) v[b][f] = 0.0;
+  }
+};
+
 // rotational . Coriolis . rotational at one grid point (i, j) of slab l for
 // the nb blocks present at l:
 //   y_b = rot2(l, j) sum_a V_l[a][b] phi(l, i, a) sum_b1 V_l[a][b1] x_b1
@@ -85,11 +100,16 @@ __device__ __forceinline__ void decode_slab_unit(const JobDev &jb, int u, int gr
 // exponential as a dense sum over the eigenpairs.  With ENERGY, accumulates the
 // Coriolis energy of x with the exact tridiagonal entries of C_l
 // (evolutionCUDAaux.cu:211-241; the scalar rotational phase cancels in
-// <x_b|C|x_b1>) and the rotational energy l(l+1)(I1 + I2)|y|^2 of the output.
-template <bool ENERGY, int NB>
+// <x_b|C|x_b1>), the rotational energy l(l+1)(I1 + I2)|y|^2
+// (omegawavepacket.cu:397-428), |y|^2 and, when c0 (this point of block 0 in
+// the initial-state FBR array, blocks LM apart) is given, conj(c0) y.
+// The last two equal the DVR norm and autocorrelation of the state after the
+// backward Legendre transform: the Gauss-Legendre pair is an isometry between
+// the FBR coefficients and the band-limited DVR values.
+template <bool ENERGY, int NB, int NF>
 __device__ __forceinline__ void coriolis_point(const JobDev &jb, double2 (&x)[NB], int nb, int l, int i, int j,
-                                               const double *__restrict__ V, double (&ce)[NB],
-                                               double (&re)[NB]) {
+                                               const QD_G double *__restrict__ V, FbrSums<NB, NF> &sm,
+                                               const QD_G double2 *c0, long LM) {
   const int nO = jb.nO;
   if (ENERGY) {
     const double I1 = jb.I1[i];
@@ -99,11 +119,11 @@ __device__ __forceinline__ void coriolis_point(const JobDev &jb, double2 (&x)[NB
         const double2 c = jb.cmat[l * nO + b];
         double en = c.x * cnorm(x[b]);
         if (b + 1 < nb) en += 2.0 * c.y * cdot_re(x[b], x[b + 1]);
-        ce[b] += en * I1;
+        sm.v[b][0] += en * I1;
       }
     }
   }
-  const double2 *phi = jb.corPhi + ((long)l * jb.n1 + i) * nO;
+  const QD_G double2 *phi = jb.corPhi + ((long)l * jb.n1 + i) * nO;
   double2 t[NB];
 #pragma unroll
   for (int al = 0; al < NB; al++) {
@@ -133,29 +153,38 @@ __device__ __forceinline__ void coriolis_point(const JobDev &jb, double2 (&x)[NB
           acc.y = fma(v, t[al].y, acc.y);
         }
       x[b] = cmul(r2, acc);
-      if (ENERGY) re[b] += wrot * cnorm(x[b]);
+      if (ENERGY) {
+        const double q = cnorm(x[b]);
+        sm.v[b][1] += wrot * q;
+        sm.v[b][2] += q;
+        if constexpr (NF == 5) {
+          const double2 z = c0[b * LM];
+          sm.v[b][3] += z.x * x[b].x + z.y * x[b].y;
+          sm.v[b][4] += z.x * x[b].y - z.y * x[b].x;
+        }
+      }
     }
   }
 }
 
-// Sums 2*NB per-thread energy values over the block and stores them to
-// part_cor[(2b + f) * units + u]; red must hold 4 * 2 * NB doubles.
-template <int NB>
-__device__ __forceinline__ void store_energy_partials(const JobDev &jb, int u, const double (&ce)[NB],
-                                                      const double (&re)[NB], double *red) {
+// Sums the NF NB per-thread values over the block and stores them to
+// part_cor[(5 b + f) * units + u]; red must hold 4 * NF * NB doubles.
+template <int NB, int NF>
+__device__ __forceinline__ void store_fbr_partials(const JobDev &jb, int u, const FbrSums<NB, NF> &sm, double *red) {
+  constexpr int F = NF, NV = F * NB;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int q = 0; q < 2 * NB; q++) {
-    double x = (q & 1) ? re[q >> 1] : ce[q >> 1];
+  for (int q = 0; q < NV; q++) {
+    double x = sm.v[q / F][q % F];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-    if (lane == 0) red[wave * 2 * NB + q] = x;
+    if (lane == 0) red[wave * NV + q] = x;
   }
   __syncthreads();
-  if (threadIdx.x < 2 * jb.nO) {
-    const int q = threadIdx.x;   // q = 2 b + f
-    const double sum = (q < 2 * NB) ? (red[q] + red[2 * NB + q]) + (red[4 * NB + q] + red[6 * NB + q]) : 0.0;
-    jb.part_cor[(long)q * jb.units[KK_FFT_R] + u] = sum;
+  if (threadIdx.x < F * jb.nO) {
+    const int q = threadIdx.x;   // q = F b + f, stored at field f of block b ([b][5][units])
+    const double sum = (q < NV) ? (red[q] + red[NV + q]) + (red[2 * NV + q] + red[3 * NV + q]) : 0.0;
+    jb.part_cor[(long)(5 * (q / F) + q % F) * jb.units[KK_FFT_R] + u] = sum;
   }
 }
 

@@ -73,14 +73,15 @@ __host__ __device__ constexpr int ct_cols(int n2) { return n2 <= 64 ? 16 : 8; }
 // R pass with the Coriolis mix (forward: mix then DFT; inverse: DFT, scale,
 // energies, mix)
 // ---------------------------------------------------------------------------
-template <bool INV, int NB, int N1>
+template <bool INV, int NB, int N1, bool ACF>
 __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
+  constexpr int NF = ACF ? 5 : 3;
   constexpr int P = ct_rows(N1);
   constexpr int PN = P * N1;
   constexpr int PPT = (PN + kBlock - 1) / kBlock;
   __shared__ double2 A[PN];
   __shared__ double2 tw[N1];
-  __shared__ double red[4 * 2 * NB];
+  __shared__ double red[4 * NF * NB];
   const int job = locate_job(a.unit_base, a.n_jobs, blockIdx.x);
   const JobDev &jb = a.jobs[job];
   const int u = blockIdx.x - a.unit_base[job];
@@ -89,20 +90,22 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
   const int groups = jb.n2 / P;
   const int l = jb.omin + u / groups, g = u % groups;
   const int nb = min(jb.kmax, l) - jb.omin + 1;
-  const double *V = jb.corV + (long)l * jb.nO * jb.nO;
-  double2 *base = jb.fbr + (long)l * M + (long)g * PN;
+  const QD_G double *V = jb.corV + (long)l * jb.nO * jb.nO;
+  QD_G double2 *base = jb.fbr + (long)l * M + (long)g * PN;
+  const QD_G double2 *c0 = ACF ? jb.c0 + (long)l * M + (long)g * PN : nullptr;
   for (int e = threadIdx.x; e < N1; e += kBlock) tw[e] = jb.twR[e];
   double2 x[NB][PPT];
-  double ce[NB], re[NB];
-#pragma unroll
-  for (int b = 0; b < NB; b++) { ce[b] = 0.0; re[b] = 0.0; }
+  FbrSums<NB, NF> sm;
+  sm.zero();
   if (!INV) {
 #pragma unroll
     for (int b = 0; b < NB; b++)
 #pragma unroll
       for (int k = 0; k < PPT; k++) {
         const int e = threadIdx.x + k * kBlock;
-        x[b][k] = (b < nb && (PN % kBlock == 0 || e < PN)) ? base[b * LM + e] : make_double2(0.0, 0.0);
+        // unconditional load from an in-bounds address (block nb-1, last point), zeroed if unused
+        const double2 v = base[(b < nb ? b : nb - 1) * LM + (PN % kBlock == 0 || e < PN ? e : PN - 1)];
+        x[b][k] = (b < nb && (PN % kBlock == 0 || e < PN)) ? v : make_double2(0.0, 0.0);
       }
 #pragma unroll
     for (int k = 0; k < PPT; k++) {
@@ -111,7 +114,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
         double2 xp[NB];
 #pragma unroll
         for (int b = 0; b < NB; b++) xp[b] = x[b][k];
-        coriolis_point<false, NB>(jb, xp, nb, l, e % N1, g * P + e / N1, V, ce, re);
+        coriolis_point<false, NB, NF>(jb, xp, nb, l, e % N1, g * P + e / N1, V, sm, nullptr, LM);
 #pragma unroll
         for (int b = 0; b < NB; b++) x[b][k] = xp[b];
       }
@@ -135,13 +138,23 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
     }
   } else {
     const double s = 1.0 / (double)M;
+    // all global loads in flight at once; blocks then pass through LDS in turn
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+#pragma unroll
+      for (int k = 0; k < PPT; k++) {
+        const int e = threadIdx.x + k * kBlock;
+        // unconditional load from an in-bounds address (block nb-1, last point), zeroed if unused
+        const double2 v = base[(b < nb ? b : nb - 1) * LM + (PN % kBlock == 0 || e < PN ? e : PN - 1)];
+        x[b][k] = (b < nb && (PN % kBlock == 0 || e < PN)) ? v : make_double2(0.0, 0.0);
+      }
 #pragma unroll
     for (int b = 0; b < NB; b++) {
       if (b < nb) {
 #pragma unroll
         for (int k = 0; k < PPT; k++) {
           const int e = threadIdx.x + k * kBlock;
-          if (PN % kBlock == 0 || e < PN) A[e] = base[b * LM + e];
+          if (PN % kBlock == 0 || e < PN) A[e] = x[b][k];
         }
         __syncthreads();
         stages_ip<true, N1, 1, P, N1, 1, false>(A, tw);
@@ -162,13 +175,13 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
         double2 xp[NB];
 #pragma unroll
         for (int b = 0; b < NB; b++) xp[b] = x[b][k];
-        coriolis_point<true, NB>(jb, xp, nb, l, e % N1, g * P + e / N1, V, ce, re);
+        coriolis_point<true, NB, NF>(jb, xp, nb, l, e % N1, g * P + e / N1, V, sm, ACF ? c0 + e : nullptr, LM);
 #pragma unroll
         for (int b = 0; b < NB; b++)
           if (b < nb) base[b * LM + e] = xp[b];
       }
     }
-    store_energy_partials<NB>(jb, u, ce, re, red);
+    store_fbr_partials<NB, NF>(jb, u, sm, red);
   }
 }
 
@@ -188,7 +201,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_r_ct(StepArgs a) {
   const int n1 = jb.n1;
   int b, l, g;
   decode_slab_unit(jb, u, n1 / Q, b, l, g);
-  double2 *base = jb.fbr + ((long)b * jb.L + l) * jb.M + (long)g * Q;
+  QD_G double2 *base = jb.fbr + ((long)b * jb.L + l) * jb.M + (long)g * Q;
   for (int e = threadIdx.x; e < N2; e += kBlock) tw[e] = jb.twr[e];
   for (int e = threadIdx.x; e < QN; e += kBlock) A[e] = base[(long)(e / Q) * n1 + (e % Q)];
   __syncthreads();
@@ -213,22 +226,24 @@ __global__ __launch_bounds__(kBlock) void k_fft_r_ct(StepArgs a) {
 // ---------------------------------------------------------------------------
 // launchers for one compile-time R length (instantiated in kernels_fft_ct*.hip)
 // ---------------------------------------------------------------------------
-template <bool INV, int N1>
+template <bool INV, int N1, bool ACF>
 void launch_R_nb(const StepArgs &a, int units, int nb_max, hipStream_t s) {
   const dim3 g(units), t(kBlock);
-  if (nb_max <= 1) hipLaunchKernelGGL((k_fft_R_ct<INV, 1, N1>), g, t, 0, s, a);
-  else if (nb_max <= 2) hipLaunchKernelGGL((k_fft_R_ct<INV, 2, N1>), g, t, 0, s, a);
-  else if (nb_max <= 4) hipLaunchKernelGGL((k_fft_R_ct<INV, 4, N1>), g, t, 0, s, a);
-  else if (nb_max <= 6) hipLaunchKernelGGL((k_fft_R_ct<INV, 6, N1>), g, t, 0, s, a);
-  else if (nb_max <= 9) hipLaunchKernelGGL((k_fft_R_ct<INV, 9, N1>), g, t, 0, s, a);
-  else if (nb_max <= 16) hipLaunchKernelGGL((k_fft_R_ct<INV, 16, N1>), g, t, 0, s, a);
-  else hipLaunchKernelGGL((k_fft_R_ct<INV, 24, N1>), g, t, 0, s, a);
+  if (nb_max <= 1) hipLaunchKernelGGL((k_fft_R_ct<INV, 1, N1, ACF>), g, t, 0, s, a);
+  else if (nb_max <= 2) hipLaunchKernelGGL((k_fft_R_ct<INV, 2, N1, ACF>), g, t, 0, s, a);
+  else if (nb_max <= 3) hipLaunchKernelGGL((k_fft_R_ct<INV, 3, N1, ACF>), g, t, 0, s, a);
+  else if (nb_max <= 5) hipLaunchKernelGGL((k_fft_R_ct<INV, 5, N1, ACF>), g, t, 0, s, a);
+  else if (nb_max <= 7) hipLaunchKernelGGL((k_fft_R_ct<INV, 7, N1, ACF>), g, t, 0, s, a);
+  else if (nb_max <= 9) hipLaunchKernelGGL((k_fft_R_ct<INV, 9, N1, ACF>), g, t, 0, s, a);
+  else if (nb_max <= 16) hipLaunchKernelGGL((k_fft_R_ct<INV, 16, N1, ACF>), g, t, 0, s, a);
+  else hipLaunchKernelGGL((k_fft_R_ct<INV, 24, N1, ACF>), g, t, 0, s, a);
 }
 
 #define QD_FFT_CT_DEFINE(N1)                                                                   \
-  void launch_fft_R_ct_##N1(const StepArgs &a, int units, int inverse, int nb_max, hipStream_t s) { \
-    if (inverse) launch_R_nb<true, N1>(a, units, nb_max, s);                                   \
-    else launch_R_nb<false, N1>(a, units, nb_max, s);                                          \
+  void launch_fft_R_ct_##N1(const StepArgs &a, int units, int inverse, int acf, int nb_max, hipStream_t s) { \
+    if (!inverse) launch_R_nb<false, N1, false>(a, units, nb_max, s);                          \
+    else if (acf) launch_R_nb<true, N1, true>(a, units, nb_max, s);                            \
+    else launch_R_nb<true, N1, false>(a, units, nb_max, s);                                    \
   }                                                                                            \
   void launch_fft_r_ct_##N1(const StepArgs &a, int units, hipStream_t s) {                     \
     hipLaunchKernelGGL(k_fft_r_ct<N1>, dim3(units), dim3(kBlock), 0, s, a);                    \

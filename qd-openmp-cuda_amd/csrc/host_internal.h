@@ -62,12 +62,15 @@ class DeviceJob {
   size_t footprint = 0;  // bytes of device memory held
   int records_done = 0;  // records drained to the host
   int steps = 0;         // host mirror of the device step counter
+  bool prephased = false;// step-0 V(-dt/2) phase applied
   std::vector<double> obs;      // [records][nO][5]
   std::vector<double> acf;      // [records][2]
 
   // copy the ring records not yet drained (requires the stream to be idle)
   void drain(int records_total);
   void copy_psi(double *dst, hipStream_t s) const;
+  // before the first step of a job whose step counter is 0: apply exp(+i dt/2 V)
+  void prephase_if_first(hipStream_t s);
   void flux_to_crp(const qd_inputs *in, double *crp) const;
 
   static size_t estimate_footprint(const JobShape &s, int n_energies);
@@ -103,7 +106,7 @@ class Engine {
   size_t lds_fftR_ = 0, lds_fftr_ = 0;
   int ct_n1_ = 0, ct_n2_ = 0;   // lengths of the compile-time FFT kernels, 0 = runtime path
   int fwd_tiles_ = 0, bwd_tiles_ = 0, fwd_kp_ = 0, bwd_kp_ = 0, nb_max_ = 1;
-  int mixed_ = 0, any_flux_ = 0;
+  int mixed_ = 0, any_flux_ = 0, any_acf_ = 0;
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t graph_exec_ = nullptr;
   StepArgs args(int kind) const;

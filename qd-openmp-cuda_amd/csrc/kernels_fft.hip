@@ -78,19 +78,22 @@ __device__ double2 *fft_lds(double2 *A, double2 *B, int n, int npen, int pstride
 // tiles of all blocks sit in LDS so the Coriolis mix and the R transforms
 // share them.
 // ---------------------------------------------------------------------------
-template <bool INV, int NB>
+template <bool INV, int NB, bool ACF>
 __global__ __launch_bounds__(kBlock) void k_fft_R(StepArgs a) {
+  constexpr int NF = ACF ? 5 : 3;
   extern __shared__ double2 lds[];
-  __shared__ double red[4 * 2 * NB];
+  __shared__ double red[4 * NF * NB];
   const int job = locate_job(a.unit_base, a.n_jobs, blockIdx.x);
   const JobDev &jb = a.jobs[job];
   const int u = blockIdx.x - a.unit_base[job];
   const int n1 = jb.n1, P = jb.fft_rows, M = jb.M, L = jb.L;
+  const long LM = (long)L * M;
   const int groups = jb.n2 / P;
   const int l = jb.omin + u / groups, g = u % groups;
   const int nb = min(jb.kmax, l) - jb.omin + 1;
   const int PN = P * n1;
-  const double *V = jb.corV + (long)l * jb.nO * jb.nO;
+  const QD_G double *V = jb.corV + (long)l * jb.nO * jb.nO;
+  const QD_G double2 *c0 = ACF ? jb.c0 + (long)l * M + (long)g * PN : nullptr;
   double2 *X = lds, *S = lds + nb * PN, *tw = S + PN;
   for (int e = threadIdx.x; e < n1; e += kBlock) tw[e] = jb.twR[e];
   for (int b = 0; b < nb; b++) {
@@ -98,15 +101,14 @@ __global__ __launch_bounds__(kBlock) void k_fft_R(StepArgs a) {
     for (int e = threadIdx.x; e < PN; e += kBlock) X[b * PN + e] = src[e];
   }
   __syncthreads();
-  double ce[NB], re[NB];
-#pragma unroll
-  for (int b = 0; b < NB; b++) { ce[b] = 0.0; re[b] = 0.0; }
+  FbrSums<NB, NF> sm;
+  sm.zero();
   if (!INV) {
     for (int e = threadIdx.x; e < PN; e += kBlock) {
       double2 x[NB];
 #pragma unroll
       for (int b = 0; b < NB; b++) x[b] = (b < nb) ? X[b * PN + e] : make_double2(0.0, 0.0);
-      coriolis_point<false, NB>(jb, x, nb, l, e % n1, g * P + e / n1, V, ce, re);
+      coriolis_point<false, NB, NF>(jb, x, nb, l, e % n1, g * P + e / n1, V, sm, nullptr, LM);
 #pragma unroll
       for (int b = 0; b < NB; b++)
         if (b < nb) X[b * PN + e] = x[b];
@@ -126,7 +128,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_R(StepArgs a) {
       double2 x[NB];
 #pragma unroll
       for (int b = 0; b < NB; b++) x[b] = (b < nb) ? cscale(X[b * PN + e], s) : make_double2(0.0, 0.0);
-      coriolis_point<true, NB>(jb, x, nb, l, e % n1, g * P + e / n1, V, ce, re);
+      coriolis_point<true, NB, NF>(jb, x, nb, l, e % n1, g * P + e / n1, V, sm, ACF ? c0 + e : nullptr, LM);
 #pragma unroll
       for (int b = 0; b < NB; b++)
         if (b < nb) X[b * PN + e] = x[b];
@@ -137,16 +139,18 @@ __global__ __launch_bounds__(kBlock) void k_fft_R(StepArgs a) {
     double2 *dst = jb.fbr + ((long)b * L + l) * M + (long)g * PN;
     for (int e = threadIdx.x; e < PN; e += kBlock) dst[e] = X[b * PN + e];
   }
-  if (INV) store_energy_partials<NB>(jb, u, ce, re, red);
+  if (INV) store_fbr_partials<NB, NF>(jb, u, sm, red);
 }
 
-void launch_fft_R_rt(const StepArgs &a, int total_units, int inverse, int nb_max, size_t lds, hipStream_t stream) {
+void launch_fft_R_rt(const StepArgs &a, int total_units, int inverse, int acf, int nb_max, size_t lds,
+                     hipStream_t stream) {
   const dim3 g(total_units), t(kBlock);
-#define QD_FFTR_CASE(NB)                                                              \
-  if (nb_max <= NB) {                                                                 \
-    if (inverse) hipLaunchKernelGGL((k_fft_R<true, NB>), g, t, lds, stream, a);       \
-    else hipLaunchKernelGGL((k_fft_R<false, NB>), g, t, lds, stream, a);              \
-    return;                                                                           \
+#define QD_FFTR_CASE(NB)                                                                    \
+  if (nb_max <= NB) {                                                                       \
+    if (!inverse) hipLaunchKernelGGL((k_fft_R<false, NB, false>), g, t, lds, stream, a);    \
+    else if (acf) hipLaunchKernelGGL((k_fft_R<true, NB, true>), g, t, lds, stream, a);      \
+    else hipLaunchKernelGGL((k_fft_R<true, NB, false>), g, t, lds, stream, a);              \
+    return;                                                                                 \
   }
   QD_FFTR_CASE(1)
   QD_FFTR_CASE(2)
@@ -221,9 +225,10 @@ void launch_fft_probe(double2 *data, const double2 *tw, int n, int count, int in
 
 void init_attrs_fft() {
   const int lim = 160 * 1024;
-#define QD_FFTR_ATTR(NB)                                                                                      \
-  (void)hipFuncSetAttribute((const void *)k_fft_R<false, NB>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
-  (void)hipFuncSetAttribute((const void *)k_fft_R<true, NB>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+#define QD_FFTR_ATTR(NB)                                                                                             \
+  (void)hipFuncSetAttribute((const void *)k_fft_R<false, NB, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
+  (void)hipFuncSetAttribute((const void *)k_fft_R<true, NB, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);  \
+  (void)hipFuncSetAttribute((const void *)k_fft_R<true, NB, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
   QD_FFTR_ATTR(1) QD_FFTR_ATTR(2) QD_FFTR_ATTR(4) QD_FFTR_ATTR(6) QD_FFTR_ATTR(9) QD_FFTR_ATTR(12)
   QD_FFTR_ATTR(16) QD_FFTR_ATTR(24)
 #undef QD_FFTR_ATTR

@@ -8,19 +8,19 @@ QD_FFT_CT_DEFINE(32)
 QD_FFT_CT_DEFINE(48)
 QD_FFT_CT_DEFINE(64)
 
-void launch_fft_R_ct_32(const StepArgs &a, int units, int inverse, int nb_max, hipStream_t s);
+void launch_fft_R_ct_32(const StepArgs &a, int units, int inverse, int acf, int nb_max, hipStream_t s);
 void launch_fft_r_ct_32(const StepArgs &a, int units, hipStream_t s);
-void launch_fft_R_ct_48(const StepArgs &a, int units, int inverse, int nb_max, hipStream_t s);
+void launch_fft_R_ct_48(const StepArgs &a, int units, int inverse, int acf, int nb_max, hipStream_t s);
 void launch_fft_r_ct_48(const StepArgs &a, int units, hipStream_t s);
-void launch_fft_R_ct_64(const StepArgs &a, int units, int inverse, int nb_max, hipStream_t s);
+void launch_fft_R_ct_64(const StepArgs &a, int units, int inverse, int acf, int nb_max, hipStream_t s);
 void launch_fft_r_ct_64(const StepArgs &a, int units, hipStream_t s);
-void launch_fft_R_ct_96(const StepArgs &a, int units, int inverse, int nb_max, hipStream_t s);
+void launch_fft_R_ct_96(const StepArgs &a, int units, int inverse, int acf, int nb_max, hipStream_t s);
 void launch_fft_r_ct_96(const StepArgs &a, int units, hipStream_t s);
-void launch_fft_R_ct_128(const StepArgs &a, int units, int inverse, int nb_max, hipStream_t s);
+void launch_fft_R_ct_128(const StepArgs &a, int units, int inverse, int acf, int nb_max, hipStream_t s);
 void launch_fft_r_ct_128(const StepArgs &a, int units, hipStream_t s);
-void launch_fft_R_ct_192(const StepArgs &a, int units, int inverse, int nb_max, hipStream_t s);
+void launch_fft_R_ct_192(const StepArgs &a, int units, int inverse, int acf, int nb_max, hipStream_t s);
 void launch_fft_r_ct_192(const StepArgs &a, int units, hipStream_t s);
-void launch_fft_R_ct_256(const StepArgs &a, int units, int inverse, int nb_max, hipStream_t s);
+void launch_fft_R_ct_256(const StepArgs &a, int units, int inverse, int acf, int nb_max, hipStream_t s);
 void launch_fft_r_ct_256(const StepArgs &a, int units, hipStream_t s);
 
 bool fft_ct_supported(int n) {
@@ -32,9 +32,9 @@ bool fft_ct_supported(int n) {
 int fft_ct_rows(int n1) { return ct_rows(n1); }
 int fft_ct_cols(int n2) { return ct_cols(n2); }
 
-void launch_fft_R_ct(const StepArgs &a, int total_units, int inverse, int n1, int nb_max, hipStream_t s) {
+void launch_fft_R_ct(const StepArgs &a, int total_units, int inverse, int acf, int n1, int nb_max, hipStream_t s) {
   switch (n1) {
-#define QD_R_CASE(N) case N: launch_fft_R_ct_##N(a, total_units, inverse, nb_max, s); break;
+#define QD_R_CASE(N) case N: launch_fft_R_ct_##N(a, total_units, inverse, acf, nb_max, s); break;
     QD_R_CASE(32) QD_R_CASE(48) QD_R_CASE(64) QD_R_CASE(96) QD_R_CASE(128) QD_R_CASE(192) QD_R_CASE(256)
 #undef QD_R_CASE
     default: break;

@@ -1,4 +1,5 @@
-// kernels_gemm.hip - Legendre DVR <-> FBR transforms on the matrix cores.
+// kernels_gemm.hip - Legendre DVR <-> FBR transforms on the matrix cores,
+// with the DVR pointwise work fused into their operand load and epilogue.
 //
 // Reference: OmegaWavepacket::forward_legendre_transform / backward_...
 // (omegawavepacket.cu:200-240) issue cublasZgemm with a complex B whose
@@ -11,6 +12,17 @@
 // v_mfma_f64_16x16x4f64; the operands are issued as (B^T, A^T) so the 16x16
 // accumulator tile holds C^T and every store instruction writes 16
 // consecutive rows (128 B) per column.  B is staged once per workgroup in LDS.
+//
+// Forward prologue: each A element (re or im of psi at (R_i, r_j, theta_k))
+// is multiplied by d1[i] d2[j] exp(-i dt V) before it enters the MFMA; the
+// re/im partner sits in the neighbouring lane.  (The extra exp(+i dt/2 V) of
+// step 0 is applied once by k_prephase before the first step.)
+// This is the absorber and potential phase of cudaOpenMP2.cu:342-346
+// (evolutionCUDAaux.cu:5-11, :243-252), which the reference runs as separate
+// passes over psi.
+// Backward epilogue: the per-workgroup partial sums of w_k V |psi|^2, the
+// potential energy of omegawavepacket.cu:101-132, from the accumulators (the
+// norm and autocorrelation come from the FBR coefficients in the FFT pass).
 //
 // The mixed form splits both operands into FP32 hi/lo parts and accumulates
 // hi*hi and hi*lo + lo*hi with v_mfma_f32_16x16x4f32 in FP32, flushing the FP32
@@ -25,117 +37,175 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 
 __host__ __device__ constexpr int lds_stride(int nt_tiles) { return nt_tiles * 16 + ((nt_tiles & 1) ? 0 : 16); }
 
-// C[r, c] = sum_k A[r, k] Bs[k][c] for rows r0..r0+15, columns 0..N-1
-template <int NT>
-__device__ __forceinline__ void strip_f64(const double *__restrict__ A, long lda, int K, int ksteps,
-                                          const double *__restrict__ Bs, int Ws, double *__restrict__ C,
-                                          long ldc, int N, int r0, int rows) {
-  const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
-  const int r = r0 + lr;
-  const bool rok = r < rows;
-  v4d acc[NT];
-#pragma unroll
-  for (int t = 0; t < NT; t++) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
-  double an[4];
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    const int k = 4 * q + lk;
-    an[q] = (rok && k < K) ? A[(long)k * lda + r] : 0.0;
-  }
-  for (int s0 = 0; s0 < ksteps; s0 += 4) {
-    double ac[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) ac[q] = an[q];
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int k = 4 * (s0 + 4 + q) + lk;
-      an[q] = (rok && k < K) ? A[(long)k * lda + r] : 0.0;
-    }
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      if (s0 + q < ksteps) {
-        const double *brow = Bs + (4 * (s0 + q) + lk) * Ws + lr;
-#pragma unroll
-        for (int t = 0; t < NT; t++)
-          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(brow[t * 16], ac[q], acc[t], 0, 0, 0);
-      }
-    }
-  }
-  // v_mfma_f64_16x16x4f64 accumulator layout: item it of lane (lr, lk) holds
-  // D[lk + 4 it][lr], i.e. C[r0 + lr][t*16 + lk + 4 it]
-  if (rok) {
-#pragma unroll
-    for (int t = 0; t < NT; t++) {
-#pragma unroll
-      for (int it = 0; it < 4; it++) {
-        const int c = t * 16 + lk + 4 * it;
-        if (c < N) C[(long)c * ldc + r] = acc[t][it];
-      }
-    }
-  }
+constexpr int kCh = 8;   // k-steps (of 4 k each) per pipelined tile
+
+// One wave's view of a Legendre GEMM: C (rows x N, ldc) = op(A) (rows x K, lda) . B.
+struct GemmWave {
+  const double *A;
+  long lda;
+  int K, ksteps;
+  double *C;
+  long ldc;
+  int N;
+  long rows;
+  const double *Bs;              // FP64 B in LDS, [kp][Ws]
+  const float *Bh, *Bl;          // FP32 hi/lo B in LDS (mixed form)
+  int Ws;
+  const QD_G double *vph;        // forward: phase table viewed as real (2M x nt), same indexing as A
+  const double *ws;              // backward epilogue: quadrature weights in LDS
+  const QD_G double *pot;        // backward epilogue: potential [nt][M]
+  long M;
+};
+
+// Value of the neighbouring lane (lane ^ 1) through a DPP quad permutation
+// (1, 0, 3, 2): a VALU move, no LDS round trip.
+__device__ __forceinline__ double swap_adjacent(double x) {
+  const long long v = __double_as_longlong(x);
+  int lo = (int)(v & 0xffffffffLL), hi = (int)(v >> 32);
+  lo = __builtin_amdgcn_mov_dpp(lo, 0xB1, 0xF, 0xF, false);
+  hi = __builtin_amdgcn_mov_dpp(hi, 0xB1, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
-// mixed precision: hi/lo split FP32 products, FP32 accumulation flushed to FP64 every 4 k-steps
-template <int NT>
-__device__ __forceinline__ void strip_mixed(const double *__restrict__ A, long lda, int K, int ksteps,
-                                            const float *__restrict__ Bh, const float *__restrict__ Bl, int Ws,
-                                            double *__restrict__ C, long ldc, int N, int r0, int rows) {
+// Column of accumulator item `it` of tile t for this lane.
+// v_mfma_f64_16x16x4f64: item it of lane (lr, lk) holds D[lk + 4 it][lr];
+// v_mfma_f32_16x16x4f32: item it holds D[4 lk + it][lr] (D = C^T tile).
+template <bool MIXED>
+__device__ __forceinline__ int acc_col(int t, int lk, int it) {
+  return MIXED ? t * 16 + 4 * lk + it : t * 16 + lk + 4 * it;
+}
+
+// Computes the strips strip0 + s * sstride (s < nstrips) of one wave.  The
+// (strip, k-chunk) tiles are software-pipelined: the loads of the next tile
+// (kCh A values and, forward, kCh phase components per lane) are in flight
+// while the current tile runs on the matrix cores.
+//   PHASE: multiply each A element by d1 d2 exp(-i dt V) (forward prologue);
+//          a lane holds one real component, its partner lane (r ^ 1) the other.
+//   OBS:   accumulate w V |psi|^2 of the output (backward epilogue) into ob.
+template <int NT, bool MIXED, bool PHASE, bool OBS>
+__device__ __forceinline__ void gemm_wave(const GemmWave &gw, long strip0, int nstrips, int sstride, double &ob) {
   const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
-  const int r = r0 + lr;
-  const bool rok = r < rows;
+  const bool odd = lr & 1;
+  const int nch = (gw.ksteps + kCh - 1) / kCh;
+  int ns = 0;
+  for (int s = 0; s < nstrips; s++)
+    if ((strip0 + (long)s * sstride) * 16 < gw.rows) ns = s + 1;
+  const int ntiles = ns * nch;
+  double an[kCh], anx[kCh], tn[kCh], tnx[kCh];
+  // Loads are unconditional from clamped (always in-bounds) addresses and the
+  // out-of-range values are zeroed afterwards: without branches around the
+  // loads the compiler can count them and wait only for the older tile.
+  auto load = [&](int tile, double(&a)[kCh], double(&tp)[kCh]) {
+    const int s = tile / nch, c = tile % nch;
+    const long r = (strip0 + (long)s * sstride) * 16 + lr;
+    const bool rok = r < gw.rows;
+    const long rc = rok ? r : gw.rows - 1;
+#pragma unroll
+    for (int q = 0; q < kCh; q++) {
+      const int k = 4 * (c * kCh + q) + lk;
+      const bool ok = rok && k < gw.K;
+      const long off = (long)min(k, gw.K - 1) * gw.lda + rc;
+      const double av = gw.A[off];
+      a[q] = ok ? av : 0.0;
+      // even lane takes Re T, odd lane Im T of the same complex phase
+      if (PHASE) {
+        const double tv = gw.vph[off];
+        tp[q] = ok ? tv : (odd ? 0.0 : 1.0);
+      }
+    }
+  };
   v4d acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; t++) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
-  double an[4];
+  if (ntiles > 0) load(0, an, tn);
+  for (int tile = 0; tile < ntiles; tile++) {
+    if (tile + 1 < ntiles) load(tile + 1, anx, tnx);
+    const int c = tile % nch;
+    if (c == 0) {
 #pragma unroll
-  for (int q = 0; q < 4; q++) {
-    const int k = 4 * q + lk;
-    an[q] = (rok && k < K) ? A[(long)k * lda + r] : 0.0;
-  }
-  for (int s0 = 0; s0 < ksteps; s0 += 4) {
-    float ah[4], al[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      ah[q] = (float)an[q];
-      al[q] = (float)(an[q] - (double)ah[q]);
+      for (int t = 0; t < NT; t++) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
     }
+    double ac[kCh];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int k = 4 * (s0 + 4 + q) + lk;
-      an[q] = (rok && k < K) ? A[(long)k * lda + r] : 0.0;
+    for (int q = 0; q < kCh; q++) {
+      if (PHASE) {
+        const double oa = swap_adjacent(an[q]);
+        const double ot = swap_adjacent(tn[q]);
+        const double Tr = odd ? ot : tn[q], Ti = odd ? tn[q] : ot;
+        // re' = re Tr - im Ti (even lane), im' = re Ti + im Tr (odd lane)
+        ac[q] = fma(odd ? oa : -oa, Ti, an[q] * Tr);
+      } else {
+        ac[q] = an[q];
+      }
     }
-    v4f hh[NT], hl[NT];
+    if (!MIXED) {
 #pragma unroll
-    for (int t = 0; t < NT; t++) { hh[t] = v4f{0.f, 0.f, 0.f, 0.f}; hl[t] = v4f{0.f, 0.f, 0.f, 0.f}; }
+      for (int q = 0; q < kCh; q++) {
+        if (c * kCh + q < gw.ksteps) {
+          const double *brow = gw.Bs + (4 * (c * kCh + q) + lk) * gw.Ws + lr;
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      if (s0 + q < ksteps) {
-        const int off = (4 * (s0 + q) + lk) * Ws + lr;
+          for (int t = 0; t < NT; t++)
+            acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(brow[t * 16], ac[q], acc[t], 0, 0, 0);
+        }
+      }
+    } else {
+      // FP32 hi/lo products, FP32 accumulation over 4 k-steps, flushed to FP64
 #pragma unroll
-        for (int t = 0; t < NT; t++) {
-          const float bh = Bh[off + t * 16], bl = Bl[off + t * 16];
-          hh[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(bh, ah[q], hh[t], 0, 0, 0);
-          hl[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(bl, ah[q], hl[t], 0, 0, 0);
-          hl[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(bh, al[q], hl[t], 0, 0, 0);
+      for (int q0 = 0; q0 < kCh; q0 += 4) {
+        v4f hh[NT], hl[NT];
+#pragma unroll
+        for (int t = 0; t < NT; t++) { hh[t] = v4f{0.f, 0.f, 0.f, 0.f}; hl[t] = v4f{0.f, 0.f, 0.f, 0.f}; }
+#pragma unroll
+        for (int q = q0; q < q0 + 4; q++) {
+          if (c * kCh + q < gw.ksteps) {
+            const float ah = (float)ac[q];
+            const float al = (float)(ac[q] - (double)ah);
+            const int off = (4 * (c * kCh + q) + lk) * gw.Ws + lr;
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+              const float bh = gw.Bh[off + t * 16], bl = gw.Bl[off + t * 16];
+              hh[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(bh, ah, hh[t], 0, 0, 0);
+              hl[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(bl, ah, hl[t], 0, 0, 0);
+              hl[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(bh, al, hl[t], 0, 0, 0);
+            }
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+          for (int it = 0; it < 4; it++) acc[t][it] += (double)hh[t][it] + (double)hl[t][it];
+      }
+    }
+    if (c == nch - 1) {   // epilogue of strip tile / nch
+      const long r = (strip0 + (long)(tile / nch) * sstride) * 16 + lr;
+      const bool rok = r < gw.rows;
+      const long m = (rok ? r : gw.rows - 1) >> 1;
+      double vq[NT][4];
+      if (OBS) {   // issue every potential load before using any of them
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+          for (int it = 0; it < 4; it++) {
+            const int cc = min(acc_col<MIXED>(t, lk, it), gw.N - 1);
+            vq[t][it] = gw.pot[(long)cc * gw.M + m];
+          }
+      }
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+#pragma unroll
+        for (int it = 0; it < 4; it++) {
+          const int col = acc_col<MIXED>(t, lk, it);
+          const bool ok = rok && col < gw.N;
+          const double v = acc[t][it];
+          if (ok) gw.C[(long)col * gw.ldc + r] = v;
+          if (OBS) ob += (ok ? gw.ws[min(col, gw.N - 1)] * v * v : 0.0) * vq[t][it];
         }
       }
     }
 #pragma unroll
-    for (int t = 0; t < NT; t++) {
-#pragma unroll
-      for (int it = 0; it < 4; it++) acc[t][it] += (double)hh[t][it] + (double)hl[t][it];
-    }
-  }
-  // v_mfma_f32_16x16x4f32 accumulator layout: item it of lane (lr, lk) holds
-  // D[4 lk + it][lr], i.e. C[r0 + lr][t*16 + 4 lk + it]
-  if (rok) {
-#pragma unroll
-    for (int t = 0; t < NT; t++) {
-#pragma unroll
-      for (int it = 0; it < 4; it++) {
-        const int c = t * 16 + 4 * lk + it;
-        if (c < N) C[(long)c * ldc + r] = acc[t][it];
-      }
+    for (int q = 0; q < kCh; q++) {
+      an[q] = anx[q];
+      if (PHASE) tn[q] = tnx[q];
     }
   }
 }
@@ -143,53 +213,64 @@ __device__ __forceinline__ void strip_mixed(const double *__restrict__ A, long l
 template <int NT, bool FWD, bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_leg(StepArgs a) {
   extern __shared__ double lds_d[];
+  __shared__ double red[4];
+  __shared__ double w_s[FWD ? 1 : kMaxTheta];
   const int job = locate_job(a.unit_base, a.n_jobs, blockIdx.x);
   const JobDev &jb = a.jobs[job];
   const int u = blockIdx.x - a.unit_base[job];
   const int groups = FWD ? jb.gemm_fwd_groups : jb.gemm_bwd_groups;
-  const int b = u / groups, g = u % groups;
+  if (!FWD)
+    for (int e = threadIdx.x; e < jb.nt; e += kBlock) w_s[e] = jb.w[e];
+  // consecutive workgroups take the same rows of consecutive blocks, so the
+  // per-row tables (phase, potential) are reused from L2
+  const int b = u % jb.nO, g = u / jb.nO;
   const int om = jb.omin + b, nl = jb.lmax - om + 1;
-  const long G = (long)jb.M * jb.nt;
-  const long slab0 = ((long)b * jb.L + om) * jb.M;
-  const double *A = FWD ? (const double *)(jb.psi + b * G) : (const double *)(jb.fbr + slab0);
-  double *C = FWD ? (double *)(jb.fbr + slab0) : (double *)(jb.psi + b * G);
-  const int K = FWD ? jb.nt : nl;
-  const int N = FWD ? nl : jb.nt;
+  const long M = jb.M;
+  const long G = M * jb.nt;
+  const long slab0 = ((long)b * jb.L + om) * M;
   const int kp = FWD ? jb.fwd_kp : jb.bwd_kp;
   const int np = FWD ? jb.fwd_np : jb.bwd_np;
   constexpr int Ws = lds_stride(NT);
   constexpr int W = NT * 16;
-  const long rows = 2L * jb.M;
-  const int wave = threadIdx.x >> 6;
-  const int S = jb.gemm_strips;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double *Bs = lds_d;
+  float *Bh = (float *)lds_d, *Bl = Bh + kp * Ws;
   if (!MIXED) {
     const double *Bg = (FWD ? jb.wP : jb.Pt) + (long)b * kp * np;
-    double *Bs = lds_d;
     for (int e = threadIdx.x; e < kp * W; e += kBlock) {
       const int k = e / W, c = e % W;
       Bs[k * Ws + c] = c < np ? Bg[k * np + c] : 0.0;
     }
-    __syncthreads();
-    for (int s = 0; s < S; s++) {
-      const long r0 = ((long)(g * S + s) * 4 + wave) * 16;
-      if (r0 >= rows) break;
-      strip_f64<NT>(A, rows, K, kp / 4, Bs, Ws, C, rows, N, (int)r0, (int)rows);
-    }
   } else {
     const float *Bgh = (FWD ? jb.wP_hi : jb.Pt_hi) + (long)b * kp * np;
     const float *Bgl = (FWD ? jb.wP_lo : jb.Pt_lo) + (long)b * kp * np;
-    float *Bh = (float *)lds_d, *Bl = Bh + kp * Ws;
     for (int e = threadIdx.x; e < kp * W; e += kBlock) {
       const int k = e / W, c = e % W;
       Bh[k * Ws + c] = c < np ? Bgh[k * np + c] : 0.f;
       Bl[k * Ws + c] = c < np ? Bgl[k * np + c] : 0.f;
     }
+  }
+  __syncthreads();
+  GemmWave gw;
+  gw.A = FWD ? (const double *)(jb.psi + b * G) : (const double *)(jb.fbr + slab0);
+  gw.C = FWD ? (double *)(jb.fbr + slab0) : (double *)(jb.psi + b * G);
+  gw.rows = 2 * M;
+  gw.lda = gw.ldc = 2 * M;
+  gw.K = FWD ? jb.nt : nl;
+  gw.N = FWD ? nl : jb.nt;
+  gw.ksteps = kp / 4;
+  gw.Bs = Bs; gw.Bh = Bh; gw.Bl = Bl; gw.Ws = Ws;
+  gw.vph = (const QD_G double *)jb.vphase;
+  gw.ws = w_s; gw.pot = jb.pot; gw.M = M;
+  double ob = 0.0;
+  const int S = jb.gemm_strips;
+  gemm_wave<NT, MIXED, FWD, !FWD>(gw, (long)g * S * 4 + wave, S, 4, ob);
+  if (!FWD) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) ob += __shfl_xor(ob, off, 64);
+    if (lane == 0) red[wave] = ob;
     __syncthreads();
-    for (int s = 0; s < S; s++) {
-      const long r0 = ((long)(g * S + s) * 4 + wave) * 16;
-      if (r0 >= rows) break;
-      strip_mixed<NT>(A, rows, K, kp / 4, Bh, Bl, Ws, C, rows, N, (int)r0, (int)rows);
-    }
+    if (threadIdx.x == 0) jb.part_obs[(long)b * groups + g] = (red[0] + red[1]) + (red[2] + red[3]);
   }
 }
 
@@ -239,7 +320,12 @@ __global__ __launch_bounds__(kBlock) void k_gemm_probe(const double *A, int lda,
   }
   __syncthreads();
   const int r0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
-  if (r0 < rows) strip_f64<NT>(A, lda, K, kp / 4, Bs, Ws, C, ldc, N, r0, rows);
+  GemmWave gw{};
+  gw.A = A; gw.lda = lda; gw.K = K; gw.ksteps = kp / 4;
+  gw.C = C; gw.ldc = ldc; gw.N = N; gw.rows = rows;
+  gw.Bs = Bs; gw.Ws = Ws;
+  double ob = 0.0;
+  gemm_wave<NT, false, false, false>(gw, r0 / 16, 1, 1, ob);
 }
 
 void launch_gemm_probe(const double *A, int lda, const double *B, int kp, int np, double *C, int ldc, int rows,

@@ -110,6 +110,21 @@ JobShape validate_inputs(const qd_inputs *in) {
   QD_REQUIRE(os.associated_legendres, "associated_legendres is required");
   QD_REQUIRE(os.wave_packets, "wave_packets is required");
   s.legendre_n_max = os.legendre_n_max;
+  QD_REQUIRE(s.nt <= kMaxTheta, "theta.n must not exceed 512");
+  // The propagation relies on sum_k w_k P_l(x_k) P_l'(x_k) = delta_ll' (the
+  // forward/backward Legendre pair of omegawavepacket.cu:134-240 is then a
+  // projector, and norms can be taken on either side of it).
+  for (int b = 0; b < s.nO; b++) {
+    const int nl = s.lmax - os.omegas[b] + 1;
+    const double *P = os.associated_legendres + (size_t)b * s.nt * s.legendre_n_max;
+    for (int c = 0; c < nl; c++)
+      for (int c2 = c; c2 < nl; c2++) {
+        double acc = 0.0;
+        for (int k = 0; k < s.nt; k++) acc += in->theta.w[k] * P[(size_t)c * s.nt + k] * P[(size_t)c2 * s.nt + k];
+        QD_REQUIRE(std::fabs(acc - (c == c2 ? 1.0 : 0.0)) < 1e-9,
+                   "associated_legendres are not orthonormal under theta.w (sum_k w_k P_l P_l' != delta)");
+      }
+  }
   s.M = s.n1 * s.n2;
   s.dt = in->time.time_step;
   s.dr1 = in->r1.dr; s.dr2 = in->r2.dr; s.left1 = in->r1.left; s.left2 = in->r2.left;
@@ -118,7 +133,7 @@ JobShape validate_inputs(const qd_inputs *in) {
   s.mixed = in->precision == QD_PREC_MIXED;
   QD_REQUIRE(in->precision == QD_PREC_FP64 || in->precision == QD_PREC_MIXED, "unknown precision");
   s.steps0 = *in->time.steps;
-  s.acf = true;
+  s.acf = in->autocorrelation != nullptr;   // the initial state is kept and read every step only on request
   const qd_crp &c = in->crp;
   s.crp = c.calculate_CRP != 0 && c.n_energies > 0;
   if (s.crp) {
@@ -134,6 +149,8 @@ JobShape validate_inputs(const qd_inputs *in) {
              "steps_to_copy_psi_from_device_to_host must be > 0 when copy-back is enabled");
   // LDS limits of the Legendre GEMMs (B operand staged whole)
   const int nl_max = s.lmax - s.omin + 1;
+  QD_REQUIRE(!s.acf || round_up(nl_max, 16) <= 128,
+             "the autocorrelation needs l_max - omegas[0] + 1 <= 128");
   QD_REQUIRE(gemm_lds_bytes(round_up(s.nt, 4), round_up(nl_max, 16), s.mixed) <= kLdsLimit &&
                  gemm_lds_bytes(round_up(nl_max, 4), round_up(s.nt, 16), s.mixed) <= kLdsLimit,
              "theta.n / l_max too large for the LDS-resident Legendre tables");
@@ -180,7 +197,8 @@ DeviceJob::~DeviceJob() {
 size_t DeviceJob::estimate_footprint(const JobShape &s, int n_energies) {
   const size_t G = (size_t)s.M * s.nt;
   size_t b = 0;
-  b += 16 * G * s.nO * 2;                 // psi + psi0
+  b += 16 * G * s.nO;                    // psi
+  b += 16 * (size_t)s.M * s.L * s.nO * (s.acf ? 1 : 0);   // FBR of the initial state
   b += 16 * (size_t)s.M * s.L * s.nO;     // fbr
   b += 24 * G;                            // vphase + pot
   b += 16 * (size_t)s.L * s.n1 * s.nO * s.nO;
@@ -217,17 +235,15 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   else d.fft_rows = pick_divisor(n2, rows_c, 4, 5120 / (nO + 1), n1);
   if (fft_ct_supported(n2) && n1 % fft_ct_cols(n2) == 0) d.fft_cols = fft_ct_cols(n2);
   else d.fft_cols = pick_divisor(n1, cols_c, 5, 2048, n2);
-  d.obs_chunks = std::max(1, (M + 2047) / 2048);
-  d.fbr_chunks = (M + kBlock - 1) / kBlock;
   long slabs = 0;
   for (int b = 0; b < nO; b++) slabs += L - (sh.omin + b);
-  d.units[KK_DVR] = (int)((G + kBlock * 4 - 1) / (kBlock * 4));
+  d.units[KK_DVR] = 0;   // fused into the forward Legendre prologue
   d.units[KK_LEG_FWD] = nO * d.gemm_fwd_groups;
   d.units[KK_LEG_BWD] = nO * d.gemm_bwd_groups;
   d.units[KK_FBR] = 0;
   d.units[KK_FFT_R] = (L - sh.omin) * (n2 / d.fft_rows);
   d.units[KK_FFT_r] = (int)(slabs * (n1 / d.fft_cols));
-  d.units[KK_OBS] = nO * nt * d.obs_chunks;
+  d.units[KK_OBS] = 0;   // fused into the backward Legendre epilogue
   d.units[KK_REDUCE] = nO;
   d.units[KK_FLUX] = 0;
 
@@ -314,8 +330,8 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   QD_HIP(hipMemcpyAsync(d.psi, in->omega_states.wave_packets, 16 * G * nO, hipMemcpyHostToDevice, st));
   d.fbr = (double2 *)alloc(16 * (size_t)M * L * nO);
   QD_HIP(hipMemsetAsync(d.fbr, 0, 16 * (size_t)M * L * nO, st));
-  d.psi0 = (double2 *)alloc(16 * G * nO);
-  QD_HIP(hipMemcpyAsync(d.psi0, in->omega_states.wave_packets, 16 * G * nO, hipMemcpyHostToDevice, st));
+  d.psi0 = nullptr;
+  d.c0 = nullptr;
   d.vphase = upload(vph, st);
   d.pot = upload(pot, st);
   d.w = upload(std::vector<double>(in->theta.w, in->theta.w + nt), st);
@@ -340,9 +356,20 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   d.cmat = upload(cmat, st);
   d.twR = upload(twiddles(n1), st);
   d.twr = upload(twiddles(n2), st);
-  d.part_cor = (double *)alloc(sizeof(double) * (size_t)d.units[KK_FFT_R] * nO * 2);
+  d.part_cor = (double *)alloc(sizeof(double) * (size_t)d.units[KK_FFT_R] * nO * 5);
   d.part_ke = (double *)alloc(sizeof(double) * (size_t)d.units[KK_FFT_r]);
-  d.part_obs = (double *)alloc(sizeof(double) * (size_t)d.units[KK_OBS] * 4);
+  d.part_obs = (double *)alloc(sizeof(double) * (size_t)nO * d.gemm_bwd_groups);
+  if (sh.acf) {
+    // FBR coefficients of the initial state (psi still holds it): the
+    // autocorrelation <psi(0)|psi(t)> is formed on them in the FFT pass
+    d.c0 = (double2 *)alloc(16 * (size_t)M * L * nO);
+    for (int b = 0; b < nO; b++) {
+      const int om = sh.omin + b, nl = sh.lmax - om + 1;
+      launch_gemm_probe((const double *)(d.psi + (size_t)b * G), 2 * M, d.wP + (size_t)b * d.fwd_kp * d.fwd_np,
+                        d.fwd_kp, d.fwd_np, (double *)(d.c0 + ((size_t)b * L + om) * M), 2 * M, 2 * M, nt, nl, st);
+      QD_HIP(hipGetLastError());
+    }
+  }
   d.ring = kRing;
   d.obs_ring = (double *)alloc(sizeof(double) * (size_t)kRing * nO * 5);
   d.acf_ring = (double2 *)alloc(sizeof(double2) * (size_t)kRing * nO);
@@ -377,6 +404,13 @@ void DeviceJob::copy_psi(double *dst, hipStream_t s) const {
   const size_t bytes = 16 * (size_t)shape.M * shape.nt * shape.nO;
   QD_HIP(hipMemcpyAsync(dst, dev.psi, bytes, hipMemcpyDeviceToHost, s));
   QD_HIP(hipStreamSynchronize(s));
+}
+
+void DeviceJob::prephase_if_first(hipStream_t s) {
+  if (steps != 0 || prephased) return;
+  launch_prephase(dev.psi, dev.pot, (long)shape.M * shape.nt, shape.nO, shape.dt, s);
+  QD_HIP(hipGetLastError());
+  prephased = true;
 }
 
 void DeviceJob::flux_to_crp(const qd_inputs *, double *) const {}
@@ -423,6 +457,7 @@ void Engine::set_jobs(const std::vector<DeviceJob *> &jobs) {
   nb_max_ = 1;
   mixed_ = jobs[0]->dev.mixed;
   any_flux_ = 0;
+  any_acf_ = 0;
   for (int q = 0; q < n; q++) {
     const JobDev &d = jobs[q]->dev;
     QD_REQUIRE(d.mixed == mixed_, "all jobs of one engine must share a precision");
@@ -439,6 +474,7 @@ void Engine::set_jobs(const std::vector<DeviceJob *> &jobs) {
     bwd_kp_ = std::max(bwd_kp_, d.bwd_kp);
     nb_max_ = std::max(nb_max_, d.nO);
     any_flux_ |= d.flux_on;
+    any_acf_ |= d.c0 != nullptr;
   }
   for (int k = 0; k < KK_COUNT; k++) total_[k] = bases[(size_t)k * (n + 1) + n];
   QD_HIP(hipMemcpyAsync(d_jobs_, descs.data(), sizeof(JobDev) * n, hipMemcpyHostToDevice, stream_));
@@ -460,24 +496,22 @@ void Engine::enqueue(hipEvent_t *ev) {
     if (ev) QD_HIP(hipEventRecord(ev[q], s));
   };
   mark(0);
-  launch_dvr(args(KK_DVR), total_[KK_DVR], s);
   mark(1);
   launch_leg_fwd(args(KK_LEG_FWD), total_[KK_LEG_FWD], fwd_tiles_, fwd_kp_, mixed_, s);
   mark(2);
   mark(3);
-  if (ct_n1_) launch_fft_R_ct(args(KK_FFT_R), total_[KK_FFT_R], 0, ct_n1_, nb_max_, s);
-  else launch_fft_R_rt(args(KK_FFT_R), total_[KK_FFT_R], 0, nb_max_, lds_fftR_, s);
+  if (ct_n1_) launch_fft_R_ct(args(KK_FFT_R), total_[KK_FFT_R], 0, 0, ct_n1_, nb_max_, s);
+  else launch_fft_R_rt(args(KK_FFT_R), total_[KK_FFT_R], 0, 0, nb_max_, lds_fftR_, s);
   mark(4);
   if (ct_n2_) launch_fft_r_ct(args(KK_FFT_r), total_[KK_FFT_r], ct_n2_, s);
   else launch_fft_r_rt(args(KK_FFT_r), total_[KK_FFT_r], lds_fftr_, s);
   mark(5);
-  if (ct_n1_) launch_fft_R_ct(args(KK_FFT_R), total_[KK_FFT_R], 1, ct_n1_, nb_max_, s);
-  else launch_fft_R_rt(args(KK_FFT_R), total_[KK_FFT_R], 1, nb_max_, lds_fftR_, s);
+  if (ct_n1_) launch_fft_R_ct(args(KK_FFT_R), total_[KK_FFT_R], 1, any_acf_, ct_n1_, nb_max_, s);
+  else launch_fft_R_rt(args(KK_FFT_R), total_[KK_FFT_R], 1, any_acf_, nb_max_, lds_fftR_, s);
   mark(6);
   mark(7);
   launch_leg_bwd(args(KK_LEG_BWD), total_[KK_LEG_BWD], bwd_tiles_, bwd_kp_, mixed_, s);
   mark(8);
-  launch_obs(args(KK_OBS), total_[KK_OBS], s);
   if (any_flux_) launch_flux(args(KK_FLUX), total_[KK_FLUX], s);
   mark(9);
   launch_reduce(args(KK_REDUCE), total_[KK_REDUCE], s);

@@ -144,9 +144,8 @@ qd_status qd_run(qd_handle *h, int32_t n_steps) {
     const bool copy = (op.wave_to_matlab || op.wave_callback) && op.steps_to_copy_psi_from_device_to_host > 0;
     for (int s = 0; s < n_steps; s++) {
       const int steps_before = j.steps;
-      // the first step (steps == 0) carries the V(-dt/2) phase: run it eagerly,
-      // every later step replays the captured graph
-      if (h->use_graph && steps_before != 0) h->engine->step_graph();
+      j.prephase_if_first(h->stream);   // V(-dt/2) of step 0 (cudaOpenMP2.cu:344)
+      if (h->use_graph) h->engine->step_graph();
       else h->engine->step();
       h->records_enqueued++;
       j.steps++;
@@ -202,6 +201,7 @@ qd_status qd_get_autocorrelation(qd_handle *h, int32_t first, int32_t count, dou
     qd_status st = sync_and_check(h);
     if (st != QD_OK) return st;
     DeviceJob &j = *h->job;
+    if (!j.shape.acf) return fail(h, QD_ERR_STATE, "autocorrelation was not requested at create");
     if (first < 0 || count < 0 || first + count > j.records_done)
       return fail(h, QD_ERR_INVALID, "autocorrelation records out of range");
     std::memcpy(dst, j.acf.data() + (size_t)first * 2, sizeof(double) * 2 * count);
@@ -227,6 +227,7 @@ qd_status qd_profile_steps(qd_handle *h, int32_t n_steps, double *us) {
     double acc[QD_KERNEL_KINDS] = {};
     qd_status st = QD_OK;
     for (int s = 0; s < n_steps && st == QD_OK; s++) {
+      h->job->prephase_if_first(h->stream);
       h->engine->step(ev);
       h->records_enqueued++;
       h->job->steps++;

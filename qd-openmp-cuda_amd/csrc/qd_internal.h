@@ -10,10 +10,21 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Device pointers of the job descriptor are global-memory pointers.  Marking
+// them so in the device compile lets the compiler emit global_load/store
+// (instead of flat accesses, whose waits also drain the LDS counter); the host
+// compile sees plain pointers with the same layout.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define QD_G __attribute__((address_space(1)))
+#else
+#define QD_G
+#endif
+
 namespace qd {
 
 constexpr int kMaxOmega = 24;     // omega blocks per job (K_max <= 23)
 constexpr int kFftMaxN = 512;     // longest FFT pencil
+constexpr int kMaxTheta = 512;    // theta points per job
 constexpr int kBlock = 256;       // threads per workgroup for the step kernels
 
 // per-kernel work-unit counts (index into JobDev::units / LaunchTables)
@@ -43,62 +54,62 @@ struct JobDev {
   int bwd_kp, bwd_np;        // padded K and N, backward
   int fft_rows;              // pencils per workgroup along R
   int fft_cols;              // pencils per workgroup along r
-  int obs_chunks;            // workgroups per (block, theta slice) in the DVR observables
-  int fbr_chunks;            // workgroups per slab in the FBR pass
   int units[KK_COUNT];       // work units (workgroups) per kernel for this job
   // physics
   double dt, dr1, dr2;
   // state
-  double2 *psi;              // [nO][nt][M] DVR wavepackets
-  double2 *fbr;              // [nO][L][M]  FBR wavepackets (slabs l < omega unused)
-  double2 *psi0;             // [nO][nt][M] initial state (autocorrelation), may be null
+  QD_G double2 *psi;              // [nO][nt][M] DVR wavepackets
+  QD_G double2 *fbr;              // [nO][L][M]  FBR wavepackets (slabs l < omega unused)
+  QD_G double2 *psi0;             // unused on the device (kept for layout); the initial state
+                                  // enters the autocorrelation through c0
+  QD_G double2 *c0;               // [nO][L][M] FBR coefficients of the initial state, or null
   // tables
-  const double2 *vphase;     // [nt][M]  d1[i] d2[j] exp(-i dt V)
-  const double *pot;         // [nt][M]
-  const double *w;           // [nt]
-  const double *wP;          // [nO][fwd_kp][fwd_np] row-major (c fastest): w_k P_l(x_k)
-  const double *Pt;          // [nO][bwd_kp][bwd_np] row-major: P_l(x_k), k = l - omega
-  const float *wP_hi, *wP_lo;// split FP32 copies for the mixed form
-  const float *Pt_hi, *Pt_lo;
-  const double *corV;        // [L][nO][nO] eigenvectors of C_l: corV[(l nO + a) nO + b] = v_a(omega_b)
-  const double2 *corPhi;     // [L][n1][nO] exp(-i dt/2 l(l+1) I1_i) exp(-i dt/2 e_a I1_i)
-  const double2 *rot2;       // [L][n2] exp(-i dt/2 l(l+1) / (2 m2 r_j^2))
-  const double2 *kin1;       // [n1] exp(-i dt k_i^2 / (2 m1))
-  const double2 *kin2;       // [n2] exp(-i dt k_j^2 / (2 m2))
-  const double *kinE1;       // [n1] k_i^2/(2 m1)
-  const double *kinE2;       // [n2]
-  const double *I1;          // [n1] 1/(2 m1 R_i^2)
-  const double *I2;          // [n2] 1/(2 m2 r_j^2)
-  const double2 *cmat;       // [L][nO] (C_l(omega,omega), C_l(omega,omega+1)) exact entries
-  const double2 *twR;        // [n1] exp(-2 pi i e / n1)
-  const double2 *twr;        // [n2] exp(-2 pi i e / n2)
+  const QD_G double2 *vphase;     // [nt][M]  d1[i] d2[j] exp(-i dt V)
+  const QD_G double *pot;         // [nt][M]
+  const QD_G double *w;           // [nt]
+  const QD_G double *wP;          // [nO][fwd_kp][fwd_np] row-major (c fastest): w_k P_l(x_k)
+  const QD_G double *Pt;          // [nO][bwd_kp][bwd_np] row-major: P_l(x_k), k = l - omega
+  const QD_G float *wP_hi, *wP_lo;// split FP32 copies for the mixed form
+  const QD_G float *Pt_hi, *Pt_lo;
+  const QD_G double *corV;        // [L][nO][nO] eigenvectors of C_l: corV[(l nO + a) nO + b] = v_a(omega_b)
+  const QD_G double2 *corPhi;     // [L][n1][nO] exp(-i dt/2 l(l+1) I1_i) exp(-i dt/2 e_a I1_i)
+  const QD_G double2 *rot2;       // [L][n2] exp(-i dt/2 l(l+1) / (2 m2 r_j^2))
+  const QD_G double2 *kin1;       // [n1] exp(-i dt k_i^2 / (2 m1))
+  const QD_G double2 *kin2;       // [n2] exp(-i dt k_j^2 / (2 m2))
+  const QD_G double *kinE1;       // [n1] k_i^2/(2 m1)
+  const QD_G double *kinE2;       // [n2]
+  const QD_G double *I1;          // [n1] 1/(2 m1 R_i^2)
+  const QD_G double *I2;          // [n2] 1/(2 m2 r_j^2)
+  const QD_G double2 *cmat;       // [L][nO] (C_l(omega,omega), C_l(omega,omega+1)) exact entries
+  const QD_G double2 *twR;        // [n1] exp(-2 pi i e / n1)
+  const QD_G double2 *twr;        // [n2] exp(-2 pi i e / n2)
   // reduction scratch (per-workgroup partial sums)
-  double *part_cor;          // [nO][2][units FFT_R] Coriolis and rotational energy partials
-  double *part_ke;           // [units FFT_r]       kinetic energy of one (block, slab, column group)
-  double *part_obs;          // [nO][nt][obs_chunks][4] (norm, potential, acf re, acf im)
-  double *obs_ring;          // [ring][nO][5] per-step observables
-  double2 *acf_ring;         // [ring][nO] autocorrelation contribution of each block
+  QD_G double *part_cor;          // [nO][5][units FFT_R] Coriolis, rotational energy, norm, acf re/im
+  QD_G double *part_ke;           // [units FFT_r]       kinetic energy of one (block, slab, column group)
+  QD_G double *part_obs;          // [nO][gemm_bwd_groups] w V |psi|^2 (potential energy)
+  QD_G double *obs_ring;          // [ring][nO][5] per-step observables
+  QD_G double2 *acf_ring;         // [ring][nO] autocorrelation contribution of each block
   int ring;                  // ring capacity (records)
-  int *counters;             // device: [0] step counter (the reference's time.steps),
+  QD_G int *counters;             // device: [0] step counter (the reference's time.steps),
                              //         [1] records written, [2] reduce ticket;
                              //         advanced by the last reduce workgroup of the step
   // flux
   int flux_on, flux_ns, flux_ng, flux_ne;
-  const double *flux_coef;   // [ng] derivative weights
-  const double *flux_E;      // [ne]
-  double2 *flux_A;           // [ne][nO][nt][n1]
-  double2 *flux_B;           // [ne][nO][nt][n1]
+  const QD_G double *flux_coef;   // [ng] derivative weights
+  const QD_G double *flux_E;      // [ne]
+  QD_G double2 *flux_A;           // [ne][nO][nt][n1]
+  QD_G double2 *flux_B;           // [ne][nO][nt][n1]
 };
 
 // Per-launch arguments common to all step kernels.
 struct StepArgs {
-  const JobDev *jobs;
-  const int *unit_base;      // [n_jobs + 1] prefix of units for this kernel kind
+  const QD_G JobDev *jobs;
+  const QD_G int *unit_base;      // [n_jobs + 1] prefix of units for this kernel kind
   int n_jobs;
 };
 
 // Locate the job owning workgroup `u` (binary search over the prefix table).
-__device__ __forceinline__ int locate_job(const int *base, int n_jobs, int u) {
+__device__ __forceinline__ int locate_job(const QD_G int *base, int n_jobs, int u) {
   int lo = 0, hi = n_jobs - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -108,7 +119,8 @@ __device__ __forceinline__ int locate_job(const int *base, int n_jobs, int u) {
 }
 
 // Host-side launchers (each enqueues one kernel on `stream`).
-void launch_dvr(const StepArgs &a, int total_units, hipStream_t stream);
+// step-0 phase exp(+i dt/2 V) on every block of one job (before its first step)
+void launch_prephase(double2 *psi, const double *pot, long G, int nO, double dt, hipStream_t stream);
 void launch_leg_fwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, hipStream_t stream);
 void launch_leg_bwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, hipStream_t stream);
 // FFT along R fused with rotational . Coriolis . rotational (before the forward
@@ -116,16 +128,15 @@ void launch_leg_bwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max
 // and the r pass with the kinetic phase.  _rt: any length factoring into
 // 2, 3, 5 (dynamic LDS); _ct: compile-time lengths (fft_ct_supported), with
 // fft_ct_rows(n1) rows / fft_ct_cols(n2) columns per workgroup.
-void launch_fft_R_rt(const StepArgs &a, int total_units, int inverse, int nb_max, size_t lds, hipStream_t stream);
+void launch_fft_R_rt(const StepArgs &a, int total_units, int inverse, int acf, int nb_max, size_t lds, hipStream_t stream);
 void launch_fft_r_rt(const StepArgs &a, int total_units, size_t lds, hipStream_t stream);
 bool fft_ct_supported(int n);
 int fft_ct_rows(int n1);
 int fft_ct_cols(int n2);
-void launch_fft_R_ct(const StepArgs &a, int total_units, int inverse, int n1, int nb_max, hipStream_t stream);
+void launch_fft_R_ct(const StepArgs &a, int total_units, int inverse, int acf, int n1, int nb_max, hipStream_t stream);
 void launch_fft_r_ct(const StepArgs &a, int total_units, int n2, hipStream_t stream);
 // standalone batched 1-D FFT of `count` contiguous pencils (test probe)
 void launch_fft_probe(double2 *data, const double2 *tw, int n, int count, int inverse, hipStream_t stream);
-void launch_obs(const StepArgs &a, int total_units, hipStream_t stream);
 void launch_reduce(const StepArgs &a, int total_units, hipStream_t stream);
 void launch_flux(const StepArgs &a, int total_units, hipStream_t stream);
 
