@@ -224,7 +224,9 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   d.bwd_kp = round_up(nl_max, 4);
   d.bwd_np = round_up(nt, 16);
   const long strips = (2L * M + 15) / 16;
-  d.gemm_strips = (int)std::max(1L, strips * nO / (4L * 1536));
+  // long-lived workgroups: about two per CU in one round, so each wave runs
+  // many pipelined tiles after paying the B staging and pipeline fill once
+  d.gemm_strips = (int)std::max(1L, (strips * nO + 4L * 512 - 1) / (4L * 512));
   d.gemm_fwd_groups = (int)((strips + 4L * d.gemm_strips - 1) / (4L * d.gemm_strips));
   d.gemm_bwd_groups = d.gemm_fwd_groups;
   static const int rows_c[] = {8, 4, 2, 1};
