@@ -41,7 +41,15 @@ def kernel_model(job):
     """Algorithmic bytes and flops per launch of each kernel kind for one step.
 
     G = n1 n2 ntheta DVR points per block, F_b = n1 n2 (l_max - omega_b + 1)
-    FBR points of block b; every complex value is 16 B."""
+    FBR points of block b; a complex value is 16 B, the potential 8 B.  Per
+    block and step: forward Legendre pass (absorber, potential phase, DGEMM)
+    reads psi and V and writes the FBR slabs, 24 G + 16 F_b; each of the three
+    FFT passes (R forward with the first rot.C.rot half step, r forward .
+    kinetic . r inverse, R inverse with the second half step) reads and writes
+    the FBR state, 32 F_b; the backward Legendre pass reads the FBR state and
+    writes psi, 16 F_b + 16 G.  Total 40 G + 128 F_b.  Flops: the real-B
+    Legendre product, 4 n1 n2 ntheta N_l per block and direction.  Kinds fused
+    into these kernels (dvr, fbr1, fbr2, obs) carry no traffic of their own."""
     r1, r2, th, os_ = job["r1"], job["r2"], job["theta"], job["omega_states"]
     n1, n2, nt = r1["n"], r2["n"], th["n"]
     M = n1 * n2
@@ -51,18 +59,19 @@ def kernel_model(job):
     lmax = os_["l_max"]
     F = [M * (lmax - o + 1) for o in omegas]
     Ft = sum(F)
-    gemm_flops = sum(2 * (2 * M) * nt * (lmax - o + 1) for o in omegas)
+    gemm_flops = sum(4 * M * nt * (lmax - o + 1) for o in omegas)
+    zero = {"bytes": 0, "flops": 0}
     return {
-        "dvr": {"bytes": G * (16 + 32 * nO), "flops": 0, "bound": "hbm"},
-        "leg_fwd": {"bytes": nO * 16 * G + 16 * Ft, "flops": gemm_flops, "bound": "mfma"},
-        "fbr1": {"bytes": 0, "flops": 0, "bound": "hbm"},
-        "fft_R_fwd": {"bytes": 32 * Ft, "flops": 0, "bound": "hbm"},
-        "fft_r": {"bytes": 32 * Ft, "flops": 0, "bound": "hbm"},
-        "fft_R_inv": {"bytes": 32 * Ft, "flops": 0, "bound": "hbm"},
-        "fbr2": {"bytes": 0, "flops": 0, "bound": "hbm"},
-        "leg_bwd": {"bytes": nO * 16 * G + 16 * Ft, "flops": gemm_flops, "bound": "mfma"},
-        "obs": {"bytes": nO * 32 * G + 8 * G, "flops": 0, "bound": "hbm"},
-        "reduce": {"bytes": 0, "flops": 0, "bound": "hbm"},
+        "dvr": zero,
+        "leg_fwd": {"bytes": nO * 24 * G + 16 * Ft, "flops": gemm_flops},
+        "fbr1": zero,
+        "fft_R_fwd": {"bytes": 32 * Ft, "flops": 0},
+        "fft_r": {"bytes": 32 * Ft, "flops": 0},
+        "fft_R_inv": {"bytes": 32 * Ft, "flops": 0},
+        "fbr2": zero,
+        "leg_bwd": {"bytes": nO * 16 * G + 16 * Ft, "flops": gemm_flops},
+        "obs": zero,
+        "reduce": zero,
     }, G * nO
 
 
@@ -155,11 +164,12 @@ def main():
                  "ratio_vs_fp64": el64 / elm, "kernels_us": {k: round(v, 2) for k, v in profm.items()}}
         print(f"# mixed: {elm / args.steps * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
 
-    # roofline of the dominant kernel (largest time per step)
+    # roofline of the dominant kernel (largest time per step); its bound is the
+    # larger of bytes / HBM peak and flops / MFMA peak for that kernel
     dom = max(prof64, key=lambda k: prof64[k])
     m = model[dom]
     us = prof64[dom]
-    if m["bound"] == "mfma":
+    if m["flops"] / (FP64_MFMA_PEAK_TFLOPS * 1e12) > m["bytes"] / (HBM_PEAK_GBS * 1e9):
         achieved = m["flops"] / (us * 1e-6) / 1e12
         roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP64_MFMA_PEAK_TFLOPS}
@@ -167,8 +177,16 @@ def main():
         achieved = m["bytes"] / (us * 1e-6) / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS}
-    roof["kernel"] = dom
     roof["traffic"] = read_traffic(dom)
+    roof["kernel"] = dom
+    roof["kernel_us"] = us
+    roof["algorithmic_bytes"] = m["bytes"]
+    roof["algorithmic_flops"] = m["flops"]
+    # whole step against the same per-kernel model: sum of max(bytes/BW, flops/peak)
+    t_model = sum(max(mk["bytes"] / (HBM_PEAK_GBS * 1e9), mk["flops"] / (FP64_MFMA_PEAK_TFLOPS * 1e12))
+                  for mk in model.values())
+    roof["step_model_us"] = t_model * 1e6
+    roof["step_model_frac"] = t_model / (el64 / args.steps)
     per_kernel = {}
     for k, us_k in prof64.items():
         if us_k < 0.5:          # kinds fused into a neighbouring kernel
