@@ -31,6 +31,7 @@ def lib():
         _lib.oracle_coriolis_dense.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]
         _lib.oracle_coriolis_exp.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                              C.c_void_p]
+        _lib.oracle_first_derivative.argtypes = [C.c_int, C.c_double, C.c_void_p]
     return _lib
 
 
@@ -73,3 +74,10 @@ def coriolis_exp(J: int, l: int, omega_min: int, n: int, tau: float, I: float):
     out = np.zeros((n, n), dtype=np.complex128)
     lib().oracle_coriolis_exp(J, l, omega_min, n, tau, I, out.ctypes.data)
     return out
+
+
+def first_derivative(n_points: int, h: float):
+    import numpy as np
+    c = np.zeros(n_points)
+    lib().oracle_first_derivative(n_points, h, c.ctypes.data)
+    return c

@@ -71,7 +71,12 @@ class DeviceJob {
   void copy_psi(double *dst, hipStream_t s) const;
   // before the first step of a job whose step counter is 0: apply exp(+i dt/2 V)
   void prephase_if_first(hipStream_t s);
-  void flux_to_crp(const qd_inputs *in, double *crp) const;
+  // flux: after each enqueued step (steps already advanced) fold a full
+  // window into the energy accumulators; flux_to_crp folds the open window
+  // and writes P(E) for every CRP energy (synchronous)
+  void flux_after_step(hipStream_t s);
+  void flux_flush(hipStream_t s);
+  void flux_to_crp(double *crp, hipStream_t s);
 
   static size_t estimate_footprint(const JobShape &s, int n_energies);
 
@@ -79,7 +84,13 @@ class DeviceJob {
   std::vector<void *> allocs_;
   void *alloc(size_t bytes);
   template <class T> T *upload(const std::vector<T> &v, hipStream_t s);
+  int flux_window_start_ = 0;    // step counter at the first step of the open window
+  double2 *flux_A_ = nullptr, *flux_B_ = nullptr, *flux_ph_ = nullptr;
+  double *flux_E_ = nullptr, *flux_eta_ = nullptr, *flux_out_ = nullptr;
 };
+
+// central first-derivative weights on n (odd) points of spacing h
+std::vector<double> first_derivative_weights(int n, double h);
 
 // Launches the step kernels for a set of resident jobs.
 class Engine {

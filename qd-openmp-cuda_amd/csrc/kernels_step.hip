@@ -146,7 +146,4 @@ void launch_copy(const double2 *src, double2 *dst, long n, hipStream_t stream) {
   hipLaunchKernelGGL(k_copy, dim3(4096), dim3(kBlock), 0, stream, src, dst, n);
 }
 
-// flux accumulation at the dividing surface (filled in by the CRP pass)
-void launch_flux(const StepArgs &, int, hipStream_t) {}
-
 }  // namespace qd

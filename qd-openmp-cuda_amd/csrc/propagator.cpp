@@ -27,6 +27,7 @@ namespace {
 
 constexpr size_t kLdsLimit = 160 * 1024;
 constexpr int kRing = 1024;
+constexpr int kFluxWindow = 64;   // steps buffered per energy transform
 
 bool factors_235(int n) {
   if (n < 2) return false;
@@ -202,7 +203,7 @@ size_t DeviceJob::estimate_footprint(const JobShape &s, int n_energies) {
   b += 16 * (size_t)s.M * s.L * s.nO;     // fbr
   b += 24 * G;                            // vphase + pot
   b += 16 * (size_t)s.L * s.n1 * s.nO * s.nO;
-  b += 16 * (size_t)n_energies * s.nO * s.nt * s.n1 * 2;
+  b += 16 * (size_t)(n_energies + kFluxWindow) * s.nO * s.nt * s.n1 * 2;   // flux accumulators, window
   b += 4 << 20;                           // tables, partials, ring
   return b;
 }
@@ -379,7 +380,55 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   const int cnt[3] = {sh.steps0, 0, 0};
   QD_HIP(hipMemcpyAsync(d.counters, cnt, sizeof(cnt), hipMemcpyHostToDevice, st));
   d.flux_on = 0;
+  d.units[KK_FLUX] = 0;
+  if (sh.crp) {
+    const qd_crp &c = in->crp;
+    const int ng = c.n_gradient_points, half = ng / 2, ne = c.n_energies;
+    d.flux_on = 1;
+    d.flux_ns = c.n_dividing_surface;
+    d.flux_ng = ng;
+    d.flux_ne = ne;
+    d.flux_T = kFluxWindow;
+    d.flux_S = nO * nt * n1;
+    d.units[KK_FLUX] = (d.flux_S + kBlock - 1) / kBlock;
+    // restores the leapfrogged V(dt/2) on the stencil rows
+    std::vector<double2> rph((size_t)ng * nt * n1);
+    for (int g = 0; g < ng; g++)
+      for (int k = 0; k < nt; k++)
+        for (int i = 0; i < n1; i++) {
+          const long idx = (long)k * M + (long)(d.flux_ns - half + g) * n1 + i;
+          rph[((size_t)g * nt + k) * n1 + i] = phase(-0.5 * dt * pot[idx]);
+        }
+    d.flux_coef = upload(first_derivative_weights(ng, sh.dr2), st);
+    d.flux_rph = upload(rph, st);
+    const size_t S = d.flux_S;
+    d.flux_W = (double2 *)alloc(16 * S * 2 * kFluxWindow);
+    flux_A_ = (double2 *)alloc(16 * S * ne);
+    flux_B_ = (double2 *)alloc(16 * S * ne);
+    QD_HIP(hipMemsetAsync(flux_A_, 0, 16 * S * ne, st));
+    QD_HIP(hipMemsetAsync(flux_B_, 0, 16 * S * ne, st));
+    flux_ph_ = (double2 *)alloc(16 * (size_t)ne * kFluxWindow);
+    flux_E_ = upload(std::vector<double>(c.energies, c.energies + ne), st);
+    flux_eta_ = upload(std::vector<double>(c.eta_sq, c.eta_sq + ne), st);
+    flux_out_ = (double *)alloc(sizeof(double) * ne);
+    flux_window_start_ = sh.steps0;
+  }
   QD_HIP(hipStreamSynchronize(st));   // host tables go out of scope
+}
+
+std::vector<double> first_derivative_weights(int n, double h) {
+  // c_m = (-1)^(m+1) (k!)^2 / (m (k-m)! (k+m)!) / h, c_-m = -c_m, k = n / 2:
+  // exact for polynomials of degree <= n - 1, as a product of ratios
+  const int half = n / 2;
+  std::vector<double> c(n, 0.0);
+  for (int m = 1; m <= half; m++) {
+    double r = 1.0;
+    for (int q = 0; q < m; q++) r *= (double)(half - q) / (double)(half + q + 1);
+    const double cm = ((m % 2) ? 1.0 : -1.0) * r / m / h;
+    c[half + m] = cm;
+    c[half - m] = -cm;
+  }
+  return c;
 }
 
 void DeviceJob::drain(int records_total) {
@@ -415,7 +464,40 @@ void DeviceJob::prephase_if_first(hipStream_t s) {
   prephased = true;
 }
 
-void DeviceJob::flux_to_crp(const qd_inputs *, double *) const {}
+void DeviceJob::flux_flush(hipStream_t s) {
+  if (!dev.flux_on || steps == flux_window_start_) return;
+  FluxWindow w;
+  w.W = dev.flux_W;
+  w.A = flux_A_;
+  w.B = flux_B_;
+  w.ph = flux_ph_;
+  w.E = flux_E_;
+  w.dt = shape.dt;
+  w.S = dev.flux_S;
+  w.ne = dev.flux_ne;
+  w.T = dev.flux_T;
+  w.s0 = flux_window_start_;
+  w.n = steps - flux_window_start_;
+  launch_flux_window(w, s);
+  QD_HIP(hipGetLastError());
+  flux_window_start_ = steps;
+}
+
+void DeviceJob::flux_after_step(hipStream_t s) {
+  if (dev.flux_on && steps - flux_window_start_ >= dev.flux_T) flux_flush(s);
+}
+
+void DeviceJob::flux_to_crp(double *crp, hipStream_t s) {
+  if (!dev.flux_on) return;
+  flux_flush(s);
+  // P = Im(sum w conj(A) B) dr1 / (2 pi mu_r eta)
+  const double scale = shape.dr1 / (2.0 * M_PI * shape.m2);
+  launch_flux_finish(flux_A_, flux_B_, dev.w, flux_eta_, flux_out_, dev.flux_S, shape.n1, shape.nt, dev.flux_ne,
+                     scale, s);
+  QD_HIP(hipGetLastError());
+  QD_HIP(hipMemcpyAsync(crp, flux_out_, sizeof(double) * dev.flux_ne, hipMemcpyDeviceToHost, s));
+  QD_HIP(hipStreamSynchronize(s));
+}
 
 // ---------------------------------------------------------------------------
 // Engine

@@ -149,6 +149,7 @@ qd_status qd_run(qd_handle *h, int32_t n_steps) {
       else h->engine->step();
       h->records_enqueued++;
       j.steps++;
+      j.flux_after_step(h->stream);
       const bool ring_full = h->records_enqueued - j.records_done >= j.dev.ring;
       const bool copy_now = copy && steps_before % op.steps_to_copy_psi_from_device_to_host == 0;
       if (ring_full || copy_now || s == n_steps - 1) {
@@ -213,8 +214,7 @@ qd_status qd_get_crp(qd_handle *h, double *crp) {
   if (!h || !crp) return fail(h, QD_ERR_INVALID, "NULL argument");
   QD_GUARD(h, {
     if (!h->job->shape.crp) return fail(h, QD_ERR_STATE, "CRP was not requested (calculate_CRP == 0)");
-    QD_HIP(hipStreamSynchronize(h->stream));
-    h->job->flux_to_crp(h->in, crp);
+    h->job->flux_to_crp(crp, h->stream);
     return QD_OK;
   })
 }
@@ -231,6 +231,7 @@ qd_status qd_profile_steps(qd_handle *h, int32_t n_steps, double *us) {
       h->engine->step(ev);
       h->records_enqueued++;
       h->job->steps++;
+      h->job->flux_after_step(h->stream);
       QD_HIP(hipEventSynchronize(ev[QD_KERNEL_KINDS]));
       for (int k = 0; k < QD_KERNEL_KINDS; k++) {
         float ms = 0.f;

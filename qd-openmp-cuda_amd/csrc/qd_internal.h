@@ -93,12 +93,27 @@ struct JobDev {
   QD_G int *counters;             // device: [0] step counter (the reference's time.steps),
                              //         [1] records written, [2] reduce ticket;
                              //         advanced by the last reduce workgroup of the step
-  // flux
-  int flux_on, flux_ns, flux_ng, flux_ne;
-  const QD_G double *flux_coef;   // [ng] derivative weights
-  const QD_G double *flux_E;      // [ne]
-  QD_G double2 *flux_A;           // [ne][nO][nt][n1]
-  QD_G double2 *flux_B;           // [ne][nO][nt][n1]
+  // flux at the dividing surface r2 = r_s (index flux_ns): per step the
+  // surface kernel writes the values and the ng-point r-derivative of the
+  // state with V(dt/2) restored into window slot (step % flux_T); every
+  // flux_T steps the window is folded into the energy accumulators
+  int flux_on, flux_ns, flux_ng, flux_ne, flux_T;
+  int flux_S;                     // surface points per step: nO * nt * n1
+  const QD_G double *flux_coef;   // [ng] first-derivative weights
+  const QD_G double2 *flux_rph;   // [ng][nt][n1] exp(-i dt/2 V) on the stencil rows
+  QD_G double2 *flux_W;           // [T][2][S] window: values, derivatives
+};
+
+// Arguments of the windowed time-to-energy transform of one job:
+//   A[e][s] += sum_q ph[e][q] W[(s0 + q) % T][0][s],  B likewise with W[.][1]
+//   ph[e][q] = exp(i E_e t_q) dt, t_q = (s0 + q + 1) dt
+struct FluxWindow {
+  const double2 *W;
+  double2 *A, *B;                 // [ne][S]
+  double2 *ph;                    // [ne][T]
+  const double *E;                // [ne]
+  double dt;
+  int S, ne, T, s0, n;
 };
 
 // Per-launch arguments common to all step kernels.
@@ -138,7 +153,13 @@ void launch_fft_r_ct(const StepArgs &a, int total_units, int n2, hipStream_t str
 // standalone batched 1-D FFT of `count` contiguous pencils (test probe)
 void launch_fft_probe(double2 *data, const double2 *tw, int n, int count, int inverse, hipStream_t stream);
 void launch_reduce(const StepArgs &a, int total_units, hipStream_t stream);
+// surface values of every flux job (one step, part of the step sequence)
 void launch_flux(const StepArgs &a, int total_units, hipStream_t stream);
+// fold the window of one job into its energy accumulators
+void launch_flux_window(const FluxWindow &w, hipStream_t stream);
+// P(E_e) = Im(sum_s w_k conj(A) B) dr1 / (2 pi mu_r eta_e) for every energy
+void launch_flux_finish(const double2 *A, const double2 *B, const double *w, const double *eta, double *out,
+                        int S, int n1, int nt, int ne, double scale, hipStream_t stream);
 
 // plain copy kernel (traffic calibration)
 void launch_copy(const double2 *src, double2 *dst, long n, hipStream_t stream);

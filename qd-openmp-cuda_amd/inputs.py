@@ -212,17 +212,21 @@ def cubic_absorber(x: np.ndarray, width: float, strength: float) -> np.ndarray:
 
 
 def fornberg_first_derivative(n_points: int, h: float) -> np.ndarray:
-    """Central finite-difference weights for d/dx on n_points (odd) equispaced points."""
+    """Central finite-difference weights for d/dx on n_points (odd) equispaced
+    points, exact for polynomials of degree n_points - 1:
+    c_m = (-1)^(m+1) (k!)^2 / (m (k-m)! (k+m)!) / h, c_-m = -c_m, k = n_points // 2."""
     if n_points < 3 or n_points % 2 == 0:
         raise ValueError("n_points must be odd and >= 3")
     half = n_points // 2
-    offs = np.arange(-half, half + 1, dtype=np.float64)
-    # solve the Vandermonde system sum_j c_j o_j^q = q! delta_{q,1}
-    A = np.vander(offs, n_points, increasing=True).T
-    b = np.zeros(n_points)
-    b[1] = 1.0
-    c = np.linalg.solve(A, b)
-    return c / h
+    c = np.zeros(n_points)
+    for m in range(1, half + 1):
+        r = 1.0
+        for q in range(m):
+            r *= (half - q) / (half + q + 1)
+        cm = (1.0 if m % 2 else -1.0) * r / m / h
+        c[half + m] = cm
+        c[half - m] = -cm
+    return c
 
 
 # ----------------------------------------------------------------------------

@@ -1,0 +1,96 @@
+"""Reaction probabilities from the device flux path (surface kernel, windowed
+energy transform, finish) against the FP64 oracle and a known answer.
+
+Bars: P(E) within 1e-9 absolute of the oracle (FP64 form); the Eckart case
+also within 2e-4 of the analytic transmission.
+"""
+import dataclasses
+import sys
+
+import numpy as np
+import pytest
+
+from qd_openmp_cuda_amd import abi, api, inputs
+from oracle import oracle
+from flux_cases import eckart_job, eckart_transmission
+
+pytestmark = pytest.mark.gpu
+
+CRP_TOL = 1e-9
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _multiblock_job(n_energies=10):
+    # J = 5, p = 1, K_max = 3: four omega blocks; surface r = 2.8 bohr keeps the
+    # 21-point stencil inside the 32-point r grid
+    spec = inputs.JobSpec(n1=64, n2=32, ntheta=24, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
+                          J=5, parity=1, k_max=3, omega0=1, j0=2, E_coll=0.02, flux_r=2.8,
+                          n_energies=n_energies, energy_range=(0.012, 0.03))
+    return inputs.build_job(spec)
+
+
+def test_flux_eckart_matches_oracle_and_transmission():
+    job = eckart_job()
+    _say("case eckart 32x512x4 J=0 steps=1600 energies=12")
+    a = abi.Inputs(job, want_acf=False)
+    b = abi.Inputs(job, want_acf=False)
+    oracle.evolve(a, 8)
+    api.evolve(b)
+    T = eckart_transmission(job["meta"]["e_trans"])
+    e_or = float(np.abs(b.crp - a.crp).max())
+    e_an = float(np.abs(b.crp - T).max())
+    _say(f"  crp vs oracle {e_or:.3e}  vs analytic {e_an:.3e}")
+    assert e_or < CRP_TOL
+    assert e_an < 2e-4
+
+
+def test_flux_multiblock_parity():
+    """Four omega blocks, 100 steps: one full 64-step window plus a partial one."""
+    job = _multiblock_job()
+    _say("case flux 64x32x24 J=5 p=1 k_max=3 steps=100 energies=10")
+    a = abi.Inputs(job, total_steps=100)
+    b = abi.Inputs(job, total_steps=100)
+    oracle.evolve(a, 8)
+    api.evolve(b)
+    # after 100 steps the flux has not converged, so P is not bounded by 1:
+    # compare on the scale max(1, |P|)
+    e = float((np.abs(b.crp - a.crp) / np.maximum(1.0, np.abs(a.crp))).max())
+    _say(f"  crp vs oracle {e:.3e}  (max |P| {np.abs(a.crp).max():.3e})")
+    assert np.any(a.crp != 0.0)
+    assert e < CRP_TOL
+    assert np.abs(b.psi - a.psi).max() < 1e-10 * np.abs(a.psi).max()
+
+
+def test_crp_readout_mid_run():
+    """qd_get_crp folds the open window; reading it after 50 steps and again
+    after 100 gives the 100-step one-shot result."""
+    job = _multiblock_job(n_energies=6)
+    _say("case flux mid-run readout 64x32x24 steps=50+50")
+    one = abi.Inputs(job, total_steps=100)
+    api.evolve(one)
+    inp = abi.Inputs(job, total_steps=100)
+    with api.Propagator(inp) as p:
+        p.run(50)
+        half = p.crp()
+        p.run(50)
+        full = p.crp()
+    ref = abi.Inputs(job, total_steps=50)
+    api.evolve(ref)
+    _say(f"  50: {np.abs(half - ref.crp).max():.3e}  100: {np.abs(full - one.crp).max():.3e}")
+    assert np.abs(half - ref.crp).max() <= 1e-12 * max(1.0, np.abs(ref.crp).max())
+    assert np.abs(full - one.crp).max() <= 1e-12 * max(1.0, np.abs(one.crp).max())
+
+
+def test_crp_not_requested_is_an_error():
+    job = _multiblock_job(n_energies=0)
+    inp = abi.Inputs(job, total_steps=2)
+    with api.Propagator(inp) as p:
+        p.run(2)
+        with pytest.raises(api.QDError):
+            lib_out = np.zeros(1)
+            rc = api.lib().qd_get_crp(p.h, lib_out.ctypes.data)
+            api._check(rc, p.h)

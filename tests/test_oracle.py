@@ -211,3 +211,42 @@ def test_norm_check_and_copy_back():
     bad["omega_states"] = dict(job["omega_states"], wave_packets=job["omega_states"]["wave_packets"] * 1.01)
     with pytest.raises(RuntimeError, match="norm"):
         oracle.evolve(abi.Inputs(bad, total_steps=2))
+
+
+# ---------------------------------------------------------------- flux / CRP
+@pytest.mark.parametrize("n", [3, 5, 9, 21])
+def test_first_derivative_weights_exact(n):
+    """Stencil weights against the exact rational solution of the moment
+    (Vandermonde) system sum_m c_m m^q = delta_q1, q = 0..n-1."""
+    from fractions import Fraction
+    half = n // 2
+    offs = list(range(-half, half + 1))
+    A = [[Fraction(m) ** q for m in offs] for q in range(n)]
+    b = [Fraction(int(q == 1)) for q in range(n)]
+    for col in range(n):        # exact Gauss-Jordan
+        piv = next(r for r in range(col, n) if A[r][col] != 0)
+        A[col], A[piv] = A[piv], A[col]
+        b[col], b[piv] = b[piv], b[col]
+        for r in range(n):
+            if r != col and A[r][col] != 0:
+                f = A[r][col] / A[col][col]
+                A[r] = [x - f * y for x, y in zip(A[r], A[col])]
+                b[r] -= f * b[col]
+    exact = np.array([float(b[q] / A[q][q]) for q in range(n)])
+    h = 0.05
+    for got in (oracle.first_derivative(n, h), inputs.fornberg_first_derivative(n, h)):
+        assert np.abs(got * h - exact).max() < 4e-16 * max(1.0, np.abs(exact).max())
+    assert n != 3 or np.allclose(exact, [-0.5, 0.0, 0.5], atol=0, rtol=0)
+
+
+def test_flux_eckart_transmission():
+    """Known answer: the oracle's P(E) for a packet crossing a symmetric
+    Eckart barrier (separable J = 0 job) equals the analytic transmission
+    T(E - E_R0) to 2e-4 over T = 0.08..0.99."""
+    from flux_cases import eckart_job, eckart_transmission
+    job = eckart_job()
+    inp = abi.Inputs(job, want_acf=False)
+    oracle.evolve(inp, 8)
+    T = eckart_transmission(job["meta"]["e_trans"])
+    assert float(np.abs(inp.crp - T).max()) < 2e-4
+    assert inp.obs[-1, 0, 0] < 1e-3      # the packet has left the grid
