@@ -8,7 +8,12 @@ omega blocks with inputs resident in HBM.  value = total DVR grid points
 (FP32 hi/lo Legendre) form is measured on the same workload and reported
 beside it.
 
+The serving form (qd_server_*) is measured on a seeded Poisson stream of
+partial-wave jobs and reported under "serving" (grid-point-steps/s, jobs per
+hour, p50/p99 latency from submit to result), FP64 and mixed.
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-mixed]
+                    [--no-serve] [--serve-jobs N] [--serve-steps S]
 """
 import argparse
 import dataclasses
@@ -49,7 +54,8 @@ def kernel_model(job):
     the FBR state, 32 F_b; the backward Legendre pass reads the FBR state and
     writes psi, 16 F_b + 16 G.  Total 40 G + 128 F_b.  Flops: the real-B
     Legendre product, 4 n1 n2 ntheta N_l per block and direction.  Kinds fused
-    into these kernels (dvr, fbr1, fbr2, obs) carry no traffic of their own."""
+    into these kernels (dvr, fbr1, fbr2) carry no traffic of their own; the
+    flux kernel runs only when reaction probabilities are requested."""
     r1, r2, th, os_ = job["r1"], job["r2"], job["theta"], job["omega_states"]
     n1, n2, nt = r1["n"], r2["n"], th["n"]
     M = n1 * n2
@@ -70,7 +76,7 @@ def kernel_model(job):
         "fft_R_inv": {"bytes": 32 * Ft, "flops": 0},
         "fbr2": zero,
         "leg_bwd": {"bytes": nO * 16 * G + 16 * Ft, "flops": gemm_flops},
-        "obs": zero,
+        "flux": zero,
         "reduce": zero,
     }, G * nO
 
@@ -130,6 +136,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=64)
+    ap.add_argument("--serve-steps", type=int, default=200)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -189,12 +198,33 @@ def main():
     roof["step_model_frac"] = t_model / (el64 / args.steps)
     per_kernel = {}
     for k, us_k in prof64.items():
-        if us_k < 0.5:          # kinds fused into a neighbouring kernel
+        if us_k <= 0.0:         # kinds fused into a neighbouring kernel
             continue
         mk = model[k]
         per_kernel[k] = {"us": round(us_k, 2),
                          "GB/s": round(mk["bytes"] / (us_k * 1e-6) / 1e9, 1) if us_k > 0 else None,
                          "TFLOP/s": round(mk["flops"] / (us_k * 1e-6) / 1e12, 2) if mk["flops"] else None}
+
+    serving = None
+    if not args.no_serve:
+        serving = {}
+        from qd_openmp_cuda_amd import serve
+        stream = inputs.serve_jobs(n_jobs=args.serve_jobs, total_steps=args.serve_steps)
+        # "poisson": the seeded arrival times (throughput bounded by the arrival
+        # rate); "saturated": every request at t = 0 (the server's capacity)
+        for name, prec in (("fp64", abi.QD_PREC_FP64), ("mixed", abi.QD_PREC_MIXED)):
+            for mode, scale in (("poisson", 1.0), ("saturated", 0.0)):
+                prepared = serve.prepare(stream, precision=prec)
+                r = serve.run_stream(prepared, time_scale=scale)
+                del prepared
+                r.pop("latency_s")
+                serving[f"{name}_{mode}"] = r
+                print(f"# serving {name} {mode}: {r['grid_point_steps_per_s']:.4g} gps/s, "
+                      f"p50 {r['latency_p50_s']:.2f} s, p99 {r['latency_p99_s']:.2f} s, {r['failed']} failed",
+                      file=sys.stderr, flush=True)
+        serving["workload"] = (f"{args.serve_jobs} Poisson arrivals (4/s, seed 5), J uniform 0..40, "
+                               f"K_max=min(J,8), shapes 128x64x48 / 192x128x64 / 256x192x96, "
+                               f"{args.serve_steps} steps each")
 
     cpu = None
     if not args.no_cpu:
@@ -219,6 +249,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "mixed": mixed,
+        "serving": serving,
         "kernels": per_kernel,
     }
     print(json.dumps(out))

@@ -200,21 +200,62 @@ const char *qd_last_global_error(void);
 /* Device-side timing of the kernels of one step, for measurement tools:
  * runs n_steps steps with HIP events around each launch on the handle's
  * stream and returns the mean microseconds per launch of each kernel kind
- * in us[QD_KERNEL_KINDS]. The steps advance the state like qd_run. */
+ * in us[QD_KERNEL_KINDS]; kinds fused into another kernel read 0.  The steps
+ * advance the state like qd_run. */
 enum {
-  QD_K_DVR = 0,      /* absorber and potential phase (DVR)          */
-  QD_K_LEG_FWD = 1,  /* forward Legendre transform                   */
-  QD_K_FBR1 = 2,     /* rotational-Coriolis-rotational, first half   */
-  QD_K_FFT_R_FWD = 3,/* forward FFT along R                          */
-  QD_K_FFT_r = 4,    /* FFT along r, kinetic phase, inverse FFT      */
-  QD_K_FFT_R_INV = 5,/* inverse FFT along R and 1/(n1 n2) scale      */
-  QD_K_FBR2 = 6,     /* rotational-Coriolis-rotational, second half  */
-  QD_K_LEG_BWD = 7,  /* backward Legendre transform                  */
-  QD_K_OBS = 8,      /* DVR observables, autocorrelation, flux       */
-  QD_K_REDUCE = 9,   /* final reduction of the per-step observables  */
+  QD_K_DVR = 0,      /* absorber and potential phase: fused into QD_K_LEG_FWD      */
+  QD_K_LEG_FWD = 1,  /* forward Legendre transform (with the DVR phase)            */
+  QD_K_FBR1 = 2,     /* first rotational-Coriolis-rotational half step: fused into QD_K_FFT_R_FWD */
+  QD_K_FFT_R_FWD = 3,/* forward FFT along R                                        */
+  QD_K_FFT_r = 4,    /* FFT along r, kinetic phase, inverse FFT                    */
+  QD_K_FFT_R_INV = 5,/* inverse FFT along R, 1/(n1 n2) scale, second half step     */
+  QD_K_FBR2 = 6,     /* second half step: fused into QD_K_FFT_R_INV                */
+  QD_K_LEG_BWD = 7,  /* backward Legendre transform (with the potential energy)    */
+  QD_K_OBS = 8,      /* flux surface values (when CRP is requested)                */
+  QD_K_REDUCE = 9,   /* final reduction of the per-step observables                */
   QD_KERNEL_KINDS = 10
 };
 qd_status qd_profile_steps(qd_handle *h, int32_t n_steps, double *us);
+
+/* ---------------------------------------------------------------------------
+ * Serving form (no counterpart in the reference, which runs one partial wave
+ * per process).  A server keeps many partial-wave jobs resident on the current
+ * device and advances every running job by one step per qd_server_step:
+ *   - qd_server_submit validates a job and queues it (nothing is allocated);
+ *   - each qd_server_step first admits queued jobs whose estimated device
+ *     footprint fits the byte budget (backfilling past jobs that do not fit),
+ *     then runs one step of every running job (jobs of one grid shape and
+ *     precision share each kernel launch and a HIP graph; shape classes run on
+ *     their own streams), then retires jobs that reached time.total_steps:
+ *     their outputs (time.steps, wave_packets with final_copy, observables,
+ *     autocorrelation, CRP.CRP) are written into their qd_inputs and their
+ *     device memory is freed for the next admissions.
+ * A job's results do not depend on the other jobs in the mix: FP64 results
+ * are bitwise those of qd_evolve on the same inputs.
+ * The inputs of a submitted job are borrowed until the job is DONE or FAILED.
+ * Per-step copy-back (wave_to_matlab / wave_callback) is a qd_run feature and
+ * is rejected here.
+ * ------------------------------------------------------------------------- */
+typedef struct qd_server qd_server;
+enum { QD_JOB_QUEUED = 0, QD_JOB_RUNNING = 1, QD_JOB_DONE = 2, QD_JOB_FAILED = 3 };
+
+/* hbm_budget_bytes <= 0: 90% of the device memory free at creation;
+ * max_running <= 0: no limit besides the budget. */
+qd_status qd_server_create(int64_t hbm_budget_bytes, int32_t max_running, qd_server **out);
+qd_status qd_server_submit(qd_server *s, qd_inputs *in, int32_t *job_id);
+/* admit, advance every running job by one step, retire finished jobs;
+ * *n_running / *n_queued (may be NULL) report the state after the call */
+qd_status qd_server_step(qd_server *s, int32_t *n_running, int32_t *n_queued);
+/* state (QD_JOB_*), steps done, seconds from submit to retirement (or to now),
+ * seconds from submit to admission (-1 while queued) */
+qd_status qd_server_job_status(qd_server *s, int32_t job_id, int32_t *state, int32_t *steps_done,
+                               double *latency_s, double *queued_s);
+/* message of a FAILED job */
+const char *qd_server_job_error(qd_server *s, int32_t job_id);
+/* bytes of the budget in use by running jobs (estimates) and the budget */
+qd_status qd_server_memory(qd_server *s, int64_t *used, int64_t *budget);
+void qd_server_destroy(qd_server *s);
+const char *qd_server_last_error(const qd_server *s);
 
 /* Library version string. */
 const char *qd_version(void);

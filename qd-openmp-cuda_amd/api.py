@@ -35,6 +35,15 @@ _DECLS = {
     "qd_probe_fft": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     "qd_probe_coriolis": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
     "qd_probe_copy": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
+    "qd_server_create": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_void_p)]),
+    "qd_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32)]),
+    "qd_server_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "qd_server_job_status": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "qd_server_job_error": (C.c_char_p, [C.c_void_p, C.c_int32]),
+    "qd_server_memory": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "qd_server_destroy": (None, [C.c_void_p]),
+    "qd_server_last_error": (C.c_char_p, [C.c_void_p]),
 }
 
 
@@ -121,6 +130,70 @@ class Propagator:
     def close(self) -> None:
         if self.h:
             lib().qd_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Serving form (qd_server_*): many resident jobs stepped together.
+
+    submit() queues an abi.Inputs (kept alive here until the job retires);
+    step() admits what fits the memory budget, advances every running job by
+    one step and retires finished ones (their outputs land in their Inputs)."""
+
+    QUEUED, RUNNING, DONE, FAILED = abi.QD_JOB_QUEUED, abi.QD_JOB_RUNNING, abi.QD_JOB_DONE, abi.QD_JOB_FAILED
+
+    def __init__(self, budget_bytes: int = 0, max_running: int = 0):
+        h = C.c_void_p()
+        rc = lib().qd_server_create(budget_bytes, max_running, C.byref(h))
+        if rc != abi.QD_OK:
+            raise QDError(rc, (lib().qd_server_last_error(None) or b"").decode())
+        self.h = h
+        self._inputs = {}
+
+    def _check(self, rc):
+        if rc != abi.QD_OK:
+            raise QDError(rc, (lib().qd_server_last_error(self.h) or b"").decode())
+
+    def submit(self, inputs: abi.Inputs) -> int:
+        jid = C.c_int32()
+        self._check(lib().qd_server_submit(self.h, C.addressof(inputs.struct), C.byref(jid)))
+        self._inputs[jid.value] = inputs
+        return jid.value
+
+    def step(self):
+        nr, nq = C.c_int32(), C.c_int32()
+        self._check(lib().qd_server_step(self.h, C.byref(nr), C.byref(nq)))
+        return nr.value, nq.value
+
+    def status(self, job_id: int):
+        st, done, lat, q = C.c_int32(), C.c_int32(), C.c_double(), C.c_double()
+        self._check(lib().qd_server_job_status(self.h, job_id, C.byref(st), C.byref(done), C.byref(lat),
+                                               C.byref(q)))
+        return {"state": st.value, "steps": done.value, "latency_s": lat.value, "queued_s": q.value}
+
+    def error(self, job_id: int) -> str:
+        return (lib().qd_server_job_error(self.h, job_id) or b"").decode()
+
+    def memory(self):
+        u, b = C.c_int64(), C.c_int64()
+        self._check(lib().qd_server_memory(self.h, C.byref(u), C.byref(b)))
+        return u.value, b.value
+
+    def close(self):
+        if self.h:
+            lib().qd_server_destroy(self.h)
             self.h = None
 
     def __enter__(self):

@@ -98,9 +98,11 @@ class Engine {
   explicit Engine(hipStream_t s);
   ~Engine();
   void set_jobs(const std::vector<DeviceJob *> &jobs);
-  // enqueue one step for every job; with ev != nullptr records
-  // QD_KERNEL_KINDS+1 events around the launches
+  // enqueue one step for every job; with ev != nullptr (2 QD_KERNEL_KINDS
+  // events) records ev[2k], ev[2k+1] around the launch of each kind k in
+  // timed_kinds()
   void step(hipEvent_t *ev = nullptr);
+  unsigned timed_kinds() const;
   const std::vector<DeviceJob *> &jobs() const { return jobs_; }
   hipStream_t stream() const { return stream_; }
   // graph capture of one step (after set_jobs); replays on the stream

@@ -301,6 +301,13 @@ static void launch_leg(const StepArgs &a, int total_units, int nt_tiles, int kp_
 #undef QD_LEG_CASE
 }
 
+bool leg_tiles_supported(int nt_tiles) {
+  switch (nt_tiles) {
+    case 1: case 2: case 3: case 4: case 5: case 6: case 7: case 8: case 10: case 12: case 15: return true;
+    default: return false;
+  }
+}
+
 void launch_leg_fwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, hipStream_t s) {
   launch_leg<true>(a, total_units, nt_tiles, kp_max, mixed, s);
 }

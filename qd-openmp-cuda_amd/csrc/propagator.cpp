@@ -155,6 +155,8 @@ JobShape validate_inputs(const qd_inputs *in) {
   QD_REQUIRE(gemm_lds_bytes(round_up(s.nt, 4), round_up(nl_max, 16), s.mixed) <= kLdsLimit &&
                  gemm_lds_bytes(round_up(nl_max, 4), round_up(s.nt, 16), s.mixed) <= kLdsLimit,
              "theta.n / l_max too large for the LDS-resident Legendre tables");
+  QD_REQUIRE(leg_tiles_supported(round_up(nl_max, 16) / 16) && leg_tiles_supported(round_up(s.nt, 16) / 16),
+             "no Legendre GEMM instantiation for this theta.n / l_max (column tiles of 16)");
   // initial norm (cudaOpenMP2.cu:437-440 would abort at the first step)
   const long G = (long)s.M * s.nt;
   double norm = 0.0;
@@ -576,31 +578,48 @@ StepArgs Engine::args(int kind) const {
 
 void Engine::enqueue(hipEvent_t *ev) {
   hipStream_t s = stream_;
-  auto mark = [&](int q) {
-    if (ev) QD_HIP(hipEventRecord(ev[q], s));
+  // with ev: events ev[2k], ev[2k+1] bracket the launch of kernel kind k
+  // (kinds fused into another kernel get none)
+  auto begin = [&](int k) {
+    if (ev) QD_HIP(hipEventRecord(ev[2 * k], s));
   };
-  mark(0);
-  mark(1);
+  auto end = [&](int k) {
+    if (ev) QD_HIP(hipEventRecord(ev[2 * k + 1], s));
+  };
+  begin(QD_K_LEG_FWD);
   launch_leg_fwd(args(KK_LEG_FWD), total_[KK_LEG_FWD], fwd_tiles_, fwd_kp_, mixed_, s);
-  mark(2);
-  mark(3);
+  end(QD_K_LEG_FWD);
+  begin(QD_K_FFT_R_FWD);
   if (ct_n1_) launch_fft_R_ct(args(KK_FFT_R), total_[KK_FFT_R], 0, 0, ct_n1_, nb_max_, s);
   else launch_fft_R_rt(args(KK_FFT_R), total_[KK_FFT_R], 0, 0, nb_max_, lds_fftR_, s);
-  mark(4);
+  end(QD_K_FFT_R_FWD);
+  begin(QD_K_FFT_r);
   if (ct_n2_) launch_fft_r_ct(args(KK_FFT_r), total_[KK_FFT_r], ct_n2_, s);
   else launch_fft_r_rt(args(KK_FFT_r), total_[KK_FFT_r], lds_fftr_, s);
-  mark(5);
+  end(QD_K_FFT_r);
+  begin(QD_K_FFT_R_INV);
   if (ct_n1_) launch_fft_R_ct(args(KK_FFT_R), total_[KK_FFT_R], 1, any_acf_, ct_n1_, nb_max_, s);
   else launch_fft_R_rt(args(KK_FFT_R), total_[KK_FFT_R], 1, any_acf_, nb_max_, lds_fftR_, s);
-  mark(6);
-  mark(7);
+  end(QD_K_FFT_R_INV);
+  begin(QD_K_LEG_BWD);
   launch_leg_bwd(args(KK_LEG_BWD), total_[KK_LEG_BWD], bwd_tiles_, bwd_kp_, mixed_, s);
-  mark(8);
-  if (any_flux_) launch_flux(args(KK_FLUX), total_[KK_FLUX], s);
-  mark(9);
+  end(QD_K_LEG_BWD);
+  if (any_flux_) {
+    begin(QD_K_OBS);
+    launch_flux(args(KK_FLUX), total_[KK_FLUX], s);
+    end(QD_K_OBS);
+  }
+  begin(QD_K_REDUCE);
   launch_reduce(args(KK_REDUCE), total_[KK_REDUCE], s);
-  mark(10);
+  end(QD_K_REDUCE);
   QD_HIP(hipGetLastError());
+}
+
+unsigned Engine::timed_kinds() const {
+  unsigned m = (1u << QD_K_LEG_FWD) | (1u << QD_K_FFT_R_FWD) | (1u << QD_K_FFT_r) | (1u << QD_K_FFT_R_INV) |
+               (1u << QD_K_LEG_BWD) | (1u << QD_K_REDUCE);
+  if (any_flux_) m |= 1u << QD_K_OBS;
+  return m;
 }
 
 void Engine::step(hipEvent_t *ev) {

@@ -222,8 +222,9 @@ qd_status qd_get_crp(qd_handle *h, double *crp) {
 qd_status qd_profile_steps(qd_handle *h, int32_t n_steps, double *us) {
   if (!h || !us || n_steps <= 0) return fail(h, QD_ERR_INVALID, "bad arguments");
   QD_GUARD(h, {
-    hipEvent_t ev[QD_KERNEL_KINDS + 1];
+    hipEvent_t ev[2 * QD_KERNEL_KINDS];
     for (auto &e : ev) QD_HIP(hipEventCreate(&e));
+    const unsigned timed = h->engine->timed_kinds();
     double acc[QD_KERNEL_KINDS] = {};
     qd_status st = QD_OK;
     for (int s = 0; s < n_steps && st == QD_OK; s++) {
@@ -232,17 +233,18 @@ qd_status qd_profile_steps(qd_handle *h, int32_t n_steps, double *us) {
       h->records_enqueued++;
       h->job->steps++;
       h->job->flux_after_step(h->stream);
-      QD_HIP(hipEventSynchronize(ev[QD_KERNEL_KINDS]));
+      QD_HIP(hipStreamSynchronize(h->stream));
       for (int k = 0; k < QD_KERNEL_KINDS; k++) {
+        if (!(timed & (1u << k))) continue;
         float ms = 0.f;
-        QD_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+        QD_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
         acc[k] += ms * 1000.0;
       }
       if (h->records_enqueued - h->job->records_done >= h->job->dev.ring) st = sync_and_check(h);
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
     if (st == QD_OK) st = sync_and_check(h);
-    // FFT_R appears twice and FBR twice per step; report per launch
+    // mean per launch; the R pass launches twice per step, as two kinds
     for (int k = 0; k < QD_KERNEL_KINDS; k++) us[k] = acc[k] / n_steps;
     return st;
   })

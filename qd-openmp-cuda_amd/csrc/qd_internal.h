@@ -136,6 +136,8 @@ __device__ __forceinline__ int locate_job(const QD_G int *base, int n_jobs, int 
 // Host-side launchers (each enqueues one kernel on `stream`).
 // step-0 phase exp(+i dt/2 V) on every block of one job (before its first step)
 void launch_prephase(double2 *psi, const double *pot, long G, int nO, double dt, hipStream_t stream);
+// column tile counts (N / 16) with a Legendre GEMM instantiation
+bool leg_tiles_supported(int nt_tiles);
 void launch_leg_fwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, hipStream_t stream);
 void launch_leg_bwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, hipStream_t stream);
 // FFT along R fused with rotational . Coriolis . rotational (before the forward

@@ -25,6 +25,7 @@ Conventions fixed here (the reference leaves them to un-vendored code):
 from __future__ import annotations
 
 import dataclasses
+import functools
 import math
 from typing import Optional, Sequence
 
@@ -265,6 +266,26 @@ class JobSpec:
     seed: Optional[int] = None
 
 
+@functools.lru_cache(maxsize=8)
+def _potential_cached(leps, R_range, r_range, n1, n2, ntheta, v_cap):
+    """The potential depends only on the surface and the grid: shared by every
+    job of one shape (read-only)."""
+    R = np.linspace(R_range[0], R_range[1], n1)
+    r = np.linspace(r_range[0], r_range[1], n2)
+    x, _ = gauss_legendre(ntheta)
+    V = leps.jacobi(R, r, x, v_cap)
+    V.flags.writeable = False
+    return V
+
+
+@functools.lru_cache(maxsize=64)
+def _legendre_cached(omegas, l_max, ntheta):
+    x, _ = gauss_legendre(ntheta)
+    P = legendre_tables(np.array(omegas, dtype=np.int32), l_max, x)
+    P.flags.writeable = False
+    return P
+
+
 def build_job(spec: JobSpec) -> dict:
     """Build every input array of one partial-wave job in the reference layouts."""
     mF, mH1, mH2 = (m * MASS_AU for m in spec.masses)
@@ -276,8 +297,8 @@ def build_job(spec: JobSpec) -> dict:
     l_max = spec.ntheta - 2 if spec.l_max is None else spec.l_max
     J, p = spec.J, (0 if spec.J == 0 else spec.parity)
     omegas = omega_list(J, p, l_max, spec.k_max if spec.k_max >= 0 else None)
-    P = legendre_tables(omegas, l_max, x)
-    V = spec.leps.jacobi(r1["r"], r2["r"], x, spec.v_cap)
+    P = _legendre_cached(tuple(int(o) for o in omegas), l_max, spec.ntheta)
+    V = _potential_cached(spec.leps, spec.R_range, spec.r_range, spec.n1, spec.n2, spec.ntheta, spec.v_cap)
 
     R0 = spec.R0
     if spec.seed is not None:

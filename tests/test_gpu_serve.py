@@ -1,0 +1,110 @@
+"""Serving form (qd_server_*): jobs of different shapes, block counts and
+precisions stepped together, admitted by memory footprint while others run.
+
+Bar: every job's result in the mix equals its result alone (qd_evolve on the
+same inputs) bitwise, in both precisions: the reductions are per job and in a
+fixed order, and no kernel mixes data of two jobs.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from qd_openmp_cuda_amd import abi, api, inputs
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _spec(n1, n2, nt, J, p, k_max, steps, **kw):
+    return inputs.JobSpec(n1=n1, n2=n2, ntheta=nt, R_range=(2.0, 12.0), r_range=(0.5, 5.0), J=J, parity=p,
+                          k_max=k_max, j0=max(1 if (J + p) % 2 and J else 0, 1), E_coll=0.02,
+                          total_steps=steps, **kw)
+
+
+def _pair(spec, precision=abi.QD_PREC_FP64):
+    job = inputs.build_job(spec)
+    alone = abi.Inputs(job, precision=precision, want_acf=False)
+    mixed = abi.Inputs(job, precision=precision, want_acf=False)
+    api.evolve(alone)
+    return alone, mixed
+
+
+def _same(a, b):
+    assert b.steps[0] == a.steps[0]
+    assert np.array_equal(a.psi, b.psi)
+    assert np.array_equal(a.obs, b.obs)
+    assert np.array_equal(a.crp, b.crp)
+
+
+def test_serve_mix_equals_alone():
+    specs = [
+        (_spec(64, 32, 24, 5, 1, 3, 30, n_energies=6, flux_r=2.8, energy_range=(0.012, 0.03)), abi.QD_PREC_FP64),
+        (_spec(48, 32, 20, 2, 0, -1, 20), abi.QD_PREC_FP64),
+        (_spec(64, 32, 24, 0, 0, -1, 25), abi.QD_PREC_FP64),
+        (_spec(64, 32, 24, 3, 0, 2, 15), abi.QD_PREC_MIXED),
+        (_spec(64, 32, 24, 4, 1, 4, 18), abi.QD_PREC_FP64),
+    ]
+    _say("case serve mix: 5 jobs, 3 shapes, fp64 + mixed, staggered submission")
+    pairs = [_pair(s, p) for s, p in specs]
+    with api.Server() as srv:
+        ids = [srv.submit(pairs[0][1]), srv.submit(pairs[1][1])]
+        for _ in range(5):
+            srv.step()
+        ids.append(srv.submit(pairs[2][1]))
+        ids.append(srv.submit(pairs[3][1]))
+        for _ in range(3):
+            srv.step()
+        ids.append(srv.submit(pairs[4][1]))
+        for _ in range(200):
+            nr, nq = srv.step()
+            if nr == 0 and nq == 0:
+                break
+        st = [srv.status(i) for i in ids]
+        _say(f"  states {[s['state'] for s in st]} steps {[s['steps'] for s in st]}")
+        assert all(s["state"] == api.Server.DONE for s in st)
+    for q, (alone, mixed) in enumerate(pairs):
+        _say(f"  job {q}: max |dpsi| {np.abs(alone.psi - mixed.psi).max():.3e}")
+        _same(alone, mixed)
+
+
+def test_serve_admission_by_memory_budget():
+    spec = _spec(64, 32, 24, 5, 1, 3, 12)
+    _say("case serve budget: 3 jobs of 64x32x24 J=5, budget for one")
+    with api.Server() as probe:
+        a = abi.Inputs(inputs.build_job(spec), want_acf=False)
+        probe.submit(a)
+        probe.step()
+        one, _ = probe.memory()
+    pairs = [_pair(spec) for _ in range(3)]
+    with api.Server(budget_bytes=int(one * 1.5)) as srv:
+        ids = [srv.submit(m) for _, m in pairs]
+        nr, nq = srv.step()
+        _say(f"  after first step: running {nr} queued {nq} (one job = {one} B)")
+        assert (nr, nq) == (1, 2)
+        for _ in range(100):
+            nr, nq = srv.step()
+            if nr == 0 and nq == 0:
+                break
+        st = [srv.status(i) for i in ids]
+        assert all(s["state"] == api.Server.DONE for s in st)
+        assert st[1]["queued_s"] > 0.0 and st[2]["queued_s"] >= st[1]["queued_s"]
+        used, budget = srv.memory()
+        assert used == 0 and budget == int(one * 1.5)
+    for alone, mixed in pairs:
+        _same(alone, mixed)
+
+
+def test_serve_rejects_oversize_and_copyback():
+    spec = _spec(64, 32, 24, 2, 0, -1, 4)
+    job = inputs.build_job(spec)
+    with api.Server(budget_bytes=1 << 20) as srv:
+        with pytest.raises(api.QDError):
+            srv.submit(abi.Inputs(job))
+    with api.Server() as srv:
+        with pytest.raises(api.QDError):
+            srv.submit(abi.Inputs(job, copy_every=2, callback=lambda user, steps: None))
