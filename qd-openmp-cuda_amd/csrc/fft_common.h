@@ -106,10 +106,11 @@ struct FbrSums {
 // The last two equal the DVR norm and autocorrelation of the state after the
 // backward Legendre transform: the Gauss-Legendre pair is an isometry between
 // the FBR coefficients and the band-limited DVR values.
-template <bool ENERGY, int NB, int NF>
+// phi points at the phases corPhi(l, i, a), a = 0.. (global table or an LDS copy).
+template <bool ENERGY, int NB, int NF, class PhiPtr>
 __device__ __forceinline__ void coriolis_point(const JobDev &jb, double2 (&x)[NB], int nb, int l, int i, int j,
                                                const QD_G double *__restrict__ V, FbrSums<NB, NF> &sm,
-                                               const QD_G double2 *c0, long LM) {
+                                               const QD_G double2 *c0, long LM, PhiPtr phi) {
   const int nO = jb.nO;
   if (ENERGY) {
     const double I1 = jb.I1[i];
@@ -123,7 +124,6 @@ __device__ __forceinline__ void coriolis_point(const JobDev &jb, double2 (&x)[NB
       }
     }
   }
-  const QD_G double2 *phi = jb.corPhi + ((long)l * jb.n1 + i) * nO;
   double2 t[NB];
 #pragma unroll
   for (int al = 0; al < NB; al++) {

@@ -81,6 +81,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
   constexpr int PPT = (PN + kBlock - 1) / kBlock;
   __shared__ double2 A[PN];
   __shared__ double2 tw[N1];
+  __shared__ double2 phs[N1 * NB];   // corPhi(l, i, a) of this slab, reused by the P rows
   __shared__ double red[4 * NF * NB];
   const int job = locate_job(a.unit_base, a.n_jobs, blockIdx.x);
   const JobDev &jb = a.jobs[job];
@@ -94,6 +95,15 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
   QD_G double2 *base = jb.fbr + (long)l * M + (long)g * PN;
   const QD_G double2 *c0 = ACF ? jb.c0 + (long)l * M + (long)g * PN : nullptr;
   for (int e = threadIdx.x; e < N1; e += kBlock) tw[e] = jb.twR[e];
+  {
+    const QD_G double2 *cp = jb.corPhi + (long)l * N1 * jb.nO;
+    for (int e = threadIdx.x; e < N1 * NB; e += kBlock) {
+      const int i = e / NB, a2 = e % NB;
+      const double2 v = cp[i * jb.nO + min(a2, jb.nO - 1)];
+      phs[e] = a2 < jb.nO ? v : make_double2(0.0, 0.0);
+    }
+  }
+  __syncthreads();
   double2 x[NB][PPT];
   FbrSums<NB, NF> sm;
   sm.zero();
@@ -114,7 +124,8 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
         double2 xp[NB];
 #pragma unroll
         for (int b = 0; b < NB; b++) xp[b] = x[b][k];
-        coriolis_point<false, NB, NF>(jb, xp, nb, l, e % N1, g * P + e / N1, V, sm, nullptr, LM);
+        coriolis_point<false, NB, NF>(jb, xp, nb, l, e % N1, g * P + e / N1, V, sm, nullptr, LM,
+                                      (const double2 *)phs + (e % N1) * NB);
 #pragma unroll
         for (int b = 0; b < NB; b++) x[b][k] = xp[b];
       }
@@ -175,7 +186,8 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
         double2 xp[NB];
 #pragma unroll
         for (int b = 0; b < NB; b++) xp[b] = x[b][k];
-        coriolis_point<true, NB, NF>(jb, xp, nb, l, e % N1, g * P + e / N1, V, sm, ACF ? c0 + e : nullptr, LM);
+        coriolis_point<true, NB, NF>(jb, xp, nb, l, e % N1, g * P + e / N1, V, sm, ACF ? c0 + e : nullptr, LM,
+                                     (const double2 *)phs + (e % N1) * NB);
 #pragma unroll
         for (int b = 0; b < NB; b++)
           if (b < nb) base[b * LM + e] = xp[b];

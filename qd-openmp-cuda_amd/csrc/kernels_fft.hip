@@ -108,7 +108,8 @@ __global__ __launch_bounds__(kBlock) void k_fft_R(StepArgs a) {
       double2 x[NB];
 #pragma unroll
       for (int b = 0; b < NB; b++) x[b] = (b < nb) ? X[b * PN + e] : make_double2(0.0, 0.0);
-      coriolis_point<false, NB, NF>(jb, x, nb, l, e % n1, g * P + e / n1, V, sm, nullptr, LM);
+      coriolis_point<false, NB, NF>(jb, x, nb, l, e % n1, g * P + e / n1, V, sm, nullptr, LM,
+                                    jb.corPhi + ((long)l * n1 + e % n1) * jb.nO);
 #pragma unroll
       for (int b = 0; b < NB; b++)
         if (b < nb) X[b * PN + e] = x[b];
@@ -128,7 +129,8 @@ __global__ __launch_bounds__(kBlock) void k_fft_R(StepArgs a) {
       double2 x[NB];
 #pragma unroll
       for (int b = 0; b < NB; b++) x[b] = (b < nb) ? cscale(X[b * PN + e], s) : make_double2(0.0, 0.0);
-      coriolis_point<true, NB, NF>(jb, x, nb, l, e % n1, g * P + e / n1, V, sm, ACF ? c0 + e : nullptr, LM);
+      coriolis_point<true, NB, NF>(jb, x, nb, l, e % n1, g * P + e / n1, V, sm, ACF ? c0 + e : nullptr, LM,
+                                   jb.corPhi + ((long)l * n1 + e % n1) * jb.nO);
 #pragma unroll
       for (int b = 0; b < NB; b++)
         if (b < nb) X[b * PN + e] = x[b];

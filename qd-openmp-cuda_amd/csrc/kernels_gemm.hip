@@ -27,6 +27,8 @@
 // The mixed form splits both operands into FP32 hi/lo parts and accumulates
 // hi*hi and hi*lo + lo*hi with v_mfma_f32_16x16x4f32 in FP32, flushing the FP32
 // accumulators into FP64 every 16 k-terms.
+#include <cstdlib>
+
 #include "qd_internal.h"
 #include "qd_complex.h"
 
@@ -37,7 +39,31 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 
 __host__ __device__ constexpr int lds_stride(int nt_tiles) { return nt_tiles * 16 + ((nt_tiles & 1) ? 0 : 16); }
 
-constexpr int kCh = 8;   // k-steps (of 4 k each) per pipelined tile
+// k-steps (of 4 k each) per pipelined tile: a template parameter KC in
+// {8, 12, 16}; KC = 8 runs three tiles deep, 12 and 16 two deep.  Deeper
+// pipelines keep more bytes in flight per wave but cost registers; the choice
+// keeps two waves per SIMD where the tile allows it.  The forward pass also
+// holds the phase operands, so it runs KC = 8 three deep; the backward pass
+// holds the potentials of the epilogue and runs one or two wide tiles per
+// strip.
+inline int leg_chunk(int ksteps, bool fwd, int nt_tiles) {
+  if (fwd || nt_tiles > 4) return 8;
+  int best = 8, waste = 1 << 30;
+  for (int kc : {16, 12, 8}) {
+    const int w = (ksteps + kc - 1) / kc * kc - ksteps;
+    if (w < waste) { waste = w; best = kc; }
+  }
+  return best;
+}
+
+// Logical workgroup index with the workgroups of one XCD contiguous: the
+// hardware deals workgroups round-robin over the 8 XCDs (XCD = id % 8), each
+// with its own L2; consecutive logical units (the omega blocks of one row
+// group, which read the same phase / potential rows) then share an L2.
+__device__ __forceinline__ int xcd_logical_unit(int p, int total) {
+  const int q = total >> 3, r = total & 7, x = p & 7, slot = p >> 3;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + slot;
+}
 
 // One wave's view of a Legendre GEMM: C (rows x N, ldc) = op(A) (rows x K, lda) . B.
 struct GemmWave {
@@ -76,13 +102,19 @@ __device__ __forceinline__ int acc_col(int t, int lk, int it) {
 }
 
 // Computes the strips strip0 + s * sstride (s < nstrips) of one wave.  The
-// (strip, k-chunk) tiles are software-pipelined: the loads of the next tile
-// (kCh A values and, forward, kCh phase components per lane) are in flight
-// while the current tile runs on the matrix cores.
+// (strip, k-chunk) tiles are software-pipelined DEPTH deep: the loads of tile
+// t + DEPTH - 1 (kCh A values and, forward, kCh phase components per lane)
+// are issued before tile t runs on the matrix cores, into a ring of DEPTH
+// register buffers indexed at compile time (the tile loop is unrolled by
+// DEPTH).  Loads are unconditional from clamped, always in-bounds addresses
+// (tiles past the end reload the last one) and raw values are masked only
+// where they are used, so the compiler waits for exactly the oldest tile.
+// The backward epilogue's potential loads are issued before the prefetch of
+// the same tile, so waiting for them does not wait for the prefetch.
 //   PHASE: multiply each A element by d1 d2 exp(-i dt V) (forward prologue);
 //          a lane holds one real component, its partner lane (r ^ 1) the other.
 //   OBS:   accumulate w V |psi|^2 of the output (backward epilogue) into ob.
-template <int NT, bool MIXED, bool PHASE, bool OBS>
+template <int NT, int kCh, int DEPTH, bool MIXED, bool PHASE, bool OBS>
 __device__ __forceinline__ void gemm_wave(const GemmWave &gw, long strip0, int nstrips, int sstride, double &ob) {
   const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
   const bool odd = lr & 1;
@@ -91,36 +123,29 @@ __device__ __forceinline__ void gemm_wave(const GemmWave &gw, long strip0, int n
   for (int s = 0; s < nstrips; s++)
     if ((strip0 + (long)s * sstride) * 16 < gw.rows) ns = s + 1;
   const int ntiles = ns * nch;
-  double an[kCh], anx[kCh], tn[kCh], tnx[kCh];
-  // Loads are unconditional from clamped (always in-bounds) addresses and the
-  // out-of-range values are zeroed afterwards: without branches around the
-  // loads the compiler can count them and wait only for the older tile.
+  if (ntiles <= 0) return;
+  double abuf[DEPTH][kCh], tbuf[DEPTH][kCh];
+  auto row_of = [&](int tile) { return (strip0 + (long)(tile / nch) * sstride) * 16 + lr; };
   auto load = [&](int tile, double(&a)[kCh], double(&tp)[kCh]) {
-    const int s = tile / nch, c = tile % nch;
-    const long r = (strip0 + (long)s * sstride) * 16 + lr;
-    const bool rok = r < gw.rows;
-    const long rc = rok ? r : gw.rows - 1;
+    tile = min(tile, ntiles - 1);
+    const int c = tile % nch;
+    const long r = row_of(tile);
+    const long rc = r < gw.rows ? r : gw.rows - 1;
 #pragma unroll
     for (int q = 0; q < kCh; q++) {
-      const int k = 4 * (c * kCh + q) + lk;
-      const bool ok = rok && k < gw.K;
-      const long off = (long)min(k, gw.K - 1) * gw.lda + rc;
-      const double av = gw.A[off];
-      a[q] = ok ? av : 0.0;
-      // even lane takes Re T, odd lane Im T of the same complex phase
-      if (PHASE) {
-        const double tv = gw.vph[off];
-        tp[q] = ok ? tv : (odd ? 0.0 : 1.0);
-      }
+      const int k = min(4 * (c * kCh + q) + lk, gw.K - 1);
+      const long off = (long)k * gw.lda + rc;
+      a[q] = gw.A[off];
+      if (PHASE) tp[q] = gw.vph[off];   // even lane Re T, odd lane Im T
     }
   };
   v4d acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; t++) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
-  if (ntiles > 0) load(0, an, tn);
-  for (int tile = 0; tile < ntiles; tile++) {
-    if (tile + 1 < ntiles) load(tile + 1, anx, tnx);
+  auto compute = [&](int tile, const double(&an)[kCh], const double(&tn)[kCh], const double(&vq)[NT][4]) {
     const int c = tile % nch;
+    const long r = row_of(tile);
+    const bool rok = r < gw.rows;
     if (c == 0) {
 #pragma unroll
       for (int t = 0; t < NT; t++) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
@@ -128,14 +153,17 @@ __device__ __forceinline__ void gemm_wave(const GemmWave &gw, long strip0, int n
     double ac[kCh];
 #pragma unroll
     for (int q = 0; q < kCh; q++) {
+      const bool ok = rok && 4 * (c * kCh + q) + lk < gw.K;
+      const double av = ok ? an[q] : 0.0;
       if (PHASE) {
-        const double oa = swap_adjacent(an[q]);
-        const double ot = swap_adjacent(tn[q]);
-        const double Tr = odd ? ot : tn[q], Ti = odd ? tn[q] : ot;
+        const double tv = ok ? tn[q] : (odd ? 0.0 : 1.0);
+        const double oa = swap_adjacent(av);
+        const double ot = swap_adjacent(tv);
+        const double Tr = odd ? ot : tv, Ti = odd ? tv : ot;
         // re' = re Tr - im Ti (even lane), im' = re Ti + im Tr (odd lane)
-        ac[q] = fma(odd ? oa : -oa, Ti, an[q] * Tr);
+        ac[q] = fma(odd ? oa : -oa, Ti, av * Tr);
       } else {
-        ac[q] = an[q];
+        ac[q] = av;
       }
     }
     if (!MIXED) {
@@ -176,20 +204,7 @@ __device__ __forceinline__ void gemm_wave(const GemmWave &gw, long strip0, int n
           for (int it = 0; it < 4; it++) acc[t][it] += (double)hh[t][it] + (double)hl[t][it];
       }
     }
-    if (c == nch - 1) {   // epilogue of strip tile / nch
-      const long r = (strip0 + (long)(tile / nch) * sstride) * 16 + lr;
-      const bool rok = r < gw.rows;
-      const long m = (rok ? r : gw.rows - 1) >> 1;
-      double vq[NT][4];
-      if (OBS) {   // issue every potential load before using any of them
-#pragma unroll
-        for (int t = 0; t < NT; t++)
-#pragma unroll
-          for (int it = 0; it < 4; it++) {
-            const int cc = min(acc_col<MIXED>(t, lk, it), gw.N - 1);
-            vq[t][it] = gw.pot[(long)cc * gw.M + m];
-          }
-      }
+    if (c == nch - 1) {   // epilogue of the strip
 #pragma unroll
       for (int t = 0; t < NT; t++) {
 #pragma unroll
@@ -202,22 +217,42 @@ __device__ __forceinline__ void gemm_wave(const GemmWave &gw, long strip0, int n
         }
       }
     }
+  };
+  // potentials of the strip of `tile` (backward epilogue), issued before the prefetch
+  auto load_pot = [&](int tile, double(&vq)[NT][4]) {
+    const long r = row_of(tile);
+    const long m = (r < gw.rows ? r : gw.rows - 1) >> 1;
 #pragma unroll
-    for (int q = 0; q < kCh; q++) {
-      an[q] = anx[q];
-      if (PHASE) tn[q] = tnx[q];
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+      for (int it = 0; it < 4; it++) {
+        const int cc = min(acc_col<MIXED>(t, lk, it), gw.N - 1);
+        vq[t][it] = gw.pot[(long)cc * gw.M + m];
+      }
+  };
+#pragma unroll
+  for (int d = 0; d < DEPTH - 1; d++) load(d, abuf[d], tbuf[d]);
+  for (int t0 = 0; t0 < ntiles; t0 += DEPTH) {
+#pragma unroll
+    for (int d = 0; d < DEPTH; d++) {
+      const int tile = t0 + d;
+      double vq[NT][4];
+      if (OBS && tile < ntiles && tile % nch == nch - 1) load_pot(tile, vq);
+      load(tile + DEPTH - 1, abuf[(d + DEPTH - 1) % DEPTH], tbuf[(d + DEPTH - 1) % DEPTH]);
+      if (tile < ntiles) compute(tile, abuf[d], tbuf[d], vq);
     }
   }
 }
 
-template <int NT, bool FWD, bool MIXED>
+template <int NT, int KC, bool FWD, bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_leg(StepArgs a) {
   extern __shared__ double lds_d[];
   __shared__ double red[4];
   __shared__ double w_s[FWD ? 1 : kMaxTheta];
-  const int job = locate_job(a.unit_base, a.n_jobs, blockIdx.x);
+  const int lu = xcd_logical_unit(blockIdx.x, gridDim.x);
+  const int job = locate_job(a.unit_base, a.n_jobs, lu);
   const JobDev &jb = a.jobs[job];
-  const int u = blockIdx.x - a.unit_base[job];
+  const int u = lu - a.unit_base[job];
   const int groups = FWD ? jb.gemm_fwd_groups : jb.gemm_bwd_groups;
   if (!FWD)
     for (int e = threadIdx.x; e < jb.nt; e += kBlock) w_s[e] = jb.w[e];
@@ -264,7 +299,7 @@ __global__ __launch_bounds__(kBlock) void k_leg(StepArgs a) {
   gw.ws = w_s; gw.pot = jb.pot; gw.M = M;
   double ob = 0.0;
   const int S = jb.gemm_strips;
-  gemm_wave<NT, MIXED, FWD, !FWD>(gw, (long)g * S * 4 + wave, S, 4, ob);
+  gemm_wave<NT, KC, KC == 8 ? 3 : 2, MIXED, FWD, !FWD>(gw, (long)g * S * 4 + wave, S, 4, ob);
   if (!FWD) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) ob += __shfl_xor(ob, off, 64);
@@ -274,31 +309,49 @@ __global__ __launch_bounds__(kBlock) void k_leg(StepArgs a) {
   }
 }
 
+template <int NT, int KC, bool FWD>
+static void launch_leg_k(const StepArgs &a, int total_units, int kp_max, int mixed, hipStream_t s) {
+  const size_t lds = (size_t)kp_max * lds_stride(NT) * (mixed ? 2 * sizeof(float) : sizeof(double));
+  if (mixed) hipLaunchKernelGGL((k_leg<NT, KC, FWD, true>), dim3(total_units), dim3(kBlock), lds, s, a);
+  else hipLaunchKernelGGL((k_leg<NT, KC, FWD, false>), dim3(total_units), dim3(kBlock), lds, s, a);
+}
+
+// QD_LEG_KC=8|12|16 overrides the k-chunk choice (tuning sweeps)
+static int leg_chunk_override() {
+  static const int v = [] {
+    const char *e = std::getenv("QD_LEG_KC");
+    const int k = e ? std::atoi(e) : 0;
+    return (k == 8 || k == 12 || k == 16) ? k : 0;
+  }();
+  return v;
+}
+
+template <int NT, bool FWD>
+static void launch_leg_nt(const StepArgs &a, int total_units, int kp_max, int mixed, hipStream_t s) {
+  const int kc = leg_chunk_override() ? leg_chunk_override() : leg_chunk(kp_max / 4, FWD, NT);
+  switch (NT <= 8 ? kc : 8) {
+    case 16: launch_leg_k<NT, NT <= 8 ? 16 : 8, FWD>(a, total_units, kp_max, mixed, s); break;
+    case 12: launch_leg_k<NT, NT <= 8 ? 12 : 8, FWD>(a, total_units, kp_max, mixed, s); break;
+    default: launch_leg_k<NT, 8, FWD>(a, total_units, kp_max, mixed, s); break;
+  }
+}
+
 template <bool FWD>
 static void launch_leg(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, hipStream_t s) {
-  const dim3 g(total_units), t(kBlock);
-#define QD_LEG_CASE(NT)                                                                              \
-  case NT: {                                                                                         \
-    const size_t lds = (size_t)kp_max * lds_stride(NT) * (mixed ? 2 * sizeof(float) : sizeof(double)); \
-    if (mixed) hipLaunchKernelGGL((k_leg<NT, FWD, true>), g, t, lds, s, a);                          \
-    else hipLaunchKernelGGL((k_leg<NT, FWD, false>), g, t, lds, s, a);                               \
-    break;                                                                                           \
-  }
   switch (nt_tiles) {
-    QD_LEG_CASE(1)
-    QD_LEG_CASE(2)
-    QD_LEG_CASE(3)
-    QD_LEG_CASE(4)
-    QD_LEG_CASE(5)
-    QD_LEG_CASE(6)
-    QD_LEG_CASE(7)
-    QD_LEG_CASE(8)
-    QD_LEG_CASE(10)
-    QD_LEG_CASE(12)
-    QD_LEG_CASE(15)
-    default: break;
+    case 1: launch_leg_nt<1, FWD>(a, total_units, kp_max, mixed, s); break;
+    case 2: launch_leg_nt<2, FWD>(a, total_units, kp_max, mixed, s); break;
+    case 3: launch_leg_nt<3, FWD>(a, total_units, kp_max, mixed, s); break;
+    case 4: launch_leg_nt<4, FWD>(a, total_units, kp_max, mixed, s); break;
+    case 5: launch_leg_nt<5, FWD>(a, total_units, kp_max, mixed, s); break;
+    case 6: launch_leg_nt<6, FWD>(a, total_units, kp_max, mixed, s); break;
+    case 7: launch_leg_nt<7, FWD>(a, total_units, kp_max, mixed, s); break;
+    case 8: launch_leg_nt<8, FWD>(a, total_units, kp_max, mixed, s); break;
+    case 10: launch_leg_nt<10, FWD>(a, total_units, kp_max, mixed, s); break;
+    case 12: launch_leg_nt<12, FWD>(a, total_units, kp_max, mixed, s); break;
+    case 15: launch_leg_nt<15, FWD>(a, total_units, kp_max, mixed, s); break;
+    default: break;   // rejected by leg_tiles_supported at validation
   }
-#undef QD_LEG_CASE
 }
 
 bool leg_tiles_supported(int nt_tiles) {
@@ -332,7 +385,7 @@ __global__ __launch_bounds__(kBlock) void k_gemm_probe(const double *A, int lda,
   gw.C = C; gw.ldc = ldc; gw.N = N; gw.rows = rows;
   gw.Bs = Bs; gw.Ws = Ws;
   double ob = 0.0;
-  gemm_wave<NT, false, false, false>(gw, r0 / 16, 1, 1, ob);
+  gemm_wave<NT, 8, 2, false, false, false>(gw, r0 / 16, 1, 1, ob);
 }
 
 void launch_gemm_probe(const double *A, int lda, const double *B, int kp, int np, double *C, int ldc, int rows,
@@ -358,12 +411,21 @@ void launch_gemm_probe(const double *A, int lda, const double *B, int kp, int np
 #undef QD_PROBE_CASE
 }
 
+template <int NT, int KC>
+static void set_leg_attrs_k(int lim) {
+  (void)hipFuncSetAttribute((const void *)k_leg<NT, KC, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+  (void)hipFuncSetAttribute((const void *)k_leg<NT, KC, false, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+  (void)hipFuncSetAttribute((const void *)k_leg<NT, KC, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+  (void)hipFuncSetAttribute((const void *)k_leg<NT, KC, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+}
+
 template <int NT>
 static void set_leg_attrs(int lim) {
-  (void)hipFuncSetAttribute((const void *)k_leg<NT, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
-  (void)hipFuncSetAttribute((const void *)k_leg<NT, false, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
-  (void)hipFuncSetAttribute((const void *)k_leg<NT, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
-  (void)hipFuncSetAttribute((const void *)k_leg<NT, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+  set_leg_attrs_k<NT, 8>(lim);
+  if (NT <= 8) {
+    set_leg_attrs_k<NT, NT <= 8 ? 12 : 8>(lim);
+    set_leg_attrs_k<NT, NT <= 8 ? 16 : 8>(lim);
+  }
 }
 
 void init_attrs_gemm() {

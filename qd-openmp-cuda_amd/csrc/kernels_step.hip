@@ -69,24 +69,32 @@ void launch_prephase(double2 *psi, const double *pot, long G, int nO, double dt,
 // block's partials in a fixed order and writes its part of the step record;
 // the last workgroup of a job (ticket in counters[2]) advances the counters.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double sum_contiguous(const QD_G double *src, long count, double *lds) {
-  double v[1];
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-  long q = threadIdx.x;
-  for (; q + 3 * kBlock < count; q += 4 * kBlock) {
-    a0 += src[q];
-    a1 += src[q + kBlock];
-    a2 += src[q + 2 * kBlock];
-    a3 += src[q + 3 * kBlock];
+// Sums of up to NS contiguous arrays at once: every thread walks the index
+// range of the longest array with one accumulator per array, so the loads of
+// all arrays are in flight together; clamped unconditional loads keep the
+// loop free of branches.  Fixed order for a fixed block size.
+template <int NS>
+__device__ __forceinline__ void sum_arrays(const QD_G double *const (&src)[NS], const long (&count)[NS],
+                                           double (&out)[NS], double *lds) {
+  long n = 0;
+#pragma unroll
+  for (int f = 0; f < NS; f++) {
+    out[f] = 0.0;
+    n = count[f] > n ? count[f] : n;
   }
-  for (; q < count; q += kBlock) a0 += src[q];
-  v[0] = (a0 + a1) + (a2 + a3);
-  block_sum<1>(v, lds);
-  return v[0];   // valid in thread 0
+#pragma unroll 8
+  for (long q = threadIdx.x; q < n; q += kBlock) {
+    double v[NS];
+#pragma unroll
+    for (int f = 0; f < NS; f++) v[f] = count[f] > 0 ? src[f][q < count[f] ? q : count[f] - 1] : 0.0;
+#pragma unroll
+    for (int f = 0; f < NS; f++) out[f] += q < count[f] ? v[f] : 0.0;
+  }
+  block_sum<NS>(out, lds);
 }
 
 __global__ __launch_bounds__(kBlock) void k_reduce(StepArgs a) {
-  __shared__ double red[16];   // block_sum<4> needs 4 waves x 4 values
+  __shared__ double red[4 * 7];   // block_sum<7>: 4 waves x 7 values
   const int job = locate_job(a.unit_base, a.n_jobs, blockIdx.x);
   const JobDev &jb = a.jobs[job];
   const int b = blockIdx.x - a.unit_base[job];
@@ -97,20 +105,19 @@ __global__ __launch_bounds__(kBlock) void k_reduce(StepArgs a) {
   const int gcols = jb.n1 / jb.fft_cols;
   long ke_off = 0;
   for (int q = 0; q < b; q++) ke_off += (long)(jb.L - (jb.omin + q)) * gcols;
-  const long nke = (long)(jb.L - (jb.omin + b)) * gcols;
-  const double ke = sum_contiguous(jb.part_ke + ke_off, nke, red);
   // second FBR half step partials of the inverse-R pass: [b][5][units] =
-  // Coriolis energy, rotational energy, norm, autocorrelation re / im
+  // Coriolis energy, rotational energy, norm, autocorrelation re / im;
+  // potential energy partials of the backward Legendre epilogue: [b][group]
   const long ncor = jb.units[KK_FFT_R];
   const QD_G double *pc = jb.part_cor + 5L * b * ncor;
-  const double ce = sum_contiguous(pc, ncor, red);
-  const double re = sum_contiguous(pc + ncor, ncor, red);
-  const double nr = sum_contiguous(pc + 2 * ncor, ncor, red);
-  const double ar = sum_contiguous(pc + 3 * ncor, ncor, red);
-  const double ai = sum_contiguous(pc + 4 * ncor, ncor, red);
-  // potential energy partials of the backward Legendre epilogue: [b][group]
-  const double pe = sum_contiguous(jb.part_obs + (long)b * jb.gemm_bwd_groups, jb.gemm_bwd_groups, red);
+  const QD_G double *const src[7] = {jb.part_ke + ke_off, pc, pc + ncor, pc + 2 * ncor, pc + 3 * ncor,
+                                     pc + 4 * ncor, jb.part_obs + (long)b * jb.gemm_bwd_groups};
+  const long count[7] = {(long)(jb.L - (jb.omin + b)) * gcols, ncor, ncor, ncor, ncor, ncor,
+                         (long)jb.gemm_bwd_groups};
+  double v[7];
+  sum_arrays<7>(src, count, v, red);
   if (threadIdx.x == 0) {
+    const double ke = v[0], ce = v[1], re = v[2], nr = v[3], ar = v[4], ai = v[5], pe = v[6];
     QD_G double *rec = jb.obs_ring + ((long)slot * nO + b) * 5;
     rec[0] = nr * area;
     rec[1] = pe * area;

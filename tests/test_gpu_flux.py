@@ -94,3 +94,44 @@ def test_crp_not_requested_is_an_error():
             lib_out = np.zeros(1)
             rc = api.lib().qd_get_crp(p.h, lib_out.ctypes.data)
             api._check(rc, p.h)
+
+
+def _resolved(job, frac):
+    eta = job["crp"]["eta_sq"]
+    return eta >= frac * eta.max()
+
+
+def test_flux_c2_shape_parity_200_energies():
+    """The 256x192x96 J=0 production shape with 200 energies, 20 steps: P(E)
+    against the oracle wherever the initial packet has energy content."""
+    job = inputs.build_job(inputs.config_spec("c4", total_steps=20))
+    _say("case flux 256x192x96 J=0 steps=20 energies=200")
+    a = abi.Inputs(job, want_acf=False, want_observables=False)
+    b = abi.Inputs(job, want_acf=False, want_observables=False)
+    oracle.evolve(a, 16)
+    api.evolve(b)
+    ok = _resolved(job, 1e-8)
+    e = float((np.abs(b.crp - a.crp) / np.maximum(1.0, np.abs(a.crp)))[ok].max())
+    _say(f"  crp vs oracle {e:.3e} over {int(ok.sum())} energies")
+    assert e < CRP_TOL
+
+
+def test_mixed_crp_error_c4_shape():
+    """Mixed-precision Legendre GEMMs against FP64 on the 256x192x96 J=0
+    problem, 2000 steps (the flux has passed the surface by then), 200
+    energies: |dP| <= 1e-6 wherever eta_sq >= 1e-2 of its maximum.  Beyond
+    that the error grows like 1/eta_sq (printed per energy)."""
+    job = inputs.build_job(inputs.config_spec("c4", total_steps=2000))
+    _say("case mixed crp 256x192x96 J=0 steps=2000 energies=200")
+    a = abi.Inputs(job, want_acf=False, want_observables=False, final_copy=False)
+    b = abi.Inputs(job, want_acf=False, want_observables=False, final_copy=False, precision=abi.QD_PREC_MIXED)
+    api.evolve(a)
+    api.evolve(b)
+    err = np.abs(b.crp - a.crp)
+    eta = job["crp"]["eta_sq"] / job["crp"]["eta_sq"].max()
+    for q in range(0, 200, 10):
+        _say(f"  E_coll {job['crp']['energies'][q] - job['meta']['e_int']:.4f}  eta/max {eta[q]:.1e}  "
+             f"P {a.crp[q]:+.4e}  |dP| {err[q]:.2e}")
+    ok = eta >= 1e-2
+    _say(f"  max |dP| {err[ok].max():.3e} over {int(ok.sum())} energies with eta >= 1e-2 max")
+    assert err[ok].max() <= 1e-6
