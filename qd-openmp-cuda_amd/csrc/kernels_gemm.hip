@@ -42,10 +42,10 @@ __host__ __device__ constexpr int lds_stride(int nt_tiles) { return nt_tiles * 1
 // k-steps (of 4 k each) per pipelined tile: a template parameter KC in
 // {8, 12, 16}; KC = 8 runs three tiles deep, 12 and 16 two deep.  Deeper
 // pipelines keep more bytes in flight per wave but cost registers; the choice
-// keeps two waves per SIMD where the tile allows it.  The forward pass also
-// holds the phase operands, so it runs KC = 8 three deep; the backward pass
-// holds the potentials of the epilogue and runs one or two wide tiles per
-// strip.
+// keeps two waves per SIMD (VGPRs + AGPRs <= 256) where the tile allows it.
+// The forward pass also holds the phase operands and runs KC = 8 three deep;
+// the backward pass holds the potentials of the epilogue and runs one or two
+// wide tiles per strip.
 inline int leg_chunk(int ksteps, bool fwd, int nt_tiles) {
   if (fwd || nt_tiles > 4) return 8;
   int best = 8, waste = 1 << 30;
@@ -82,6 +82,17 @@ struct GemmWave {
   const QD_G double *pot;        // backward epilogue: potential [nt][M]
   long M;
 };
+
+// Raw buffer access (32-bit offsets, hardware range check): a load whose byte
+// offset is at or past `bytes` returns 0.  The operand tiles read rows k >= K
+// of the k-padded product this way, so the zero padding costs no instruction.
+// Dword 3 holds the gfx9-family buffer format word.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ double buffer_load_f64(__amdgpu_buffer_rsrc_t r, uint32_t voff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, 0, 0));
+}
 
 // Value of the neighbouring lane (lane ^ 1) through a DPP quad permutation
 // (1, 0, 3, 2): a VALU move, no LDS round trip.
@@ -126,17 +137,36 @@ __device__ __forceinline__ void gemm_wave(const GemmWave &gw, long strip0, int n
   if (ntiles <= 0) return;
   double abuf[DEPTH][kCh], tbuf[DEPTH][kCh];
   auto row_of = [&](int tile) { return (strip0 + (long)(tile / nch) * sstride) * 16 + lr; };
+  // operand rows k < K only: rows K.. of the padded product read as zero
+  const uint32_t abytes = (uint32_t)((unsigned long)gw.K * (unsigned long)gw.lda * 8ul);
+  const __amdgpu_buffer_rsrc_t rA = make_rsrc(gw.A, abytes);
+  const __amdgpu_buffer_rsrc_t rT = make_rsrc(PHASE ? (const void *)gw.vph : (const void *)gw.A, abytes);
+  const uint32_t kstride = (uint32_t)gw.lda * 32u;   // bytes between k and k + 4
+  // The forward pass (two operands per element) loads through buffer
+  // descriptors; the backward pass keeps clamped global loads, which leave it
+  // the registers for two waves per SIMD, and masks rows k >= K where they
+  // are used.
+  constexpr bool BUF = PHASE;
   auto load = [&](int tile, double(&a)[kCh], double(&tp)[kCh]) {
     tile = min(tile, ntiles - 1);
     const int c = tile % nch;
     const long r = row_of(tile);
+    // rows past the end reread the last row: their products land in output
+    // rows that are never stored (row r of C depends on row r of A only)
     const long rc = r < gw.rows ? r : gw.rows - 1;
+    if (BUF) {
+      const uint32_t v0 = ((uint32_t)(4 * c * kCh + lk) * (uint32_t)gw.lda + (uint32_t)rc) * 8u;
 #pragma unroll
-    for (int q = 0; q < kCh; q++) {
-      const int k = min(4 * (c * kCh + q) + lk, gw.K - 1);
-      const long off = (long)k * gw.lda + rc;
-      a[q] = gw.A[off];
-      if (PHASE) tp[q] = gw.vph[off];   // even lane Re T, odd lane Im T
+      for (int q = 0; q < kCh; q++) {
+        a[q] = buffer_load_f64(rA, v0 + q * kstride);
+        if (PHASE) tp[q] = buffer_load_f64(rT, v0 + q * kstride);   // even lane Re T, odd lane Im T
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < kCh; q++) {
+        const int k = min(4 * (c * kCh + q) + lk, gw.K - 1);
+        a[q] = gw.A[(long)k * gw.lda + rc];
+      }
     }
   };
   v4d acc[NT];
@@ -153,10 +183,9 @@ __device__ __forceinline__ void gemm_wave(const GemmWave &gw, long strip0, int n
     double ac[kCh];
 #pragma unroll
     for (int q = 0; q < kCh; q++) {
-      const bool ok = rok && 4 * (c * kCh + q) + lk < gw.K;
-      const double av = ok ? an[q] : 0.0;
+      const double av = (BUF || (rok && 4 * (c * kCh + q) + lk < gw.K)) ? an[q] : 0.0;
       if (PHASE) {
-        const double tv = ok ? tn[q] : (odd ? 0.0 : 1.0);
+        const double tv = tn[q];
         const double oa = swap_adjacent(av);
         const double ot = swap_adjacent(tv);
         const double Tr = odd ? ot : tv, Ti = odd ? tv : ot;

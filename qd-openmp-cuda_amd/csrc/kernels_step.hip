@@ -65,42 +65,24 @@ void launch_prephase(double2 *psi, const double *pot, long G, int nO, double dt,
 }
 
 // ---------------------------------------------------------------------------
-// Final reduction: one workgroup per (job, omega block).  Each sums its
-// block's partials in a fixed order and writes its part of the step record;
-// the last workgroup of a job (ticket in counters[2]) advances the counters.
+// Final reduction, two levels.  Workgroup (b, c) of a job sums chunk c of each
+// of the seven partial-sum arrays of omega block b (fixed chunk bounds, fixed
+// order) into part_red[b][c][7]; the last workgroup of the job (ticket in
+// counters[2]) sums the chunks in order, writes the step record of every
+// block and advances the counters.  Many workgroups keep the loads of the
+// first level in flight together; the order of every sum is fixed, so FP64
+// results are reproducible bit for bit.
 // ---------------------------------------------------------------------------
-// Sums of up to NS contiguous arrays at once: every thread walks the index
-// range of the longest array with one accumulator per array, so the loads of
-// all arrays are in flight together; clamped unconditional loads keep the
-// loop free of branches.  Fixed order for a fixed block size.
-template <int NS>
-__device__ __forceinline__ void sum_arrays(const QD_G double *const (&src)[NS], const long (&count)[NS],
-                                           double (&out)[NS], double *lds) {
-  long n = 0;
-#pragma unroll
-  for (int f = 0; f < NS; f++) {
-    out[f] = 0.0;
-    n = count[f] > n ? count[f] : n;
-  }
-#pragma unroll 8
-  for (long q = threadIdx.x; q < n; q += kBlock) {
-    double v[NS];
-#pragma unroll
-    for (int f = 0; f < NS; f++) v[f] = count[f] > 0 ? src[f][q < count[f] ? q : count[f] - 1] : 0.0;
-#pragma unroll
-    for (int f = 0; f < NS; f++) out[f] += q < count[f] ? v[f] : 0.0;
-  }
-  block_sum<NS>(out, lds);
-}
+constexpr int kRedFields = 7;   // ke, ce, re, norm, acf re, acf im, pe
 
 __global__ __launch_bounds__(kBlock) void k_reduce(StepArgs a) {
-  __shared__ double red[4 * 7];   // block_sum<7>: 4 waves x 7 values
+  __shared__ double red[4 * kRedFields];
+  __shared__ int last;
   const int job = locate_job(a.unit_base, a.n_jobs, blockIdx.x);
   const JobDev &jb = a.jobs[job];
-  const int b = blockIdx.x - a.unit_base[job];
-  const int nO = jb.nO;
-  const double area = jb.dr1 * jb.dr2;
-  const int slot = jb.counters[1] % jb.ring;
+  const int u = blockIdx.x - a.unit_base[job];
+  const int nO = jb.nO, C = kReduceChunks;
+  const int b = u / C, c = u % C;
   // kinetic energy: FFT_r units ordered by block, slab, column group
   const int gcols = jb.n1 / jb.fft_cols;
   long ke_off = 0;
@@ -110,29 +92,55 @@ __global__ __launch_bounds__(kBlock) void k_reduce(StepArgs a) {
   // potential energy partials of the backward Legendre epilogue: [b][group]
   const long ncor = jb.units[KK_FFT_R];
   const QD_G double *pc = jb.part_cor + 5L * b * ncor;
-  const QD_G double *const src[7] = {jb.part_ke + ke_off, pc, pc + ncor, pc + 2 * ncor, pc + 3 * ncor,
-                                     pc + 4 * ncor, jb.part_obs + (long)b * jb.gemm_bwd_groups};
-  const long count[7] = {(long)(jb.L - (jb.omin + b)) * gcols, ncor, ncor, ncor, ncor, ncor,
-                         (long)jb.gemm_bwd_groups};
-  double v[7];
-  sum_arrays<7>(src, count, v, red);
+  const QD_G double *const src[kRedFields] = {jb.part_ke + ke_off, pc, pc + ncor, pc + 2 * ncor, pc + 3 * ncor,
+                                              pc + 4 * ncor, jb.part_obs + (long)b * jb.gemm_bwd_groups};
+  const long count[kRedFields] = {(long)(jb.L - (jb.omin + b)) * gcols, ncor, ncor, ncor, ncor, ncor,
+                                  (long)jb.gemm_bwd_groups};
+  double v[kRedFields];
+#pragma unroll
+  for (int f = 0; f < kRedFields; f++) {
+    const long lo = count[f] * c / C, hi = count[f] * (c + 1) / C;
+    double acc = 0.0;
+    for (long q = lo + threadIdx.x; q < hi; q += kBlock) acc += src[f][q];
+    v[f] = acc;
+  }
+  block_sum<kRedFields>(v, red);
   if (threadIdx.x == 0) {
-    const double ke = v[0], ce = v[1], re = v[2], nr = v[3], ar = v[4], ai = v[5], pe = v[6];
-    QD_G double *rec = jb.obs_ring + ((long)slot * nO + b) * 5;
-    rec[0] = nr * area;
-    rec[1] = pe * area;
-    rec[2] = ke * area / (double)jb.M;
-    rec[3] = re * area;
-    rec[4] = ce * area;
-    jb.acf_ring[(long)slot * nO + b] = make_double2(ar * area, ai * area);
+    QD_G double *dst = jb.part_red + ((long)b * C + c) * kRedFields;
+#pragma unroll
+    for (int f = 0; f < kRedFields; f++) dst[f] = v[f];
     __threadfence();
-    const int t = atomicAdd(jb.counters + 2, 1);
-    if (t == nO - 1) {
-      jb.counters[2] = 0;
-      jb.counters[0] += 1;
-      jb.counters[1] += 1;
-      __threadfence();
+    last = atomicAdd(jb.counters + 2, 1) == nO * C - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (threadIdx.x < nO * kRedFields) {
+    const int bb = threadIdx.x / kRedFields, f = threadIdx.x % kRedFields;
+    const QD_G double *p = jb.part_red + (long)bb * C * kRedFields + f;
+    double sum = 0.0;
+    for (int cc = 0; cc < C; cc++) sum += p[cc * kRedFields];
+    const int slot = jb.counters[1] % jb.ring;
+    const double area = jb.dr1 * jb.dr2;
+    QD_G double *rec = jb.obs_ring + ((long)slot * nO + bb) * 5;
+    QD_G double *acf = (QD_G double *)(jb.acf_ring + (long)slot * nO + bb);
+    switch (f) {
+      case 0: rec[2] = sum * area / (double)jb.M; break;   // kinetic
+      case 1: rec[4] = sum * area; break;                  // Coriolis
+      case 2: rec[3] = sum * area; break;                  // rotational
+      case 3: rec[0] = sum * area; break;                  // norm
+      case 4: acf[0] = sum * area; break;
+      case 5: acf[1] = sum * area; break;
+      default: rec[1] = sum * area; break;                 // potential
     }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    jb.counters[2] = 0;
+    jb.counters[0] += 1;
+    jb.counters[1] += 1;
+    __threadfence();
   }
 }
 

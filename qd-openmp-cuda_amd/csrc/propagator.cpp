@@ -249,7 +249,7 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   d.units[KK_FFT_R] = (L - sh.omin) * (n2 / d.fft_rows);
   d.units[KK_FFT_r] = (int)(slabs * (n1 / d.fft_cols));
   d.units[KK_OBS] = 0;   // fused into the backward Legendre epilogue
-  d.units[KK_REDUCE] = nO;
+  d.units[KK_REDUCE] = nO * kReduceChunks;
   d.units[KK_FLUX] = 0;
 
   // ---- host tables
@@ -364,6 +364,7 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   d.part_cor = (double *)alloc(sizeof(double) * (size_t)d.units[KK_FFT_R] * nO * 5);
   d.part_ke = (double *)alloc(sizeof(double) * (size_t)d.units[KK_FFT_r]);
   d.part_obs = (double *)alloc(sizeof(double) * (size_t)nO * d.gemm_bwd_groups);
+  d.part_red = (double *)alloc(sizeof(double) * (size_t)nO * kReduceChunks * 7);
   if (sh.acf) {
     // FBR coefficients of the initial state (psi still holds it): the
     // autocorrelation <psi(0)|psi(t)> is formed on them in the FFT pass

@@ -26,6 +26,7 @@ constexpr int kMaxOmega = 24;     // omega blocks per job (K_max <= 23)
 constexpr int kFftMaxN = 512;     // longest FFT pencil
 constexpr int kMaxTheta = 512;    // theta points per job
 constexpr int kBlock = 256;       // threads per workgroup for the step kernels
+constexpr int kReduceChunks = 16; // workgroups per omega block in the final reduction
 
 // per-kernel work-unit counts (index into JobDev::units / LaunchTables)
 enum KernelKind {
@@ -87,6 +88,7 @@ struct JobDev {
   QD_G double *part_cor;          // [nO][5][units FFT_R] Coriolis, rotational energy, norm, acf re/im
   QD_G double *part_ke;           // [units FFT_r]       kinetic energy of one (block, slab, column group)
   QD_G double *part_obs;          // [nO][gemm_bwd_groups] w V |psi|^2 (potential energy)
+  QD_G double *part_red;          // [nO][kReduceChunks][7] first level of the final reduction
   QD_G double *obs_ring;          // [ring][nO][5] per-step observables
   QD_G double2 *acf_ring;         // [ring][nO] autocorrelation contribution of each block
   int ring;                  // ring capacity (records)
