@@ -328,7 +328,7 @@ __global__ __launch_bounds__(kBlock) void k_leg(StepArgs a) {
   gw.ws = w_s; gw.pot = jb.pot; gw.M = M;
   double ob = 0.0;
   const int S = jb.gemm_strips;
-  gemm_wave<NT, KC, KC == 8 ? 3 : 2, MIXED, FWD, !FWD>(gw, (long)g * S * 4 + wave, S, 4, ob);
+  gemm_wave<NT, KC, (KC == 8 && NT <= 4) ? 3 : 2, MIXED, FWD, !FWD>(gw, (long)g * S * 4 + wave, S, 4, ob);
   if (!FWD) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) ob += __shfl_xor(ob, off, 64);

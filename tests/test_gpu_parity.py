@@ -100,6 +100,24 @@ def test_parity_coriolis_untruncated_even():
     check_parity(a, b)
 
 
+def test_parity_nine_blocks():
+    # K_max = 8 (the largest block count of the served workload): the 9-block R pass
+    spec = inputs.JobSpec(n1=64, n2=32, ntheta=24, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
+                          J=10, parity=0, k_max=8, omega0=0, j0=2, E_coll=0.02)
+    a, b = run_both(spec, 3)
+    assert len(a.omegas) == 9
+    check_parity(a, b)
+
+
+def test_parity_nineteen_blocks():
+    # 19 blocks: the widest R-pass instantiation (up to 24 blocks in registers and LDS)
+    spec = inputs.JobSpec(n1=64, n2=32, ntheta=40, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
+                          J=20, parity=0, k_max=18, omega0=0, j0=2, E_coll=0.02)
+    a, b = run_both(spec, 2)
+    assert len(a.omegas) == 19
+    check_parity(a, b)
+
+
 def test_parity_runtime_fft_lengths():
     # 80 = 2^4 * 5 and 40 are not compile-time lengths: exercises the runtime-size FFT path
     spec = inputs.JobSpec(n1=80, n2=40, ntheta=20, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
