@@ -66,7 +66,13 @@ __device__ __forceinline__ void stages_ip(double2 *A, const double2 *tw) {
 }
 
 // rows per workgroup of the R pass and columns per workgroup of the r pass
-__host__ __device__ constexpr int ct_rows(int n1) { return n1 <= 128 ? 8 : 4; }
+// Rows per workgroup of the R pass: 4 for n1 > 128; for shorter pencils 8,
+// or 2 when more than 5 omega blocks are held in registers across the
+// transform (measured: fewer rows lose more to the per-workgroup table
+// staging than they gain in occupancy once n1 > 128).
+__host__ __device__ constexpr int ct_rows(int n1, int nb) {
+  return n1 > 128 ? 4 : (nb <= 5 ? 8 : 2);
+}
 __host__ __device__ constexpr int ct_cols(int n2) { return n2 <= 64 ? 16 : 8; }
 
 // ---------------------------------------------------------------------------
@@ -76,7 +82,7 @@ __host__ __device__ constexpr int ct_cols(int n2) { return n2 <= 64 ? 16 : 8; }
 template <bool INV, int NB, int N1, bool ACF>
 __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
   constexpr int NF = ACF ? 5 : 3;
-  constexpr int P = ct_rows(N1);
+  constexpr int P = ct_rows(N1, NB);
   constexpr int PN = P * N1;
   constexpr int PPT = (PN + kBlock - 1) / kBlock;
   __shared__ double2 A[PN];
@@ -104,19 +110,20 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
     }
   }
   __syncthreads();
+  // every global load of the slab tile in flight at once; loads are
+  // unconditional from in-bounds addresses (block nb-1, last point), and
+  // values of blocks >= nb and points >= PN are never used
   double2 x[NB][PPT];
+#pragma unroll
+  for (int b = 0; b < NB; b++)
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+      const int e = threadIdx.x + k * kBlock;
+      x[b][k] = base[(b < nb ? b : nb - 1) * LM + (PN % kBlock == 0 || e < PN ? e : PN - 1)];
+    }
   FbrSums<NB, NF> sm;
   sm.zero();
   if (!INV) {
-#pragma unroll
-    for (int b = 0; b < NB; b++)
-#pragma unroll
-      for (int k = 0; k < PPT; k++) {
-        const int e = threadIdx.x + k * kBlock;
-        // unconditional load from an in-bounds address (block nb-1, last point), zeroed if unused
-        const double2 v = base[(b < nb ? b : nb - 1) * LM + (PN % kBlock == 0 || e < PN ? e : PN - 1)];
-        x[b][k] = (b < nb && (PN % kBlock == 0 || e < PN)) ? v : make_double2(0.0, 0.0);
-      }
 #pragma unroll
     for (int k = 0; k < PPT; k++) {
       const int e = threadIdx.x + k * kBlock;
@@ -149,16 +156,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
     }
   } else {
     const double s = 1.0 / (double)M;
-    // all global loads in flight at once; blocks then pass through LDS in turn
-#pragma unroll
-    for (int b = 0; b < NB; b++)
-#pragma unroll
-      for (int k = 0; k < PPT; k++) {
-        const int e = threadIdx.x + k * kBlock;
-        // unconditional load from an in-bounds address (block nb-1, last point), zeroed if unused
-        const double2 v = base[(b < nb ? b : nb - 1) * LM + (PN % kBlock == 0 || e < PN ? e : PN - 1)];
-        x[b][k] = (b < nb && (PN % kBlock == 0 || e < PN)) ? v : make_double2(0.0, 0.0);
-      }
+    // blocks pass through LDS in turn
 #pragma unroll
     for (int b = 0; b < NB; b++) {
       if (b < nb) {

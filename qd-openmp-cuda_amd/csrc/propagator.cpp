@@ -236,7 +236,7 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   static const int cols_c[] = {16, 8, 4, 2, 1};
   // compile-time FFT kernels where the lengths allow; otherwise the runtime-size
   // R pass keeps nb(l) <= nO blocks of fft_rows rows plus one scratch tile in LDS
-  if (fft_ct_supported(n1) && n2 % fft_ct_rows(n1) == 0) d.fft_rows = fft_ct_rows(n1);
+  if (fft_ct_supported(n1) && n2 % fft_ct_rows(n1, nO) == 0) d.fft_rows = fft_ct_rows(n1, nO);
   else d.fft_rows = pick_divisor(n2, rows_c, 4, 5120 / (nO + 1), n1);
   if (fft_ct_supported(n2) && n1 % fft_ct_cols(n2) == 0) d.fft_cols = fft_ct_cols(n2);
   else d.fft_cols = pick_divisor(n1, cols_c, 5, 2048, n2);
@@ -553,7 +553,7 @@ void Engine::set_jobs(const std::vector<DeviceJob *> &jobs) {
     lds_fftR_ = std::max(lds_fftR_, (size_t)((d.nO + 1) * d.fft_rows * d.n1 + d.n1) * 16);
     lds_fftr_ = std::max(lds_fftr_, (size_t)(2 * d.fft_cols * d.n2 + d.n2) * 16);
     // compile-time kernels only when every job shares the length and tiling
-    if (d.n1 != ct_n1_ || !fft_ct_supported(d.n1) || d.fft_rows != fft_ct_rows(d.n1)) ct_n1_ = 0;
+    if (d.n1 != ct_n1_ || !fft_ct_supported(d.n1)) ct_n1_ = 0;
     if (d.n2 != ct_n2_ || !fft_ct_supported(d.n2) || d.fft_cols != fft_ct_cols(d.n2)) ct_n2_ = 0;
     fwd_tiles_ = std::max(fwd_tiles_, d.fwd_np / 16);
     bwd_tiles_ = std::max(bwd_tiles_, d.bwd_np / 16);
@@ -563,6 +563,10 @@ void Engine::set_jobs(const std::vector<DeviceJob *> &jobs) {
     any_flux_ |= d.flux_on;
     any_acf_ |= d.c0 != nullptr;
   }
+  // the compile-time R pass takes fft_ct_rows(n1, nb_max) rows per workgroup:
+  // every job must have been tiled that way
+  for (int q = 0; q < n && ct_n1_; q++)
+    if (jobs[q]->dev.fft_rows != fft_ct_rows(ct_n1_, nb_max_)) ct_n1_ = 0;
   for (int k = 0; k < KK_COUNT; k++) total_[k] = bases[(size_t)k * (n + 1) + n];
   QD_HIP(hipMemcpyAsync(d_jobs_, descs.data(), sizeof(JobDev) * n, hipMemcpyHostToDevice, stream_));
   QD_HIP(hipMemcpyAsync(d_bases_, bases.data(), sizeof(int) * bases.size(), hipMemcpyHostToDevice, stream_));

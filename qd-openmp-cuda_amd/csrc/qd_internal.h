@@ -146,11 +146,12 @@ void launch_leg_bwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max
 // transform, after the inverse one; the inverse pass also forms the energies),
 // and the r pass with the kinetic phase.  _rt: any length factoring into
 // 2, 3, 5 (dynamic LDS); _ct: compile-time lengths (fft_ct_supported), with
-// fft_ct_rows(n1) rows / fft_ct_cols(n2) columns per workgroup.
+// fft_ct_rows(n1, blocks) rows / fft_ct_cols(n2) columns per workgroup (the
+// rows depend on whether the job has more than 5 omega blocks).
 void launch_fft_R_rt(const StepArgs &a, int total_units, int inverse, int acf, int nb_max, size_t lds, hipStream_t stream);
 void launch_fft_r_rt(const StepArgs &a, int total_units, size_t lds, hipStream_t stream);
 bool fft_ct_supported(int n);
-int fft_ct_rows(int n1);
+int fft_ct_rows(int n1, int n_blocks);
 int fft_ct_cols(int n2);
 void launch_fft_R_ct(const StepArgs &a, int total_units, int inverse, int acf, int n1, int nb_max, hipStream_t stream);
 void launch_fft_r_ct(const StepArgs &a, int total_units, int n2, hipStream_t stream);
