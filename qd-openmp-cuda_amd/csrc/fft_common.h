@@ -168,7 +168,7 @@ __device__ __forceinline__ void coriolis_point(const JobDev &jb, double2 (&x)[NB
 }
 
 // Sums the NF NB per-thread values over the block and stores them to
-// part_cor[(5 b + f) * cor_units + u]; red must hold 4 * NF * NB doubles.
+// part_cor[(5 b + f) * units + u]; red must hold 4 * NF * NB doubles.
 template <int NB, int NF>
 __device__ __forceinline__ void store_fbr_partials(const JobDev &jb, int u, const FbrSums<NB, NF> &sm, double *red) {
   constexpr int F = NF, NV = F * NB;
@@ -184,7 +184,7 @@ __device__ __forceinline__ void store_fbr_partials(const JobDev &jb, int u, cons
   if (threadIdx.x < F * jb.nO) {
     const int q = threadIdx.x;   // q = F b + f, stored at field f of block b ([b][5][units])
     const double sum = (q < NV) ? (red[q] + red[NV + q]) + (red[2 * NV + q] + red[3 * NV + q]) : 0.0;
-    jb.part_cor[(long)(5 * (q / F) + q % F) * jb.cor_units + u] = sum;
+    jb.part_cor[(long)(5 * (q / F) + q % F) * jb.units[KK_FFT_R] + u] = sum;
   }
 }
 

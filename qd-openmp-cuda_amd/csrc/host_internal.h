@@ -117,8 +117,7 @@ class Engine {
   int n_alloc_jobs_ = 0;
   int total_[KK_COUNT] = {};
   size_t lds_fftR_ = 0, lds_fftr_ = 0;
-  int two_pass_ = 0;            // compile-time two-pass FFT form (else runtime three-pass)
-  int ct_n1_ = 0, ct_n2_ = 0;   // lengths of the compile-time FFT kernels
+  int ct_n1_ = 0, ct_n2_ = 0;   // lengths of the compile-time FFT kernels, 0 = runtime path
   int fwd_tiles_ = 0, bwd_tiles_ = 0, fwd_kp_ = 0, bwd_kp_ = 0, nb_max_ = 1;
   int mixed_ = 0, any_flux_ = 0, any_acf_ = 0;
   hipGraph_t graph_ = nullptr;

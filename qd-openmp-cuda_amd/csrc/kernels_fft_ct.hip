@@ -8,20 +8,20 @@ QD_FFT_CT_DEFINE(32)
 QD_FFT_CT_DEFINE(48)
 QD_FFT_CT_DEFINE(64)
 
-void launch_fft_Rk_ct_32(const StepArgs &a, int units, int nb_max, hipStream_t s);
-void launch_fft_rk_ct_32(const StepArgs &a, int units, int acf, int nb_max, hipStream_t s);
-void launch_fft_Rk_ct_48(const StepArgs &a, int units, int nb_max, hipStream_t s);
-void launch_fft_rk_ct_48(const StepArgs &a, int units, int acf, int nb_max, hipStream_t s);
-void launch_fft_Rk_ct_64(const StepArgs &a, int units, int nb_max, hipStream_t s);
-void launch_fft_rk_ct_64(const StepArgs &a, int units, int acf, int nb_max, hipStream_t s);
-void launch_fft_Rk_ct_96(const StepArgs &a, int units, int nb_max, hipStream_t s);
-void launch_fft_rk_ct_96(const StepArgs &a, int units, int acf, int nb_max, hipStream_t s);
-void launch_fft_Rk_ct_128(const StepArgs &a, int units, int nb_max, hipStream_t s);
-void launch_fft_rk_ct_128(const StepArgs &a, int units, int acf, int nb_max, hipStream_t s);
-void launch_fft_Rk_ct_192(const StepArgs &a, int units, int nb_max, hipStream_t s);
-void launch_fft_rk_ct_192(const StepArgs &a, int units, int acf, int nb_max, hipStream_t s);
-void launch_fft_Rk_ct_256(const StepArgs &a, int units, int nb_max, hipStream_t s);
-void launch_fft_rk_ct_256(const StepArgs &a, int units, int acf, int nb_max, hipStream_t s);
+void launch_fft_R_ct_32(const StepArgs &a, int units, int inverse, int acf, int nb_max, hipStream_t s);
+void launch_fft_r_ct_32(const StepArgs &a, int units, hipStream_t s);
+void launch_fft_R_ct_48(const StepArgs &a, int units, int inverse, int acf, int nb_max, hipStream_t s);
+void launch_fft_r_ct_48(const StepArgs &a, int units, hipStream_t s);
+void launch_fft_R_ct_64(const StepArgs &a, int units, int inverse, int acf, int nb_max, hipStream_t s);
+void launch_fft_r_ct_64(const StepArgs &a, int units, hipStream_t s);
+void launch_fft_R_ct_96(const StepArgs &a, int units, int inverse, int acf, int nb_max, hipStream_t s);
+void launch_fft_r_ct_96(const StepArgs &a, int units, hipStream_t s);
+void launch_fft_R_ct_128(const StepArgs &a, int units, int inverse, int acf, int nb_max, hipStream_t s);
+void launch_fft_r_ct_128(const StepArgs &a, int units, hipStream_t s);
+void launch_fft_R_ct_192(const StepArgs &a, int units, int inverse, int acf, int nb_max, hipStream_t s);
+void launch_fft_r_ct_192(const StepArgs &a, int units, hipStream_t s);
+void launch_fft_R_ct_256(const StepArgs &a, int units, int inverse, int acf, int nb_max, hipStream_t s);
+void launch_fft_r_ct_256(const StepArgs &a, int units, hipStream_t s);
 
 bool fft_ct_supported(int n) {
   switch (n) {
@@ -30,20 +30,20 @@ bool fft_ct_supported(int n) {
   }
 }
 int fft_ct_rows(int n1, int n_blocks) { return ct_rows(n1, n_blocks); }
-int fft_ct_cols(int n2, int n_blocks) { return ct_cols(n2, n_blocks); }
+int fft_ct_cols(int n2) { return ct_cols(n2); }
 
-void launch_fft_Rk_ct(const StepArgs &a, int total_units, int n1, int nb_max, hipStream_t s) {
+void launch_fft_R_ct(const StepArgs &a, int total_units, int inverse, int acf, int n1, int nb_max, hipStream_t s) {
   switch (n1) {
-#define QD_R_CASE(N) case N: launch_fft_Rk_ct_##N(a, total_units, nb_max, s); break;
+#define QD_R_CASE(N) case N: launch_fft_R_ct_##N(a, total_units, inverse, acf, nb_max, s); break;
     QD_R_CASE(32) QD_R_CASE(48) QD_R_CASE(64) QD_R_CASE(96) QD_R_CASE(128) QD_R_CASE(192) QD_R_CASE(256)
 #undef QD_R_CASE
     default: break;
   }
 }
 
-void launch_fft_rk_ct(const StepArgs &a, int total_units, int acf, int n2, int nb_max, hipStream_t s) {
+void launch_fft_r_ct(const StepArgs &a, int total_units, int n2, hipStream_t s) {
   switch (n2) {
-#define QD_r_CASE(N) case N: launch_fft_rk_ct_##N(a, total_units, acf, nb_max, s); break;
+#define QD_r_CASE(N) case N: launch_fft_r_ct_##N(a, total_units, s); break;
     QD_r_CASE(32) QD_r_CASE(48) QD_r_CASE(64) QD_r_CASE(96) QD_r_CASE(128) QD_r_CASE(192) QD_r_CASE(256)
 #undef QD_r_CASE
     default: break;

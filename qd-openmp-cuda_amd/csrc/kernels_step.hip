@@ -83,26 +83,19 @@ __global__ __launch_bounds__(kBlock) void k_reduce(StepArgs a) {
   const int u = blockIdx.x - a.unit_base[job];
   const int nO = jb.nO, C = kReduceChunks;
   const int b = u / C, c = u % C;
-  // kinetic energy: two-pass form [b][ke_stride] (already normalised per
-  // axis); three-pass form: FFT_r units ordered by block, slab, column group
-  long ke_off, nke;
-  if (jb.two_pass) {
-    ke_off = (long)b * jb.ke_stride;
-    nke = jb.ke_stride;
-  } else {
-    const int gcols = jb.n1 / jb.fft_cols;
-    ke_off = 0;
-    for (int q = 0; q < b; q++) ke_off += (long)(jb.L - (jb.omin + q)) * gcols;
-    nke = (long)(jb.L - (jb.omin + b)) * gcols;
-  }
-  // second FBR half step partials: [b][5][cor_units] = Coriolis energy,
-  // rotational energy, norm, autocorrelation re / im; potential energy
-  // partials of the backward Legendre epilogue: [b][group]
-  const long ncor = jb.cor_units;
+  // kinetic energy: FFT_r units ordered by block, slab, column group
+  const int gcols = jb.n1 / jb.fft_cols;
+  long ke_off = 0;
+  for (int q = 0; q < b; q++) ke_off += (long)(jb.L - (jb.omin + q)) * gcols;
+  // second FBR half step partials of the inverse-R pass: [b][5][units] =
+  // Coriolis energy, rotational energy, norm, autocorrelation re / im;
+  // potential energy partials of the backward Legendre epilogue: [b][group]
+  const long ncor = jb.units[KK_FFT_R];
   const QD_G double *pc = jb.part_cor + 5L * b * ncor;
   const QD_G double *const src[kRedFields] = {jb.part_ke + ke_off, pc, pc + ncor, pc + 2 * ncor, pc + 3 * ncor,
                                               pc + 4 * ncor, jb.part_obs + (long)b * jb.gemm_bwd_groups};
-  const long count[kRedFields] = {nke, ncor, ncor, ncor, ncor, ncor, (long)jb.gemm_bwd_groups};
+  const long count[kRedFields] = {(long)(jb.L - (jb.omin + b)) * gcols, ncor, ncor, ncor, ncor, ncor,
+                                  (long)jb.gemm_bwd_groups};
   double v[kRedFields];
 #pragma unroll
   for (int f = 0; f < kRedFields; f++) {
@@ -132,7 +125,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce(StepArgs a) {
     QD_G double *rec = jb.obs_ring + ((long)slot * nO + bb) * 5;
     QD_G double *acf = (QD_G double *)(jb.acf_ring + (long)slot * nO + bb);
     switch (f) {
-      case 0: rec[2] = sum * area / (jb.two_pass ? 1.0 : (double)jb.M); break;   // kinetic
+      case 0: rec[2] = sum * area / (double)jb.M; break;   // kinetic
       case 1: rec[4] = sum * area; break;                  // Coriolis
       case 2: rec[3] = sum * area; break;                  // rotational
       case 3: rec[0] = sum * area; break;                  // norm

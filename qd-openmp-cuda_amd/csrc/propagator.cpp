@@ -234,35 +234,20 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   d.gemm_bwd_groups = d.gemm_fwd_groups;
   static const int rows_c[] = {8, 4, 2, 1};
   static const int cols_c[] = {16, 8, 4, 2, 1};
-  // compile-time lengths along both axes: two FFT passes per step (kinetic
-  // split, every block of a slab per workgroup in both passes); otherwise the
-  // runtime-size three-pass form, whose R pass keeps nb(l) <= nO blocks of
-  // fft_rows rows plus one scratch tile in LDS
-  d.two_pass = fft_ct_supported(n1) && fft_ct_supported(n2) && n2 % fft_ct_rows(n1, nO) == 0 &&
-               n1 % fft_ct_cols(n2, nO) == 0;
+  // compile-time FFT kernels where the lengths allow; otherwise the runtime-size
+  // R pass keeps nb(l) <= nO blocks of fft_rows rows plus one scratch tile in LDS
+  if (fft_ct_supported(n1) && n2 % fft_ct_rows(n1, nO) == 0) d.fft_rows = fft_ct_rows(n1, nO);
+  else d.fft_rows = pick_divisor(n2, rows_c, 4, 5120 / (nO + 1), n1);
+  if (fft_ct_supported(n2) && n1 % fft_ct_cols(n2) == 0) d.fft_cols = fft_ct_cols(n2);
+  else d.fft_cols = pick_divisor(n1, cols_c, 5, 2048, n2);
   long slabs = 0;
   for (int b = 0; b < nO; b++) slabs += L - (sh.omin + b);
   d.units[KK_DVR] = 0;   // fused into the forward Legendre prologue
   d.units[KK_LEG_FWD] = nO * d.gemm_fwd_groups;
   d.units[KK_LEG_BWD] = nO * d.gemm_bwd_groups;
   d.units[KK_FBR] = 0;
-  if (d.two_pass) {
-    d.fft_rows = fft_ct_rows(n1, nO);
-    d.fft_cols = fft_ct_cols(n2, nO);
-    d.units[KK_FFT_R] = (L - sh.omin) * (n2 / d.fft_rows);
-    d.units[KK_FFT_r] = (L - sh.omin) * (n1 / d.fft_cols);
-    d.cor_units = d.units[KK_FFT_r];
-    d.ke_stride = d.units[KK_FFT_R] + d.units[KK_FFT_r];
-    d.ke_off_r = d.units[KK_FFT_R];
-  } else {
-    d.fft_rows = pick_divisor(n2, rows_c, 4, 5120 / (nO + 1), n1);
-    d.fft_cols = pick_divisor(n1, cols_c, 5, 2048, n2);
-    d.units[KK_FFT_R] = (L - sh.omin) * (n2 / d.fft_rows);
-    d.units[KK_FFT_r] = (int)(slabs * (n1 / d.fft_cols));
-    d.cor_units = d.units[KK_FFT_R];
-    d.ke_stride = 0;
-    d.ke_off_r = 0;
-  }
+  d.units[KK_FFT_R] = (L - sh.omin) * (n2 / d.fft_rows);
+  d.units[KK_FFT_r] = (int)(slabs * (n1 / d.fft_cols));
   d.units[KK_OBS] = 0;   // fused into the backward Legendre epilogue
   d.units[KK_REDUCE] = nO * kReduceChunks;
   d.units[KK_FLUX] = 0;
@@ -376,8 +361,8 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   d.cmat = upload(cmat, st);
   d.twR = upload(twiddles(n1), st);
   d.twr = upload(twiddles(n2), st);
-  d.part_cor = (double *)alloc(sizeof(double) * (size_t)d.cor_units * nO * 5);
-  d.part_ke = (double *)alloc(sizeof(double) * (d.two_pass ? (size_t)nO * d.ke_stride : (size_t)d.units[KK_FFT_r]));
+  d.part_cor = (double *)alloc(sizeof(double) * (size_t)d.units[KK_FFT_R] * nO * 5);
+  d.part_ke = (double *)alloc(sizeof(double) * (size_t)d.units[KK_FFT_r]);
   d.part_obs = (double *)alloc(sizeof(double) * (size_t)nO * d.gemm_bwd_groups);
   d.part_red = (double *)alloc(sizeof(double) * (size_t)nO * kReduceChunks * 7);
   if (sh.acf) {
@@ -553,9 +538,8 @@ void Engine::set_jobs(const std::vector<DeviceJob *> &jobs) {
   std::vector<JobDev> descs(n);
   std::vector<int> bases((size_t)KK_COUNT * (n + 1), 0);
   lds_fftR_ = lds_fftr_ = 0;
-  two_pass_ = jobs[0]->dev.two_pass;
-  ct_n1_ = two_pass_ ? jobs[0]->dev.n1 : 0;
-  ct_n2_ = two_pass_ ? jobs[0]->dev.n2 : 0;
+  ct_n1_ = jobs[0]->dev.n1;
+  ct_n2_ = jobs[0]->dev.n2;
   fwd_tiles_ = bwd_tiles_ = fwd_kp_ = bwd_kp_ = 0;
   nb_max_ = 1;
   mixed_ = jobs[0]->dev.mixed;
@@ -569,8 +553,8 @@ void Engine::set_jobs(const std::vector<DeviceJob *> &jobs) {
     lds_fftR_ = std::max(lds_fftR_, (size_t)((d.nO + 1) * d.fft_rows * d.n1 + d.n1) * 16);
     lds_fftr_ = std::max(lds_fftr_, (size_t)(2 * d.fft_cols * d.n2 + d.n2) * 16);
     // compile-time kernels only when every job shares the length and tiling
-    QD_REQUIRE(d.two_pass == two_pass_ && (!two_pass_ || (d.n1 == ct_n1_ && d.n2 == ct_n2_)),
-               "jobs of one engine must share the FFT form and, with compile-time lengths, the grid lengths");
+    if (d.n1 != ct_n1_ || !fft_ct_supported(d.n1)) ct_n1_ = 0;
+    if (d.n2 != ct_n2_ || !fft_ct_supported(d.n2) || d.fft_cols != fft_ct_cols(d.n2)) ct_n2_ = 0;
     fwd_tiles_ = std::max(fwd_tiles_, d.fwd_np / 16);
     bwd_tiles_ = std::max(bwd_tiles_, d.bwd_np / 16);
     fwd_kp_ = std::max(fwd_kp_, d.fwd_kp);
@@ -579,12 +563,10 @@ void Engine::set_jobs(const std::vector<DeviceJob *> &jobs) {
     any_flux_ |= d.flux_on;
     any_acf_ |= d.c0 != nullptr;
   }
-  // the compile-time passes take fft_ct_rows / fft_ct_cols(., nb_max) rows and
-  // columns per workgroup: every job must have been tiled that way
-  for (int q = 0; q < n && two_pass_; q++)
-    QD_REQUIRE(jobs[q]->dev.fft_rows == fft_ct_rows(ct_n1_, nb_max_) &&
-                   jobs[q]->dev.fft_cols == fft_ct_cols(ct_n2_, nb_max_),
-               "jobs of one engine must share the FFT tiling (block-count class)");
+  // the compile-time R pass takes fft_ct_rows(n1, nb_max) rows per workgroup:
+  // every job must have been tiled that way
+  for (int q = 0; q < n && ct_n1_; q++)
+    if (jobs[q]->dev.fft_rows != fft_ct_rows(ct_n1_, nb_max_)) ct_n1_ = 0;
   for (int k = 0; k < KK_COUNT; k++) total_[k] = bases[(size_t)k * (n + 1) + n];
   QD_HIP(hipMemcpyAsync(d_jobs_, descs.data(), sizeof(JobDev) * n, hipMemcpyHostToDevice, stream_));
   QD_HIP(hipMemcpyAsync(d_bases_, bases.data(), sizeof(int) * bases.size(), hipMemcpyHostToDevice, stream_));
@@ -612,24 +594,18 @@ void Engine::enqueue(hipEvent_t *ev) {
   begin(QD_K_LEG_FWD);
   launch_leg_fwd(args(KK_LEG_FWD), total_[KK_LEG_FWD], fwd_tiles_, fwd_kp_, mixed_, s);
   end(QD_K_LEG_FWD);
-  if (two_pass_) {
-    begin(QD_K_FFT_R_FWD);
-    launch_fft_Rk_ct(args(KK_FFT_R), total_[KK_FFT_R], ct_n1_, nb_max_, s);
-    end(QD_K_FFT_R_FWD);
-    begin(QD_K_FFT_r);
-    launch_fft_rk_ct(args(KK_FFT_r), total_[KK_FFT_r], any_acf_, ct_n2_, nb_max_, s);
-    end(QD_K_FFT_r);
-  } else {
-    begin(QD_K_FFT_R_FWD);
-    launch_fft_R_rt(args(KK_FFT_R), total_[KK_FFT_R], 0, 0, nb_max_, lds_fftR_, s);
-    end(QD_K_FFT_R_FWD);
-    begin(QD_K_FFT_r);
-    launch_fft_r_rt(args(KK_FFT_r), total_[KK_FFT_r], lds_fftr_, s);
-    end(QD_K_FFT_r);
-    begin(QD_K_FFT_R_INV);
-    launch_fft_R_rt(args(KK_FFT_R), total_[KK_FFT_R], 1, any_acf_, nb_max_, lds_fftR_, s);
-    end(QD_K_FFT_R_INV);
-  }
+  begin(QD_K_FFT_R_FWD);
+  if (ct_n1_) launch_fft_R_ct(args(KK_FFT_R), total_[KK_FFT_R], 0, 0, ct_n1_, nb_max_, s);
+  else launch_fft_R_rt(args(KK_FFT_R), total_[KK_FFT_R], 0, 0, nb_max_, lds_fftR_, s);
+  end(QD_K_FFT_R_FWD);
+  begin(QD_K_FFT_r);
+  if (ct_n2_) launch_fft_r_ct(args(KK_FFT_r), total_[KK_FFT_r], ct_n2_, s);
+  else launch_fft_r_rt(args(KK_FFT_r), total_[KK_FFT_r], lds_fftr_, s);
+  end(QD_K_FFT_r);
+  begin(QD_K_FFT_R_INV);
+  if (ct_n1_) launch_fft_R_ct(args(KK_FFT_R), total_[KK_FFT_R], 1, any_acf_, ct_n1_, nb_max_, s);
+  else launch_fft_R_rt(args(KK_FFT_R), total_[KK_FFT_R], 1, any_acf_, nb_max_, lds_fftR_, s);
+  end(QD_K_FFT_R_INV);
   begin(QD_K_LEG_BWD);
   launch_leg_bwd(args(KK_LEG_BWD), total_[KK_LEG_BWD], bwd_tiles_, bwd_kp_, mixed_, s);
   end(QD_K_LEG_BWD);
@@ -645,9 +621,8 @@ void Engine::enqueue(hipEvent_t *ev) {
 }
 
 unsigned Engine::timed_kinds() const {
-  unsigned m = (1u << QD_K_LEG_FWD) | (1u << QD_K_FFT_R_FWD) | (1u << QD_K_FFT_r) | (1u << QD_K_LEG_BWD) |
-               (1u << QD_K_REDUCE);
-  if (!two_pass_) m |= 1u << QD_K_FFT_R_INV;
+  unsigned m = (1u << QD_K_LEG_FWD) | (1u << QD_K_FFT_R_FWD) | (1u << QD_K_FFT_r) | (1u << QD_K_FFT_R_INV) |
+               (1u << QD_K_LEG_BWD) | (1u << QD_K_REDUCE);
   if (any_flux_) m |= 1u << QD_K_OBS;
   return m;
 }

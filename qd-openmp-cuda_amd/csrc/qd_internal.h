@@ -55,11 +55,6 @@ struct JobDev {
   int bwd_kp, bwd_np;        // padded K and N, backward
   int fft_rows;              // pencils per workgroup along R
   int fft_cols;              // pencils per workgroup along r
-  int two_pass;              // 1: compile-time lengths, kinetic split into an R pass and an r pass
-                             // (fft_ct_impl.h); 0: runtime lengths, three FFT passes
-  int cor_units;             // units of the pass that forms the FBR energies (part_cor columns)
-  int ke_stride;             // two_pass: kinetic partials per block, [nO][ke_stride]
-  int ke_off_r;              // two_pass: column of the first r-pass kinetic partial
   int units[KK_COUNT];       // work units (workgroups) per kernel for this job
   // physics
   double dt, dr1, dr2;
@@ -90,9 +85,8 @@ struct JobDev {
   const QD_G double2 *twR;        // [n1] exp(-2 pi i e / n1)
   const QD_G double2 *twr;        // [n2] exp(-2 pi i e / n2)
   // reduction scratch (per-workgroup partial sums)
-  QD_G double *part_cor;          // [nO][5][cor_units] Coriolis, rotational energy, norm, acf re/im
-  QD_G double *part_ke;           // kinetic energy partials: two_pass [nO][ke_stride] (R pass units,
-                                  // then r pass units); else [units FFT_r] of (block, slab, column group)
+  QD_G double *part_cor;          // [nO][5][units FFT_R] Coriolis, rotational energy, norm, acf re/im
+  QD_G double *part_ke;           // [units FFT_r]       kinetic energy of one (block, slab, column group)
   QD_G double *part_obs;          // [nO][gemm_bwd_groups] w V |psi|^2 (potential energy)
   QD_G double *part_red;          // [nO][kReduceChunks][7] first level of the final reduction
   QD_G double *obs_ring;          // [ring][nO][5] per-step observables
@@ -148,22 +142,19 @@ void launch_prephase(double2 *psi, const double *pot, long G, int nO, double dt,
 bool leg_tiles_supported(int nt_tiles);
 void launch_leg_fwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, hipStream_t stream);
 void launch_leg_bwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, hipStream_t stream);
-// FFT passes.  _rt (runtime lengths, any factoring into 2, 3, 5; dynamic
-// LDS): R pass fused with rotational . Coriolis . rotational (before the
-// forward transform, after the inverse one; the inverse pass also forms the
-// energies), and the r pass with the kinetic phase -- three passes per step.
-// _ct (compile-time lengths, fft_ct_supported): the kinetic step split into
-// an R round trip and an r round trip -- two passes per step (Rk: first half
-// step + R round trip; rk: r round trip + second half step and energies),
-// with fft_ct_rows(n1, blocks) rows / fft_ct_cols(n2, blocks) columns per
-// workgroup.
+// FFT along R fused with rotational . Coriolis . rotational (before the forward
+// transform, after the inverse one; the inverse pass also forms the energies),
+// and the r pass with the kinetic phase.  _rt: any length factoring into
+// 2, 3, 5 (dynamic LDS); _ct: compile-time lengths (fft_ct_supported), with
+// fft_ct_rows(n1, blocks) rows / fft_ct_cols(n2) columns per workgroup (the
+// rows depend on whether the job has more than 5 omega blocks).
 void launch_fft_R_rt(const StepArgs &a, int total_units, int inverse, int acf, int nb_max, size_t lds, hipStream_t stream);
 void launch_fft_r_rt(const StepArgs &a, int total_units, size_t lds, hipStream_t stream);
 bool fft_ct_supported(int n);
 int fft_ct_rows(int n1, int n_blocks);
-int fft_ct_cols(int n2, int n_blocks);
-void launch_fft_Rk_ct(const StepArgs &a, int total_units, int n1, int nb_max, hipStream_t stream);
-void launch_fft_rk_ct(const StepArgs &a, int total_units, int acf, int n2, int nb_max, hipStream_t stream);
+int fft_ct_cols(int n2);
+void launch_fft_R_ct(const StepArgs &a, int total_units, int inverse, int acf, int n1, int nb_max, hipStream_t stream);
+void launch_fft_r_ct(const StepArgs &a, int total_units, int n2, hipStream_t stream);
 // standalone batched 1-D FFT of `count` contiguous pencils (test probe)
 void launch_fft_probe(double2 *data, const double2 *tw, int n, int count, int inverse, hipStream_t stream);
 void launch_reduce(const StepArgs &a, int total_units, hipStream_t stream);
