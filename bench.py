@@ -99,21 +99,35 @@ def read_traffic(kernel):
 
 
 def cpu_baseline(job, abi, n_steps=2):
-    """FP64 oracle on the host cores over a bounded sample of the same workload."""
+    """FP64 oracle on the host cores over a bounded sample of the same workload
+    (all threads, and one step on one thread)."""
     if ROOT not in sys.path:
         sys.path.insert(0, ROOT)
     from oracle import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    pts = job["r1"]["n"] * job["r2"]["n"] * job["theta"]["n"] * len(job["omega_states"]["omegas"])
     inp = abi.Inputs(job, total_steps=1, want_acf=False)
     oracle.evolve(inp, threads)           # warm-up step (page-in, thread pool)
     inp = abi.Inputs(job, total_steps=n_steps, want_acf=False)
     t0 = time.perf_counter()
     oracle.evolve(inp, threads)
     dt = time.perf_counter() - t0
-    pts = job["r1"]["n"] * job["r2"]["n"] * job["theta"]["n"] * len(job["omega_states"]["omegas"])
+    inp = abi.Inputs(job, total_steps=1, want_acf=False)
+    t1 = time.perf_counter()
+    oracle.evolve(inp, 1)
+    dt1 = time.perf_counter() - t1
+    cpu_model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": pts * n_steps / dt, "unit": "grid-point-steps/s", "cores": threads, "kind": "port",
             "sample": f"{n_steps} steps of the same workload after 1 warm-up step, FP64 oracle "
-                      f"(oracle/oracle.cpp, OpenMP), {dt:.2f} s"}
+                      f"(oracle/oracle.cpp, OpenMP, {threads} threads), {dt:.2f} s; {cpu_model}",
+            "single_thread": {"value": pts / dt1, "sample": f"1 step on 1 thread, {dt1:.2f} s"}}
 
 
 def mixed_crp_error(api, abi, inputs, steps):
