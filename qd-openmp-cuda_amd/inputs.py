@@ -399,3 +399,55 @@ def serve_jobs(n_jobs: int = 64, seed: int = 5, rate_per_s: float = 4.0,
                        v0=v0, j0=j0, E_coll=ec, total_steps=total_steps)
         jobs.append((t, spec))
     return jobs
+
+
+# ----------------------------------------------------------------------------
+# fixture files for the command-line driver (driver/qd_fixture_run.cpp)
+# ----------------------------------------------------------------------------
+
+def write_fixture(job: dict, path: str, total_steps: Optional[int] = None) -> None:
+    """Write one job in the binary fixture layout read by qd_fixture_run."""
+    r1, r2, th, os_, crp = job["r1"], job["r2"], job["theta"], job["omega_states"], job["crp"]
+    P = np.asfortranarray(os_["associated_legendres"], dtype=np.float64)
+    omegas = np.asarray(os_["omegas"], dtype=np.int32)
+    psi = np.asfortranarray(os_["wave_packets"], dtype=np.complex128)
+    has_dump = job["dump1"] is not None and job["dump2"] is not None
+    ts = job["time"]["total_steps"] if total_steps is None else total_steps
+    header = np.array([r1["n"], r2["n"], th["n"], th["m"], len(omegas), os_["J"], os_["parity"], os_["l_max"],
+                       P.shape[1], ts, job["time"]["steps"], job.get("k_max", -1), len(crp["energies"]),
+                       crp["n_dividing_surface"], crp["n_gradient_points"], crp["calculate_CRP"],
+                       int(has_dump)], dtype=np.int32)
+    geom = np.array([r1["left"], r1["dr"], r1["mass"], r2["left"], r2["dr"], r2["mass"],
+                     job["time"]["time_step"]], dtype=np.float64)
+    with open(path, "wb") as f:
+        f.write(b"QDFX0001")
+        f.write(header.tobytes())
+        f.write(geom.tobytes())
+        f.write(np.ascontiguousarray(th["x"], dtype=np.float64).tobytes())
+        f.write(np.ascontiguousarray(th["w"], dtype=np.float64).tobytes())
+        f.write(np.asfortranarray(job["pot"], dtype=np.float64).tobytes(order="F"))
+        f.write(omegas.tobytes())
+        f.write(P.tobytes(order="F"))
+        f.write(psi.tobytes(order="F"))
+        if has_dump:
+            f.write(np.ascontiguousarray(job["dump1"], dtype=np.float64).tobytes())
+            f.write(np.ascontiguousarray(job["dump2"], dtype=np.float64).tobytes())
+        f.write(np.ascontiguousarray(crp["energies"], dtype=np.float64).tobytes())
+        f.write(np.ascontiguousarray(crp["eta_sq"], dtype=np.float64).tobytes())
+
+
+def read_fixture_output(path: str, job: dict, total_steps: int) -> dict:
+    """Read the outputs qd_fixture_run wrote for `job`."""
+    r1, r2, th, os_ = job["r1"], job["r2"], job["theta"], job["omega_states"]
+    nO = len(os_["omegas"])
+    n_e = len(job["crp"]["energies"])
+    shape = (r1["n"], r2["n"], th["n"], nO)
+    with open(path, "rb") as f:
+        if f.read(8) != b"QDOUT001":
+            raise ValueError(f"{path} is not a driver output file")
+        steps = int(np.frombuffer(f.read(4), dtype=np.int32)[0])
+        obs = np.frombuffer(f.read(8 * total_steps * nO * 5), dtype=np.float64).reshape(total_steps, nO, 5)
+        n_psi = int(np.prod(shape))
+        psi = np.frombuffer(f.read(16 * n_psi), dtype=np.complex128).reshape(shape, order="F")
+        crp = np.frombuffer(f.read(8 * n_e), dtype=np.float64)
+    return {"steps": steps, "obs": obs, "psi": psi, "crp": crp}

@@ -127,3 +127,35 @@ def test_host_eigensolver_matches_oracle(J, l, om, n):
     assert np.allclose(np.sort(e), np.sort(eo), atol=1e-11 * scale)
     assert np.allclose(v @ np.diag(e) @ v.T, c, atol=1e-11 * scale)
     assert np.allclose(v.T @ v, np.eye(n), atol=1e-13)
+
+
+DRIVER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "qd-openmp-cuda_amd",
+                      "qd_fixture_run")
+
+
+def test_driver_usage_and_bad_fixture(tmp_path):
+    """The command-line driver links the library and rejects what is not a
+    fixture before touching the device."""
+    import subprocess
+    r = subprocess.run([DRIVER], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
+    bad = tmp_path / "bad.bin"
+    bad.write_bytes(b"not a fixture at all")
+    r = subprocess.run([DRIVER, str(bad), str(tmp_path / "out.bin")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "not a fixture" in r.stderr
+
+
+def test_fixture_header_roundtrip(tmp_path):
+    spec = inputs.JobSpec(n1=16, n2=16, ntheta=6, J=2, parity=0, k_max=-1, j0=1, n_energies=3,
+                          r_range=(0.5, 5.0), flux_r=2.0, n_gradient_points=5)
+    job = inputs.build_job(spec)
+    p = tmp_path / "f.bin"
+    inputs.write_fixture(job, str(p), total_steps=5)
+    raw = p.read_bytes()
+    assert raw[:8] == b"QDFX0001"
+    h = np.frombuffer(raw[8:8 + 17 * 4], dtype=np.int32)
+    assert list(h[:5]) == [16, 16, 6, 4, 3] and h[9] == 5 and h[12] == 3 and h[16] == 1
+    n1, n2, nt, nO, lnmax = 16, 16, 6, 3, int(h[8])
+    expect = 8 + 17 * 4 + 7 * 8 + 8 * (2 * nt + n1 * n2 * nt) + 4 * nO + 8 * nt * lnmax * nO \
+        + 16 * n1 * n2 * nt * nO + 8 * (n1 + n2) + 8 * 2 * 3
+    assert len(raw) == expect

@@ -181,3 +181,70 @@ def test_mixed_precision_close_to_fp64():
     _say(f"  mixed: psi rel {e_psi:.3e} norm {e_norm:.3e}")
     assert e_psi < 1e-5
     assert e_norm < 1e-6
+
+
+def test_copy_back_callback_matches_oracle():
+    """Copy-back every 2 steps with a host callback (the reference's
+    wave_to_matlab path, cudaOpenMP2.cu:380-386): the same step numbers and the
+    same wavepackets as the oracle, taken before the step counter advances."""
+    spec = inputs.JobSpec(n1=64, n2=32, ntheta=24, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
+                          J=3, parity=1, k_max=-1, omega0=0, j0=1, E_coll=0.02)
+    job = inputs.build_job(spec)
+    _say("case copy-back 64x32x24 J=3 every 2 steps, 7 steps")
+    snaps = {"oracle": [], "gpu": []}
+
+    def make(name):
+        holder = {}
+
+        def cb(user, steps):
+            snaps[name].append((int(steps), holder["inp"].psi.copy()))
+        inp = abi.Inputs(job, total_steps=7, copy_every=2, final_copy=False, callback=cb)
+        holder["inp"] = inp
+        return inp
+    a, b = make("oracle"), make("gpu")
+    oracle.evolve(a, n_threads=8)
+    api.evolve(b)
+    steps_a = [s for s, _ in snaps["oracle"]]
+    steps_b = [s for s, _ in snaps["gpu"]]
+    _say(f"  callback steps oracle {steps_a} gpu {steps_b}")
+    assert steps_a == steps_b == [0, 2, 4, 6]
+    for (_, pa), (_, pb) in zip(snaps["oracle"], snaps["gpu"]):
+        assert rel_l2(pb, pa) <= PSI_TOL
+    assert b.steps[0] == a.steps[0] == 7
+
+
+def test_parity_without_absorber():
+    # dump1/dump2 absent: the absorber is off (matlabData.h:27)
+    spec = inputs.JobSpec(n1=64, n2=32, ntheta=24, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
+                          J=4, parity=1, k_max=3, omega0=1, j0=1, E_coll=0.02, absorber=False)
+    a, b = run_both(spec, 5)
+    assert a.d1 is None and b.d2 is None
+    check_parity(a, b)
+
+
+def test_command_line_driver_matches_library(tmp_path):
+    """driver/qd_fixture_run (C++ over the C ABI, the counterpart of the
+    reference's Matlab driver + MEX entry) gives the library's results bit for
+    bit on the same fixture."""
+    import os
+    import subprocess
+    spec = inputs.JobSpec(n1=64, n2=32, ntheta=24, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
+                          J=5, parity=1, k_max=3, omega0=1, j0=2, E_coll=0.02, flux_r=2.8,
+                          n_energies=5, energy_range=(0.012, 0.03))
+    job = inputs.build_job(spec)
+    steps = 70
+    fx, out = tmp_path / "job.bin", tmp_path / "out.bin"
+    inputs.write_fixture(job, str(fx), total_steps=steps)
+    drv = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "qd-openmp-cuda_amd",
+                       "qd_fixture_run")
+    _say("case driver 64x32x24 J=5 70 steps, 5 energies (child process)")
+    r = subprocess.run([drv, str(fx), str(out)], capture_output=True, text=True, timeout=120)
+    _say("  " + r.stdout.strip() + r.stderr.strip())
+    assert r.returncode == 0
+    got = inputs.read_fixture_output(str(out), job, steps)
+    ref = abi.Inputs(job, total_steps=steps, want_acf=False)
+    api.evolve(ref)
+    assert got["steps"] == int(ref.steps[0]) == steps
+    assert np.array_equal(got["psi"], ref.psi)
+    assert np.array_equal(got["obs"], ref.obs)
+    assert np.array_equal(got["crp"], ref.crp)
