@@ -242,13 +242,10 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   else d.fft_cols = pick_divisor(n1, cols_c, 5, 2048, n2);
   long slabs = 0;
   for (int b = 0; b < nO; b++) slabs += L - (sh.omin + b);
-  d.units[KK_DVR] = 0;   // fused into the forward Legendre prologue
   d.units[KK_LEG_FWD] = nO * d.gemm_fwd_groups;
   d.units[KK_LEG_BWD] = nO * d.gemm_bwd_groups;
-  d.units[KK_FBR] = 0;
   d.units[KK_FFT_R] = (L - sh.omin) * (n2 / d.fft_rows);
   d.units[KK_FFT_r] = (int)(slabs * (n1 / d.fft_cols));
-  d.units[KK_OBS] = 0;   // fused into the backward Legendre epilogue
   d.units[KK_REDUCE] = nO * kReduceChunks;
   d.units[KK_FLUX] = 0;
 
@@ -335,7 +332,6 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   QD_HIP(hipMemcpyAsync(d.psi, in->omega_states.wave_packets, 16 * G * nO, hipMemcpyHostToDevice, st));
   d.fbr = (double2 *)alloc(16 * (size_t)M * L * nO);
   QD_HIP(hipMemsetAsync(d.fbr, 0, 16 * (size_t)M * L * nO, st));
-  d.psi0 = nullptr;
   d.c0 = nullptr;
   d.vphase = upload(vph, st);
   d.pot = upload(pot, st);

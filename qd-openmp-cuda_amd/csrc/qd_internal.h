@@ -28,15 +28,14 @@ constexpr int kMaxTheta = 512;    // theta points per job
 constexpr int kBlock = 256;       // threads per workgroup for the step kernels
 constexpr int kReduceChunks = 16; // workgroups per omega block in the final reduction
 
-// per-kernel work-unit counts (index into JobDev::units / LaunchTables)
+// kernels of a step that take work units (index into JobDev::units and the
+// engine's prefix tables); the absorber / potential phase, the rotational-
+// Coriolis half steps and the DVR observables are fused into these
 enum KernelKind {
-  KK_DVR = 0,
-  KK_LEG_FWD,
-  KK_FBR,
+  KK_LEG_FWD = 0,
   KK_FFT_R,
   KK_FFT_r,
   KK_LEG_BWD,
-  KK_OBS,
   KK_REDUCE,
   KK_FLUX,
   KK_COUNT
@@ -61,8 +60,6 @@ struct JobDev {
   // state
   QD_G double2 *psi;              // [nO][nt][M] DVR wavepackets
   QD_G double2 *fbr;              // [nO][L][M]  FBR wavepackets (slabs l < omega unused)
-  QD_G double2 *psi0;             // unused on the device (kept for layout); the initial state
-                                  // enters the autocorrelation through c0
   QD_G double2 *c0;               // [nO][L][M] FBR coefficients of the initial state, or null
   // tables
   const QD_G double2 *vphase;     // [nt][M]  d1[i] d2[j] exp(-i dt V)
