@@ -107,9 +107,9 @@ struct FbrSums {
 // backward Legendre transform: the Gauss-Legendre pair is an isometry between
 // the FBR coefficients and the band-limited DVR values.
 // phi points at the phases corPhi(l, i, a), a = 0.. (global table or an LDS copy).
-template <bool ENERGY, int NB, int NF, class PhiPtr>
+template <bool ENERGY, int NB, int NF, class PhiPtr, class VPtr>
 __device__ __forceinline__ void coriolis_point(const JobDev &jb, double2 (&x)[NB], int nb, int l, int i, int j,
-                                               const QD_G double *__restrict__ V, FbrSums<NB, NF> &sm,
+                                               VPtr V, FbrSums<NB, NF> &sm,
                                                const QD_G double2 *c0, long LM, PhiPtr phi) {
   const int nO = jb.nO;
   if (ENERGY) {
@@ -165,6 +165,13 @@ __device__ __forceinline__ void coriolis_point(const JobDev &jb, double2 (&x)[NB
       }
     }
   }
+}
+
+// Sum of x over the 64 lanes of the wave, written by lane 0 to *slot.
+__device__ __forceinline__ void wave_sum_to(double x, double *slot) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  if ((threadIdx.x & 63) == 0) *slot = x;
 }
 
 // Sums the NF NB per-thread values over the block and stores them to

@@ -97,7 +97,8 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
   const int groups = jb.n2 / P;
   const int l = jb.omin + u / groups, g = u % groups;
   const int nb = min(jb.kmax, l) - jb.omin + 1;
-  const QD_G double *V = jb.corV + (long)l * jb.nO * jb.nO;
+  // uniform over the workgroup and read-only: scalar loads into SGPRs
+  const QD_C double *V = (const QD_C double *)(jb.corV + (long)l * jb.nO * jb.nO);
   QD_G double2 *base = jb.fbr + (long)l * M + (long)g * PN;
   const QD_G double2 *c0 = ACF ? jb.c0 + (long)l * M + (long)g * PN : nullptr;
   for (int e = threadIdx.x; e < N1; e += kBlock) tw[e] = jb.twR[e];
@@ -156,6 +157,31 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
     }
   } else {
     const double s = 1.0 / (double)M;
+    const int wave = threadIdx.x >> 6;
+    constexpr int NV = NF * NB;
+    // Blocks absent at this slab (b >= nb) hold x = 0, and the tables are
+    // zero there (corV rows and columns, cmat entries and the last off-
+    // diagonal, corPhi past nO), so the sums over blocks below run over all
+    // NB with clamped, in-range table indices and no per-block conditions.
+    const int nOc = jb.nO - 1;
+    const QD_C double2 *cm = (const QD_C double2 *)(jb.cmat + (long)l * jb.nO);
+    // Coriolis energy of block b - 1 once block b is out of the transform:
+    //   sum I1 (C(b,b) |x_b|^2 + 2 C(b,b+1) Re(conj(x_b) x_{b+1})),
+    // wave-reduced at once so one sum is live at a time
+    auto coriolis_energy = [&](int b) {
+      const double2 c = cm[min(b, nOc)];
+      double ce = 0.0;
+#pragma unroll
+      for (int k = 0; k < PPT; k++) {
+        const int e = threadIdx.x + k * kBlock;
+        if (PN % kBlock == 0 || e < PN) {
+          double en = c.x * cnorm(x[b][k]);
+          if (b + 1 < NB) en += 2.0 * c.y * cdot_re(x[b][k], x[b + 1 < NB ? b + 1 : b][k]);
+          ce += en * jb.I1[e % N1];
+        }
+      }
+      wave_sum_to(ce, red + wave * NV + b * NF);
+    };
     // blocks pass through LDS in turn
 #pragma unroll
     for (int b = 0; b < NB; b++) {
@@ -176,22 +202,74 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
 #pragma unroll
         for (int k = 0; k < PPT; k++) x[b][k] = make_double2(0.0, 0.0);
       }
+      if (b > 0) coriolis_energy(b - 1);
     }
+    coriolis_energy(NB - 1);
+    // first stage, per point: t_a = phi(l, i, a) sum_b V_l[a][b] x_b, in place
 #pragma unroll
     for (int k = 0; k < PPT; k++) {
       const int e = threadIdx.x + k * kBlock;
       if (PN % kBlock == 0 || e < PN) {
-        double2 xp[NB];
+        const double2 *ph = (const double2 *)phs + (e % N1) * NB;
+        double2 t[NB];
 #pragma unroll
-        for (int b = 0; b < NB; b++) xp[b] = x[b][k];
-        coriolis_point<true, NB, NF>(jb, xp, nb, l, e % N1, g * P + e / N1, V, sm, ACF ? c0 + e : nullptr, LM,
-                                     (const double2 *)phs + (e % N1) * NB);
+        for (int al = 0; al < NB; al++) {
+          double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
-        for (int b = 0; b < NB; b++)
-          if (b < nb) base[b * LM + e] = xp[b];
+          for (int b = 0; b < NB; b++) {
+            const double v = V[min(al, nOc) * jb.nO + min(b, nOc)];
+            acc.x = fma(v, x[b][k].x, acc.x);
+            acc.y = fma(v, x[b][k].y, acc.y);
+          }
+          t[al] = cmul(ph[al], acc);
+        }
+#pragma unroll
+        for (int al = 0; al < NB; al++) x[al][k] = t[al];
       }
     }
-    store_fbr_partials<NB, NF>(jb, u, sm, red);
+    // second stage, block by block: y_b = rot2(l, j) sum_a V_l[a][b] t_a,
+    // stored; rotational energy l(l+1)(I1 + I2)|y|^2, |y|^2 and conj(c0) y
+    const double ll = (double)l * (l + 1);
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      double sb[NF - 1];
+#pragma unroll
+      for (int f = 0; f < NF - 1; f++) sb[f] = 0.0;
+      if (b < nb) {
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+          const int e = threadIdx.x + k * kBlock;
+          if (PN % kBlock == 0 || e < PN) {
+            const int i = e % N1, j = g * P + e / N1;
+            double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int al = 0; al < NB; al++) {
+              const double v = V[min(al, nOc) * jb.nO + b];
+              acc.x = fma(v, x[al][k].x, acc.x);
+              acc.y = fma(v, x[al][k].y, acc.y);
+            }
+            const double2 y = cmul(jb.rot2[(long)l * jb.n2 + j], acc);
+            base[b * LM + e] = y;
+            const double q = cnorm(y);
+            sb[0] += ll * (jb.I1[i] + jb.I2[j]) * q;
+            sb[1] += q;
+            if constexpr (NF == 5) {
+              const double2 z = c0[b * LM + e];
+              sb[2] += z.x * y.x + z.y * y.y;
+              sb[3] += z.x * y.y - z.y * y.x;
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int f = 0; f < NF - 1; f++) wave_sum_to(sb[f], red + wave * NV + b * NF + 1 + f);
+    }
+    __syncthreads();
+    if (threadIdx.x < NF * jb.nO) {
+      const int q = threadIdx.x;   // q = NF b + f, stored at field f of block b ([b][5][units])
+      const double sum = (q < NV) ? (red[q] + red[NV + q]) + (red[2 * NV + q] + red[3 * NV + q]) : 0.0;
+      jb.part_cor[(long)(5 * (q / NF) + q % NF) * jb.units[KK_FFT_R] + u] = sum;
+    }
   }
 }
 

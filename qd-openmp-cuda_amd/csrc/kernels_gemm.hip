@@ -20,7 +20,7 @@
 // This is the absorber and potential phase of cudaOpenMP2.cu:342-346
 // (evolutionCUDAaux.cu:5-11, :243-252), which the reference runs as separate
 // passes over psi.
-// Backward epilogue: the per-workgroup partial sums of w_k V |psi|^2, the
+// Backward epilogue: the per-workgroup partial sums of (w_k V) |psi|^2, the
 // potential energy of omegawavepacket.cu:101-132, from the accumulators (the
 // norm and autocorrelation come from the FBR coefficients in the FFT pass).
 //
@@ -78,8 +78,7 @@ struct GemmWave {
   const float *Bh, *Bl;          // FP32 hi/lo B in LDS (mixed form)
   int Ws;
   const QD_G double *vph;        // forward: phase table viewed as real (2M x nt), same indexing as A
-  const double *ws;              // backward epilogue: quadrature weights in LDS
-  const QD_G double *pot;        // backward epilogue: potential [nt][M]
+  const QD_G double *wpot;       // backward epilogue: w_k V(k, m) [NT * 16][M], zero rows k >= N
   long M;
 };
 
@@ -162,10 +161,17 @@ __device__ __forceinline__ void gemm_wave(const GemmWave &gw, long strip0, int n
         if (PHASE) tp[q] = buffer_load_f64(rT, v0 + q * kstride);   // even lane Re T, odd lane Im T
       }
     } else {
+      // a uniform row base per k (an SGPR pair) plus one 32-bit lane offset:
+      // the loads take the scalar-base form and share one address VGPR.
+      // k-steps past the last (when kCh does not divide ksteps) reread the
+      // last one; rows K <= k < 4 ksteps are allocated (zero padding past
+      // the last block) and masked where they are used.
+      const uint32_t vb = ((uint32_t)lk * (uint32_t)gw.lda + (uint32_t)rc) * 8u;
 #pragma unroll
       for (int q = 0; q < kCh; q++) {
-        const int k = min(4 * (c * kCh + q) + lk, gw.K - 1);
-        a[q] = gw.A[(long)k * gw.lda + rc];
+        const int kq = min(c * kCh + q, gw.ksteps - 1);
+        const char *Aq = (const char *)(gw.A + (long)(4 * kq) * gw.lda);
+        a[q] = *(const double *)(Aq + vb);
       }
     }
   };
@@ -242,21 +248,25 @@ __device__ __forceinline__ void gemm_wave(const GemmWave &gw, long strip0, int n
           const bool ok = rok && col < gw.N;
           const double v = acc[t][it];
           if (ok) gw.C[(long)col * gw.ldc + r] = v;
-          if (OBS) ob += (ok ? gw.ws[min(col, gw.N - 1)] * v * v : 0.0) * vq[t][it];
+          if (OBS) ob += (rok ? v * v : 0.0) * vq[t][it];   // vq = 0 for col >= N
         }
       }
     }
   };
   // potentials of the strip of `tile` (backward epilogue), issued before the prefetch
+  // weighted potentials w_k V of the strip's outputs (the table holds NT * 16
+  // theta rows, zero past N: no clamp)
   auto load_pot = [&](int tile, double(&vq)[NT][4]) {
     const long r = row_of(tile);
-    const long m = (r < gw.rows ? r : gw.rows - 1) >> 1;
+    const uint32_t m = (uint32_t)((r < gw.rows ? r : gw.rows - 1) >> 1);
+    const uint32_t vb = ((uint32_t)acc_col<MIXED>(0, lk, 0) * (uint32_t)gw.M + m) * 8u;
 #pragma unroll
     for (int t = 0; t < NT; t++)
 #pragma unroll
       for (int it = 0; it < 4; it++) {
-        const int cc = min(acc_col<MIXED>(t, lk, it), gw.N - 1);
-        vq[t][it] = gw.pot[(long)cc * gw.M + m];
+        const int c0 = acc_col<MIXED>(t, 0, it);   // uniform part of the column
+        const QD_G char *pr = (const QD_G char *)(gw.wpot + (long)c0 * gw.M);
+        vq[t][it] = *(const QD_G double *)(pr + vb);
       }
   };
 #pragma unroll
@@ -277,14 +287,11 @@ template <int NT, int KC, bool FWD, bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_leg(StepArgs a) {
   extern __shared__ double lds_d[];
   __shared__ double red[4];
-  __shared__ double w_s[FWD ? 1 : kMaxTheta];
   const int lu = xcd_logical_unit(blockIdx.x, gridDim.x);
   const int job = locate_job(a.unit_base, a.n_jobs, lu);
   const JobDev &jb = a.jobs[job];
   const int u = lu - a.unit_base[job];
   const int groups = FWD ? jb.gemm_fwd_groups : jb.gemm_bwd_groups;
-  if (!FWD)
-    for (int e = threadIdx.x; e < jb.nt; e += kBlock) w_s[e] = jb.w[e];
   // consecutive workgroups take the same rows of consecutive blocks, so the
   // per-row tables (phase, potential) are reused from L2
   const int b = u % jb.nO, g = u / jb.nO;
@@ -296,7 +303,9 @@ __global__ __launch_bounds__(kBlock) void k_leg(StepArgs a) {
   const int np = FWD ? jb.fwd_np : jb.bwd_np;
   constexpr int Ws = lds_stride(NT);
   constexpr int W = NT * 16;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the wave index is uniform over the wave: readfirstlane tells the
+  // compiler, so strip and tile indices (and the operand row bases) are scalar
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   double *Bs = lds_d;
   float *Bh = (float *)lds_d, *Bl = Bh + kp * Ws;
   if (!MIXED) {
@@ -325,10 +334,13 @@ __global__ __launch_bounds__(kBlock) void k_leg(StepArgs a) {
   gw.ksteps = kp / 4;
   gw.Bs = Bs; gw.Bh = Bh; gw.Bl = Bl; gw.Ws = Ws;
   gw.vph = (const QD_G double *)jb.vphase;
-  gw.ws = w_s; gw.pot = jb.pot; gw.M = M;
+  gw.wpot = jb.wpot; gw.M = M;
   double ob = 0.0;
   const int S = jb.gemm_strips;
   gemm_wave<NT, KC, (KC == 8 && NT <= 4) ? 3 : 2, MIXED, FWD, !FWD>(gw, (long)g * S * 4 + wave, S, 4, ob);
+  // first kernel of the step: apply the previous step's counter advance (at
+  // the end, so the store cannot hold back the descriptor loads above)
+  if (FWD && u == 0 && threadIdx.x == 0) advance_counters(jb);
   if (!FWD) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) ob += __shfl_xor(ob, off, 64);

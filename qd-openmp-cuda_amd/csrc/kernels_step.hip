@@ -3,13 +3,13 @@
 //   k_prephase  the V(-dt/2) start phase of cudaOpenMP2.cu:344 (steps == 0),
 //               applied once before the first step
 //   k_reduce  fixed-order reduction of the per-workgroup partial sums into one
-//             observable record per step; advances the device step counters
-//             (replaces the host-synchronous cublasZdotc calls and
+//             observable record per step; flags the device step counters for
+//             advance (replaces the host-synchronous cublasZdotc calls and
 //              print_energies, cudaOpenMP2.cu:425-445)
 //
-// All reductions are two-level and in a fixed order, so an FP64 run is
-// bitwise reproducible and a job's results do not depend on which other jobs
-// share the launch.
+// All reductions are in a fixed order, so an FP64 run is bitwise
+// reproducible and a job's results do not depend on which other jobs share
+// the launch.
 #include <algorithm>
 
 #include "qd_internal.h"
@@ -65,65 +65,59 @@ void launch_prephase(double2 *psi, const double *pot, long G, int nO, double dt,
 }
 
 // ---------------------------------------------------------------------------
-// Final reduction, two levels.  Workgroup (b, c) of a job sums chunk c of each
-// of the seven partial-sum arrays of omega block b (fixed chunk bounds, fixed
-// order) into part_red[b][c][7]; the last workgroup of the job (ticket in
-// counters[2]) sums the chunks in order, writes the step record of every
-// block and advances the counters.  Many workgroups keep the loads of the
-// first level in flight together; the order of every sum is fixed, so FP64
-// results are reproducible bit for bit.
+// Final reduction.  Workgroup (b, f) of a job sums all partials of field f of
+// omega block b in a fixed order (thread t takes elements t, t + 256, ... in
+// turn; then the fixed block reduction) and writes that field of the step
+// record: no workgroup waits for another, so the kernel needs no
+// device-scope fence (each of which writes back an XCD's L2 on this chip).
+// Workgroup (0, 0) sets the pending-advance flag; the first workgroup of the
+// next step's first kernel advances the step counters (advance_counters), so
+// no workgroup here can read a counter that another has already advanced.
 // ---------------------------------------------------------------------------
-constexpr int kRedFields = 7;   // ke, ce, re, norm, acf re, acf im, pe
-
 __global__ __launch_bounds__(kBlock) void k_reduce(StepArgs a) {
-  __shared__ double red[4 * kRedFields];
-  __shared__ int last;
+  __shared__ double red[4];
   const int job = locate_job(a.unit_base, a.n_jobs, blockIdx.x);
   const JobDev &jb = a.jobs[job];
   const int u = blockIdx.x - a.unit_base[job];
-  const int nO = jb.nO, C = kReduceChunks;
-  const int b = u / C, c = u % C;
-  // kinetic energy: FFT_r units ordered by block, slab, column group
-  const int gcols = jb.n1 / jb.fft_cols;
-  long ke_off = 0;
-  for (int q = 0; q < b; q++) ke_off += (long)(jb.L - (jb.omin + q)) * gcols;
-  // second FBR half step partials of the inverse-R pass: [b][5][units] =
-  // Coriolis energy, rotational energy, norm, autocorrelation re / im;
-  // potential energy partials of the backward Legendre epilogue: [b][group]
-  const long ncor = jb.units[KK_FFT_R];
-  const QD_G double *pc = jb.part_cor + 5L * b * ncor;
-  const QD_G double *const src[kRedFields] = {jb.part_ke + ke_off, pc, pc + ncor, pc + 2 * ncor, pc + 3 * ncor,
-                                              pc + 4 * ncor, jb.part_obs + (long)b * jb.gemm_bwd_groups};
-  const long count[kRedFields] = {(long)(jb.L - (jb.omin + b)) * gcols, ncor, ncor, ncor, ncor, ncor,
-                                  (long)jb.gemm_bwd_groups};
-  double v[kRedFields];
-#pragma unroll
-  for (int f = 0; f < kRedFields; f++) {
-    const long lo = count[f] * c / C, hi = count[f] * (c + 1) / C;
-    double acc = 0.0;
-    for (long q = lo + threadIdx.x; q < hi; q += kBlock) acc += src[f][q];
-    v[f] = acc;
+  const int nO = jb.nO;
+  const int b = u / kRedFields, f = u % kRedFields;
+  const QD_G double *src;
+  long n;
+  if (f == 0) {   // kinetic energy: FFT_r units ordered by block, slab, column group
+    const int gcols = jb.n1 / jb.fft_cols;
+    long off = 0;
+    for (int q = 0; q < b; q++) off += (long)(jb.L - (jb.omin + q)) * gcols;
+    src = jb.part_ke + off;
+    n = (long)(jb.L - (jb.omin + b)) * gcols;
+  } else if (f < 6) {
+    // second FBR half step partials of the inverse-R pass: [b][5][units] =
+    // Coriolis energy, rotational energy, norm, autocorrelation re / im
+    n = jb.units[KK_FFT_R];
+    src = jb.part_cor + (5L * b + (f - 1)) * n;
+  } else {        // potential energy partials of the backward Legendre epilogue: [b][group]
+    n = jb.gemm_bwd_groups;
+    src = jb.part_obs + (long)b * n;
   }
-  block_sum<kRedFields>(v, red);
+  // eight loads in flight per thread, summed in element order
+  double acc = 0.0;
+  for (long q0 = threadIdx.x; q0 < n; q0 += 8L * kBlock) {
+    double v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const long q = q0 + (long)k * kBlock;
+      v[k] = q < n ? src[q] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) acc += v[k];
+  }
+  double vv[1] = {acc};
+  block_sum<1>(vv, red);
   if (threadIdx.x == 0) {
-    QD_G double *dst = jb.part_red + ((long)b * C + c) * kRedFields;
-#pragma unroll
-    for (int f = 0; f < kRedFields; f++) dst[f] = v[f];
-    __threadfence();
-    last = atomicAdd(jb.counters + 2, 1) == nO * C - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  if (threadIdx.x < nO * kRedFields) {
-    const int bb = threadIdx.x / kRedFields, f = threadIdx.x % kRedFields;
-    const QD_G double *p = jb.part_red + (long)bb * C * kRedFields + f;
-    double sum = 0.0;
-    for (int cc = 0; cc < C; cc++) sum += p[cc * kRedFields];
+    const double sum = vv[0];
     const int slot = jb.counters[1] % jb.ring;
     const double area = jb.dr1 * jb.dr2;
-    QD_G double *rec = jb.obs_ring + ((long)slot * nO + bb) * 5;
-    QD_G double *acf = (QD_G double *)(jb.acf_ring + (long)slot * nO + bb);
+    QD_G double *rec = jb.obs_ring + ((long)slot * nO + b) * 5;
+    QD_G double *acf = (QD_G double *)(jb.acf_ring + (long)slot * nO + b);
     switch (f) {
       case 0: rec[2] = sum * area / (double)jb.M; break;   // kinetic
       case 1: rec[4] = sum * area; break;                  // Coriolis
@@ -133,14 +127,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce(StepArgs a) {
       case 5: acf[1] = sum * area; break;
       default: rec[1] = sum * area; break;                 // potential
     }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    jb.counters[2] = 0;
-    jb.counters[0] += 1;
-    jb.counters[1] += 1;
-    __threadfence();
+    if (u == 0) jb.counters[2] = 1;
   }
 }
 

@@ -197,13 +197,26 @@ DeviceJob::~DeviceJob() {
   for (void *p : allocs_) (void)hipFree(p);
 }
 
+// zeroed slabs past the last block that the GEMM operand reads may touch:
+// psi rows nt..fwd_kp-1 of the last block; fbr rows l = omega_last + k for
+// k < bwd_kp of the last block, which run past slab L - 1 of that block
+static size_t gemm_pad_slabs_psi(const JobShape &s, const JobDev &d) {
+  (void)s;
+  return (size_t)(d.fwd_kp - d.nt);
+}
+static size_t gemm_pad_slabs_fbr(const JobShape &s, const JobDev &d) {
+  const int om_last = s.omin + s.nO - 1;
+  return (size_t)std::max(0, om_last + d.bwd_kp - s.L);
+}
+
 size_t DeviceJob::estimate_footprint(const JobShape &s, int n_energies) {
   const size_t G = (size_t)s.M * s.nt;
   size_t b = 0;
-  b += 16 * G * s.nO;                    // psi
+  b += 16 * (G + 4 * (size_t)s.M) * s.nO; // psi, with the GEMM padding
   b += 16 * (size_t)s.M * s.L * s.nO * (s.acf ? 1 : 0);   // FBR of the initial state
-  b += 16 * (size_t)s.M * s.L * s.nO;     // fbr
+  b += 16 * (size_t)s.M * (s.L + 4) * s.nO;   // fbr, with the GEMM padding
   b += 24 * G;                            // vphase + pot
+  b += 8 * (G + 16 * (size_t)s.M);        // weighted potential (padded to 16 theta rows)
   b += 16 * (size_t)s.L * s.n1 * s.nO * s.nO;
   b += 16 * (size_t)(n_energies + kFluxWindow) * s.nO * s.nt * s.n1 * 2;   // flux accumulators, window
   b += 4 << 20;                           // tables, partials, ring
@@ -246,7 +259,7 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   d.units[KK_LEG_BWD] = nO * d.gemm_bwd_groups;
   d.units[KK_FFT_R] = (L - sh.omin) * (n2 / d.fft_rows);
   d.units[KK_FFT_r] = (int)(slabs * (n1 / d.fft_cols));
-  d.units[KK_REDUCE] = nO * kReduceChunks;
+  d.units[KK_REDUCE] = nO * kRedFields;
   d.units[KK_FLUX] = 0;
 
   // ---- host tables
@@ -328,14 +341,28 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   }
 
   // ---- device state
-  d.psi = (double2 *)alloc(16 * G * nO);
+  // The Legendre GEMMs read operand rows k < kp (K rounded up to 4) without
+  // clamping: psi and fbr carry zeroed slabs past the last block, so those
+  // reads stay inside the allocations (rows k >= K are masked where they are
+  // used).
+  const size_t psi_pad = gemm_pad_slabs_psi(sh, d), fbr_pad = gemm_pad_slabs_fbr(sh, d);
+  d.psi = (double2 *)alloc(16 * (G * nO + psi_pad * M));
+  QD_HIP(hipMemsetAsync(d.psi, 0, 16 * (G * nO + psi_pad * M), st));
   QD_HIP(hipMemcpyAsync(d.psi, in->omega_states.wave_packets, 16 * G * nO, hipMemcpyHostToDevice, st));
-  d.fbr = (double2 *)alloc(16 * (size_t)M * L * nO);
-  QD_HIP(hipMemsetAsync(d.fbr, 0, 16 * (size_t)M * L * nO, st));
+  d.fbr = (double2 *)alloc(16 * (size_t)M * (L * nO + fbr_pad));
+  QD_HIP(hipMemsetAsync(d.fbr, 0, 16 * (size_t)M * (L * nO + fbr_pad), st));
   d.c0 = nullptr;
   d.vphase = upload(vph, st);
   d.pot = upload(pot, st);
   d.w = upload(std::vector<double>(in->theta.w, in->theta.w + nt), st);
+  {
+    // quadrature-weighted potential of the potential-energy epilogue,
+    // w_k V(k, m), padded with zero rows to the bwd_np output columns
+    std::vector<double> wpot((size_t)d.bwd_np * M, 0.0);
+    for (int k = 0; k < nt; k++)
+      for (int m = 0; m < M; m++) wpot[(size_t)k * M + m] = in->theta.w[k] * pot[(size_t)k * M + m];
+    d.wpot = upload(wpot, st);
+  }
   d.wP = upload(wP, st);
   d.Pt = upload(Pt, st);
   if (sh.mixed) {
@@ -360,7 +387,6 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   d.part_cor = (double *)alloc(sizeof(double) * (size_t)d.units[KK_FFT_R] * nO * 5);
   d.part_ke = (double *)alloc(sizeof(double) * (size_t)d.units[KK_FFT_r]);
   d.part_obs = (double *)alloc(sizeof(double) * (size_t)nO * d.gemm_bwd_groups);
-  d.part_red = (double *)alloc(sizeof(double) * (size_t)nO * kReduceChunks * 7);
   if (sh.acf) {
     // FBR coefficients of the initial state (psi still holds it): the
     // autocorrelation <psi(0)|psi(t)> is formed on them in the FFT pass

@@ -277,9 +277,11 @@ qd_status qd_probe_gemm(const double *A, int32_t rows, int32_t k, const double *
     for (int kk = 0; kk < k; kk++)
       for (int c = 0; c < n; c++) Bp[(size_t)kk * np + c] = B[(size_t)c * k + kk];   // B is (k x n) col-major
     double *dA, *dB, *dC;
-    QD_HIP(hipMalloc(&dA, sizeof(double) * rows * k));
+    // the kernel reads A rows k < kp (rows k >= K masked): allocate them
+    QD_HIP(hipMalloc(&dA, sizeof(double) * rows * kp));
     QD_HIP(hipMalloc(&dB, sizeof(double) * Bp.size()));
     QD_HIP(hipMalloc(&dC, sizeof(double) * rows * n));
+    QD_HIP(hipMemset(dA, 0, sizeof(double) * rows * kp));
     QD_HIP(hipMemcpy(dA, A, sizeof(double) * rows * k, hipMemcpyHostToDevice));
     QD_HIP(hipMemcpy(dB, Bp.data(), sizeof(double) * Bp.size(), hipMemcpyHostToDevice));
     launch_gemm_probe(dA, rows, dB, kp, np, dC, rows, rows, k, n, nullptr);

@@ -16,8 +16,10 @@
 // compile sees plain pointers with the same layout.
 #if defined(__HIP_DEVICE_COMPILE__)
 #define QD_G __attribute__((address_space(1)))
+#define QD_C __attribute__((address_space(4)))
 #else
 #define QD_G
+#define QD_C
 #endif
 
 namespace qd {
@@ -26,7 +28,7 @@ constexpr int kMaxOmega = 24;     // omega blocks per job (K_max <= 23)
 constexpr int kFftMaxN = 512;     // longest FFT pencil
 constexpr int kMaxTheta = 512;    // theta points per job
 constexpr int kBlock = 256;       // threads per workgroup for the step kernels
-constexpr int kReduceChunks = 16; // workgroups per omega block in the final reduction
+constexpr int kRedFields = 7;     // fields of the final reduction: ke, ce, re, norm, acf re, acf im, pe
 
 // kernels of a step that take work units (index into JobDev::units and the
 // engine's prefix tables); the absorber / potential phase, the rotational-
@@ -64,7 +66,8 @@ struct JobDev {
   // tables
   const QD_G double2 *vphase;     // [nt][M]  d1[i] d2[j] exp(-i dt V)
   const QD_G double *pot;         // [nt][M]
-  const QD_G double *w;           // [nt]
+  const QD_G double *w;           // [nt] Gauss-Legendre weights (flux surface integral)
+  const QD_G double *wpot;        // [bwd_np][M] w_k V(theta_k, m), zero rows k >= nt
   const QD_G double *wP;          // [nO][fwd_kp][fwd_np] row-major (c fastest): w_k P_l(x_k)
   const QD_G double *Pt;          // [nO][bwd_kp][bwd_np] row-major: P_l(x_k), k = l - omega
   const QD_G float *wP_hi, *wP_lo;// split FP32 copies for the mixed form
@@ -85,13 +88,13 @@ struct JobDev {
   QD_G double *part_cor;          // [nO][5][units FFT_R] Coriolis, rotational energy, norm, acf re/im
   QD_G double *part_ke;           // [units FFT_r]       kinetic energy of one (block, slab, column group)
   QD_G double *part_obs;          // [nO][gemm_bwd_groups] w V |psi|^2 (potential energy)
-  QD_G double *part_red;          // [nO][kReduceChunks][7] first level of the final reduction
   QD_G double *obs_ring;          // [ring][nO][5] per-step observables
   QD_G double2 *acf_ring;         // [ring][nO] autocorrelation contribution of each block
   int ring;                  // ring capacity (records)
   QD_G int *counters;             // device: [0] step counter (the reference's time.steps),
-                             //         [1] records written, [2] reduce ticket;
-                             //         advanced by the last reduce workgroup of the step
+                             //         [1] records written, [2] advance pending: set by
+                             //         the step's reduction, applied by advance_counters
+                             //         at the start of the next step's first kernel
   // flux at the dividing surface r2 = r_s (index flux_ns): per step the
   // surface kernel writes the values and the ng-point r-derivative of the
   // state with V(dt/2) restored into window slot (step % flux_T); every
@@ -130,6 +133,18 @@ __device__ __forceinline__ int locate_job(const QD_G int *base, int n_jobs, int 
     if (base[mid] <= u) lo = mid; else hi = mid - 1;
   }
   return lo;
+}
+
+// Applies the step-counter advance flagged by the previous step's reduction;
+// called by one thread of the job's first workgroup of each step's first
+// kernel (kernel boundaries order it after that reduction and before this
+// step's readers of the counters).
+__device__ __forceinline__ void advance_counters(const JobDev &jb) {
+  if (jb.counters[2]) {
+    jb.counters[0] += 1;
+    jb.counters[1] += 1;
+    jb.counters[2] = 0;
+  }
 }
 
 // Host-side launchers (each enqueues one kernel on `stream`).
