@@ -169,8 +169,7 @@ __device__ __forceinline__ void coriolis_point(const JobDev &jb, double2 (&x)[NB
 
 // Sum of x over the 64 lanes of the wave, written by lane 0 to *slot.
 __device__ __forceinline__ void wave_sum_to(double x, double *slot) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  x = wave_sum64(x);
   if ((threadIdx.x & 63) == 0) *slot = x;
 }
 
@@ -182,9 +181,7 @@ __device__ __forceinline__ void store_fbr_partials(const JobDev &jb, int u, cons
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int q = 0; q < NV; q++) {
-    double x = sm.v[q / F][q % F];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    const double x = wave_sum64(sm.v[q / F][q % F]);
     if (lane == 0) red[wave * NV + q] = x;
   }
   __syncthreads();

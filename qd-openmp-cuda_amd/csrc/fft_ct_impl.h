@@ -304,8 +304,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_r_ct(StepArgs a) {
   __syncthreads();
   stages_ip<true, N2, 1, Q, 1, Q, true>(A, tw);
   for (int e = threadIdx.x; e < QN; e += kBlock) base[(long)(e / Q) * n1 + (e % Q)] = A[e];
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) ke += __shfl_xor(ke, off, 64);
+  ke = wave_sum64(ke);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ke;
   __syncthreads();
   if (threadIdx.x == 0) jb.part_ke[u] = (red[0] + red[1]) + (red[2] + red[3]);

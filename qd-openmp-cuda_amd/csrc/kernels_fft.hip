@@ -194,8 +194,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_r(StepArgs a) {
   __syncthreads();
   double2 *Z = fft_lds<true, true>(X, Y, n2, Q, 1, Q, tw);
   for (int e = threadIdx.x; e < Q * n2; e += kBlock) base[(long)(e / Q) * n1 + (e % Q)] = Z[e];
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) ke += __shfl_xor(ke, off, 64);
+  ke = wave_sum64(ke);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ke;
   __syncthreads();
   if (threadIdx.x == 0) jb.part_ke[u] = (red[0] + red[1]) + (red[2] + red[3]);

@@ -342,8 +342,7 @@ __global__ __launch_bounds__(kBlock) void k_leg(StepArgs a) {
   // the end, so the store cannot hold back the descriptor loads above)
   if (FWD && u == 0 && threadIdx.x == 0) advance_counters(jb);
   if (!FWD) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) ob += __shfl_xor(ob, off, 64);
+    ob = wave_sum64(ob);
     if (lane == 0) red[wave] = ob;
     __syncthreads();
     if (threadIdx.x == 0) jb.part_obs[(long)b * groups + g] = (red[0] + red[1]) + (red[2] + red[3]);

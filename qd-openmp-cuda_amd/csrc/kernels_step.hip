@@ -18,13 +18,9 @@
 namespace qd {
 
 // ---------------------------------------------------------------------------
-// block reductions (wavefront shuffles + LDS across the 4 waves)
+// block reductions (wave sums + LDS across the 4 waves)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
+__device__ __forceinline__ double wave_sum(double v) { return wave_sum64(v); }
 
 // Sums NV values over the 256-thread block; result valid in thread 0.
 template <int NV>

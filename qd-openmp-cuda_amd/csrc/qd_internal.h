@@ -135,6 +135,38 @@ __device__ __forceinline__ int locate_job(const QD_G int *base, int n_jobs, int 
   return lo;
 }
 
+#if defined(__HIPCC__)   // device helpers below need the amdgcn builtins (not in the g++ host objects)
+// Lane permutation of a double within rows of 16 lanes (DPP control CTRL):
+// a VALU move, no LDS traffic.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const long long v = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(v & 0xffffffffLL), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(v >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ double readlane_f64(double x, int lane) {
+  const long long v = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffffLL), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(v >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum of x over the 64 lanes of the wave, the same (uniform) value in every
+// lane; all lanes must be active.  Within each row of 16 lanes: add the quad
+// neighbours (lane ^ 1, lane ^ 2), then the row rotated by 4 and by 8; then
+// the four row sums, read from lanes 0, 16, 32, 48, in a fixed order.  DPP
+// moves and lane reads only (no LDS permutes), and a fixed order of every
+// addition, so the result is reproducible bit for bit.
+__device__ __forceinline__ double wave_sum64(double x) {
+  x += dpp_f64<0xB1>(x);    // quad_perm [1, 0, 3, 2]
+  x += dpp_f64<0x4E>(x);    // quad_perm [2, 3, 0, 1]
+  x += dpp_f64<0x124>(x);   // row_ror 4
+  x += dpp_f64<0x128>(x);   // row_ror 8
+  return (readlane_f64(x, 0) + readlane_f64(x, 16)) + (readlane_f64(x, 32) + readlane_f64(x, 48));
+}
+#endif
+
 // Applies the step-counter advance flagged by the previous step's reduction;
 // called by one thread of the job's first workgroup of each step's first
 // kernel (kernel boundaries order it after that reduction and before this
