@@ -25,10 +25,32 @@ __device__ __forceinline__ double2 rot_i(double2 a) {   // multiply by -i (forwa
   return INV ? cmul_pi(a) : cmul_mi(a);
 }
 
+// a * exp(-+ i pi / 4): the radix-8 twiddle w^1 (forward -, inverse +)
+template <bool INV>
+__device__ __forceinline__ double2 rot_w8(double2 a) {
+  constexpr double h = 0.70710678118654752440;
+  return INV ? make_double2(h * (a.x - a.y), h * (a.x + a.y)) : make_double2(h * (a.x + a.y), h * (a.y - a.x));
+}
+
 // In-register DFT of R points, forward exp(-2 pi i qs/R) or inverse.
 template <bool INV, int R>
 __device__ __forceinline__ void dft_small(const double2 (&v)[R], double2 (&X)[R]) {
-  if constexpr (R == 2) {
+  if constexpr (R == 8) {
+    // two 4-point DFTs of the even and odd points, combined with w^k
+    const double2 ev[4] = {v[0], v[2], v[4], v[6]}, od[4] = {v[1], v[3], v[5], v[7]};
+    double2 E[4], O[4];
+    dft_small<INV, 4>(ev, E);
+    dft_small<INV, 4>(od, O);
+    const double2 o1 = rot_w8<INV>(O[1]), o2 = rot_i<INV>(O[2]), o3 = rot_i<INV>(rot_w8<INV>(O[3]));
+    X[0] = cadd(E[0], O[0]);
+    X[4] = csub(E[0], O[0]);
+    X[1] = cadd(E[1], o1);
+    X[5] = csub(E[1], o1);
+    X[2] = cadd(E[2], o2);
+    X[6] = csub(E[2], o2);
+    X[3] = cadd(E[3], o3);
+    X[7] = csub(E[3], o3);
+  } else if constexpr (R == 2) {
     X[0] = cadd(v[0], v[1]);
     X[1] = csub(v[0], v[1]);
   } else if constexpr (R == 4) {

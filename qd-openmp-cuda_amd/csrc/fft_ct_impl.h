@@ -15,13 +15,29 @@
 
 namespace qd {
 
-// Stockham stages NS..N of an in-place transform of NPEN pencils; pencil p
-// element e is A[p * PS + e * ES].  PF maps consecutive threads to
-// consecutive pencils.  Entry and exit are synchronised.
-template <bool INV, int N, int NS, int NPEN, int PS, int ES, bool PF>
+// Radix plans.  PlanStd: 4s first, then 2, 3s, 5s.  PlanEnds: the first and
+// the last stage share the radix R1 (8 when 64 divides n, else 4; n must be a
+// multiple of R1^2) with the standard plan in between, so the last stage of
+// a transform leaves each thread the points (idx + q n/R1) that the first
+// stage of the next transform takes.
+struct PlanStd {
+  static constexpr int radix(int n, int ns) { return next_radix(n / ns); }
+};
+struct PlanEnds {
+  static constexpr int first(int n) { return n % 64 == 0 ? 8 : 4; }
+  static constexpr int radix(int n, int ns) {
+    return (ns == 1 || ns * first(n) == n) ? first(n) : next_radix(n / (ns * first(n)));
+  }
+};
+
+// Stockham stages NS.. (up to, not including, the stage of stride NSEND) of
+// an in-place transform of NPEN pencils; pencil p element e is
+// A[p * PS + e * ES].  PF maps consecutive threads to consecutive pencils.
+// Entry and exit are synchronised.
+template <bool INV, int N, int NS, int NPEN, int PS, int ES, bool PF, class PLAN = PlanStd, int NSEND = N>
 __device__ __forceinline__ void stages_ip(double2 *A, const double2 *tw) {
-  if constexpr (NS < N) {
-    constexpr int R = next_radix(N / NS);
+  if constexpr (NS < NSEND) {
+    constexpr int R = PLAN::radix(N, NS);
     constexpr int NB = N / R;
     constexpr int TOTAL = NPEN * NB;
     constexpr int MB = (TOTAL + kBlock - 1) / kBlock;
@@ -61,11 +77,10 @@ __device__ __forceinline__ void stages_ip(double2 *A, const double2 *tw) {
       }
     }
     __syncthreads();
-    stages_ip<INV, N, NS * R, NPEN, PS, ES, PF>(A, tw);
+    stages_ip<INV, N, NS * R, NPEN, PS, ES, PF, PLAN, NSEND>(A, tw);
   }
 }
 
-// rows per workgroup of the R pass and columns per workgroup of the r pass
 // Rows per workgroup of the R pass: 4 for n1 > 128; for shorter pencils 8,
 // or 2 when more than 5 omega blocks are held in registers across the
 // transform (measured: fewer rows lose more to the per-workgroup table
@@ -73,7 +88,9 @@ __device__ __forceinline__ void stages_ip(double2 *A, const double2 *tw) {
 __host__ __device__ constexpr int ct_rows(int n1, int nb) {
   return n1 > 128 ? 4 : (nb <= 5 ? 8 : 2);
 }
-__host__ __device__ constexpr int ct_cols(int n2) { return n2 <= 64 ? 16 : 8; }
+// Columns per workgroup of the r pass: 16 or 8, so the boundary stages of
+// its radix plan have 128..256 butterflies for the 256 threads.
+__host__ __device__ constexpr int ct_cols(int n2) { return (n2 <= 64 || n2 == 128) ? 16 : 8; }
 
 // ---------------------------------------------------------------------------
 // R pass with the Coriolis mix (forward: mix then DFT; inverse: DFT, scale,
@@ -274,13 +291,23 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// r pass: forward DFT along r, kinetic phase and energy, inverse DFT
+// r pass: forward DFT along r, kinetic phase and energy, inverse DFT.
+// PlanEnds makes the boundary stages of both transforms take the points
+// (p, idx + q n2/R1) of butterfly (p, idx): the forward first stage reads
+// them from HBM, the forward last stage ends with them in registers, where
+// the kinetic phase is applied and the inverse first stage runs at once, and
+// the inverse last stage writes them to HBM.  LDS sees only the middle
+// stages: 8 accesses per point instead of 20 with an LDS round trip per stage.
 // ---------------------------------------------------------------------------
 template <int N2>
 __global__ __launch_bounds__(kBlock) void k_fft_r_ct(StepArgs a) {
   constexpr int Q = ct_cols(N2);
   constexpr int QN = Q * N2;
-  __shared__ double2 A[QN];
+  constexpr int R1 = PlanEnds::first(N2), NB1 = N2 / R1;   // radix and butterflies per pencil at both ends
+  constexpr int NSL = N2 / R1;                           // stride of the last stage (= NB1)
+  constexpr int T1 = Q * NB1, MB = (T1 + kBlock - 1) / kBlock;
+  static_assert(N2 % (R1 * R1) == 0, "PlanEnds needs n2 to be a multiple of R1^2");
+  __shared__ double2 A[QN];   // pencil p element j at A[p + j Q]
   __shared__ double2 tw[N2];
   __shared__ double red[4];
   const int job = locate_job(a.unit_base, a.n_jobs, blockIdx.x);
@@ -289,21 +316,88 @@ __global__ __launch_bounds__(kBlock) void k_fft_r_ct(StepArgs a) {
   const int n1 = jb.n1;
   int b, l, g;
   decode_slab_unit(jb, u, n1 / Q, b, l, g);
-  QD_G double2 *base = jb.fbr + ((long)b * jb.L + l) * jb.M + (long)g * Q;
+  QD_G double2 *base = jb.fbr + ((long)b * jb.L + l) * jb.M + (long)g * Q;   // (p, j) at base[j n1 + p]
   for (int e = threadIdx.x; e < N2; e += kBlock) tw[e] = jb.twr[e];
-  for (int e = threadIdx.x; e < QN; e += kBlock) A[e] = base[(long)(e / Q) * n1 + (e % Q)];
-  __syncthreads();
-  stages_ip<false, N2, 1, Q, 1, Q, true>(A, tw);
-  double ke = 0.0;
-  for (int e = threadIdx.x; e < QN; e += kBlock) {
-    const int i = g * Q + (e % Q), j = e / Q;
-    const double2 x = cmul(A[e], cmul(jb.kin1[i], jb.kin2[j]));
-    A[e] = x;
-    ke += cnorm(x) * (jb.kinE1[i] + jb.kinE2[j]);
+  double2 X[MB][R1];
+  // forward first stage (stride 1) on points read from HBM
+#pragma unroll
+  for (int m = 0; m < MB; m++) {
+    const int t = threadIdx.x + m * kBlock;
+    if (T1 % kBlock == 0 || t < T1) {
+      const int p = t % Q, idx = t / Q;
+      double2 v[R1];
+#pragma unroll
+      for (int q = 0; q < R1; q++) v[q] = base[(long)(idx + q * NB1) * n1 + p];
+      dft_small<false, R1>(v, X[m]);
+#pragma unroll
+      for (int q = 0; q < R1; q++) A[p + (idx * R1 + q) * Q] = X[m][q];
+    }
   }
   __syncthreads();
-  stages_ip<true, N2, 1, Q, 1, Q, true>(A, tw);
-  for (int e = threadIdx.x; e < QN; e += kBlock) base[(long)(e / Q) * n1 + (e % Q)] = A[e];
+  stages_ip<false, N2, R1, Q, 1, Q, true, PlanEnds, NSL>(A, tw);
+  // forward last stage (stride NSL, twiddles w^(q idx)) into registers:
+  // outputs at j = idx + q NSL; kinetic phase and energy; inverse first stage
+  double ke = 0.0;
+#pragma unroll
+  for (int m = 0; m < MB; m++) {
+    const int t = threadIdx.x + m * kBlock;
+    if (T1 % kBlock == 0 || t < T1) {
+      const int p = t % Q, idx = t / Q;
+      const int i = g * Q + p;
+      double2 v[R1], y[R1];
+#pragma unroll
+      for (int q = 0; q < R1; q++) v[q] = A[p + (idx + q * NB1) * Q];
+      if constexpr (NSL > 1) {
+#pragma unroll
+        for (int q = 1; q < R1; q++) v[q] = cmul(v[q], tw[q * idx]);
+      }
+      dft_small<false, R1>(v, y);
+      const double2 k1 = jb.kin1[i];
+      const double e1 = jb.kinE1[i];
+#pragma unroll
+      for (int q = 0; q < R1; q++) {
+        const int j = idx + q * NSL;
+        const double2 x = cmul(y[q], cmul(k1, jb.kin2[j]));
+        ke += cnorm(x) * (e1 + jb.kinE2[j]);
+        y[q] = x;
+      }
+      dft_small<true, R1>(y, X[m]);
+    }
+  }
+  __syncthreads();   // every read of the last stage precedes the in-place writes
+#pragma unroll
+  for (int m = 0; m < MB; m++) {
+    const int t = threadIdx.x + m * kBlock;
+    if (T1 % kBlock == 0 || t < T1) {
+      const int p = t % Q, idx = t / Q;
+#pragma unroll
+      for (int q = 0; q < R1; q++) A[p + (idx * R1 + q) * Q] = X[m][q];
+    }
+  }
+  __syncthreads();
+  stages_ip<true, N2, R1, Q, 1, Q, true, PlanEnds, NSL>(A, tw);
+  // inverse last stage, written to HBM
+#pragma unroll
+  for (int m = 0; m < MB; m++) {
+    const int t = threadIdx.x + m * kBlock;
+    if (T1 % kBlock == 0 || t < T1) {
+      const int p = t % Q, idx = t / Q;
+      double2 v[R1], y[R1];
+#pragma unroll
+      for (int q = 0; q < R1; q++) v[q] = A[p + (idx + q * NB1) * Q];
+      if constexpr (NSL > 1) {
+#pragma unroll
+        for (int q = 1; q < R1; q++) {
+          double2 w = tw[q * idx];
+          w.y = -w.y;
+          v[q] = cmul(v[q], w);
+        }
+      }
+      dft_small<true, R1>(v, y);
+#pragma unroll
+      for (int q = 0; q < R1; q++) base[(long)(idx + q * NSL) * n1 + p] = y[q];
+    }
+  }
   ke = wave_sum64(ke);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ke;
   __syncthreads();

@@ -118,6 +118,16 @@ def test_parity_nineteen_blocks():
     check_parity(a, b)
 
 
+@pytest.mark.parametrize("n2", [48, 96, 128, 192, 256])
+def test_parity_r_pass_lengths(n2):
+    # every compile-time length of the r pass (32 and 64 are covered above):
+    # radix plans 4-3-4, 4-2-3-4, 8-2-8, 8-3-8, 8-4-8 with 16 or 8 columns per workgroup
+    spec = inputs.JobSpec(n1=32, n2=n2, ntheta=8, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
+                          J=1, parity=1, k_max=-1, omega0=0, j0=1, E_coll=0.02)
+    a, b = run_both(spec, 3)
+    check_parity(a, b)
+
+
 def test_parity_runtime_fft_lengths():
     # 80 = 2^4 * 5 and 40 are not compile-time lengths: exercises the runtime-size FFT path
     spec = inputs.JobSpec(n1=80, n2=40, ntheta=20, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
