@@ -178,7 +178,7 @@ __device__ __forceinline__ void gemm_wave(const GemmWave &gw, long strip0, int n
   v4d acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; t++) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
-  auto compute = [&](int tile, const double(&an)[kCh], const double(&tn)[kCh], const double(&vq)[NT][4]) {
+  auto compute = [&](int tile, const double(&an)[kCh], const double(&tn)[kCh], const double(&vh)[NT][2]) {
     const int c = tile % nch;
     const long r = row_of(tile);
     const bool rok = r < gw.rows;
@@ -240,6 +240,15 @@ __device__ __forceinline__ void gemm_wave(const GemmWave &gw, long strip0, int n
       }
     }
     if (c == nch - 1) {   // epilogue of the strip
+      // w V of item it: this lane loaded items 2 odd + {0, 1}, its partner
+      // lane (the other component of the same point) the other two
+      double vo[NT][2];
+      if (OBS) {
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+          for (int h = 0; h < 2; h++) vo[t][h] = swap_adjacent(vh[t][h]);
+      }
 #pragma unroll
       for (int t = 0; t < NT; t++) {
 #pragma unroll
@@ -248,25 +257,30 @@ __device__ __forceinline__ void gemm_wave(const GemmWave &gw, long strip0, int n
           const bool ok = rok && col < gw.N;
           const double v = acc[t][it];
           if (ok) gw.C[(long)col * gw.ldc + r] = v;
-          if (OBS) ob += (rok ? v * v : 0.0) * vq[t][it];   // vq = 0 for col >= N
+          if (OBS) {
+            const double wv = ((it >> 1) == (int)odd) ? vh[t][it & 1] : vo[t][it & 1];   // 0 for col >= N
+            ob += (rok ? v * v : 0.0) * wv;
+          }
         }
       }
     }
   };
-  // potentials of the strip of `tile` (backward epilogue), issued before the prefetch
-  // weighted potentials w_k V of the strip's outputs (the table holds NT * 16
-  // theta rows, zero past N: no clamp)
-  auto load_pot = [&](int tile, double(&vq)[NT][4]) {
+  // weighted potentials w_k V of the strip of `tile` (backward epilogue),
+  // issued before the prefetch.  The two lanes of one point (rows 2m, 2m + 1)
+  // need the same 4 NT values: each loads the half with items it = 2 odd + h
+  // and the epilogue swaps the halves (the table holds NT * 16 theta rows,
+  // zero past N: no clamp).
+  auto load_pot = [&](int tile, double(&vh)[NT][2]) {
     const long r = row_of(tile);
     const uint32_t m = (uint32_t)((r < gw.rows ? r : gw.rows - 1) >> 1);
-    const uint32_t vb = ((uint32_t)acc_col<MIXED>(0, lk, 0) * (uint32_t)gw.M + m) * 8u;
+    const uint32_t vb = ((uint32_t)acc_col<MIXED>(0, lk, 2 * (int)odd) * (uint32_t)gw.M + m) * 8u;
 #pragma unroll
     for (int t = 0; t < NT; t++)
 #pragma unroll
-      for (int it = 0; it < 4; it++) {
-        const int c0 = acc_col<MIXED>(t, 0, it);   // uniform part of the column
+      for (int h = 0; h < 2; h++) {
+        const int c0 = acc_col<MIXED>(t, 0, h);   // uniform part of the column
         const QD_G char *pr = (const QD_G char *)(gw.wpot + (long)c0 * gw.M);
-        vq[t][it] = *(const QD_G double *)(pr + vb);
+        vh[t][h] = *(const QD_G double *)(pr + vb);
       }
   };
 #pragma unroll
@@ -275,10 +289,10 @@ __device__ __forceinline__ void gemm_wave(const GemmWave &gw, long strip0, int n
 #pragma unroll
     for (int d = 0; d < DEPTH; d++) {
       const int tile = t0 + d;
-      double vq[NT][4];
-      if (OBS && tile < ntiles && tile % nch == nch - 1) load_pot(tile, vq);
+      double vh[NT][2];
+      if (OBS && tile < ntiles && tile % nch == nch - 1) load_pot(tile, vh);
       load(tile + DEPTH - 1, abuf[(d + DEPTH - 1) % DEPTH], tbuf[(d + DEPTH - 1) % DEPTH]);
-      if (tile < ntiles) compute(tile, abuf[d], tbuf[d], vq);
+      if (tile < ntiles) compute(tile, abuf[d], tbuf[d], vh);
     }
   }
 }
