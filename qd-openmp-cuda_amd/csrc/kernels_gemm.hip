@@ -45,9 +45,10 @@ __host__ __device__ constexpr int lds_stride(int nt_tiles) { return nt_tiles * 1
 // keeps two waves per SIMD (VGPRs + AGPRs <= 256) where the tile allows it.
 // The forward pass also holds the phase operands and runs KC = 8 three deep;
 // the backward pass holds the potentials of the epilogue and runs one or two
-// wide tiles per strip.
-inline int leg_chunk(int ksteps, bool fwd, int nt_tiles) {
-  if (fwd || nt_tiles > 4) return 8;
+// wide tiles per strip, except in the mixed form, whose FP32 hi/lo operands
+// and partial accumulators leave two waves per SIMD only at KC = 8.
+inline int leg_chunk(int ksteps, bool fwd, int nt_tiles, bool mixed) {
+  if (fwd || nt_tiles > 4 || mixed) return 8;
   int best = 8, waste = 1 << 30;
   for (int kc : {16, 12, 8}) {
     const int w = (ksteps + kc - 1) / kc * kc - ksteps;
@@ -382,7 +383,7 @@ static int leg_chunk_override() {
 
 template <int NT, bool FWD>
 static void launch_leg_nt(const StepArgs &a, int total_units, int kp_max, int mixed, hipStream_t s) {
-  const int kc = leg_chunk_override() ? leg_chunk_override() : leg_chunk(kp_max / 4, FWD, NT);
+  const int kc = leg_chunk_override() ? leg_chunk_override() : leg_chunk(kp_max / 4, FWD, NT, mixed != 0);
   switch (NT <= 8 ? kc : 8) {
     case 16: launch_leg_k<NT, NT <= 8 ? 16 : 8, FWD>(a, total_units, kp_max, mixed, s); break;
     case 12: launch_leg_k<NT, NT <= 8 ? 12 : 8, FWD>(a, total_units, kp_max, mixed, s); break;
