@@ -11,6 +11,8 @@
 // slab l stay in registers (the Coriolis mix couples blocks point by point)
 // and only one block at a time passes through LDS.
 #pragma once
+#include <type_traits>
+
 #include "fft_common.h"
 
 namespace qd {
@@ -81,12 +83,55 @@ __device__ __forceinline__ void stages_ip(double2 *A, const double2 *tw) {
   }
 }
 
-// Rows per workgroup of the R pass: 4 for n1 > 128; for shorter pencils 8,
-// or 2 when more than 5 omega blocks are held in registers across the
-// transform (measured: fewer rows lose more to the per-workgroup table
-// staging than they gain in occupancy once n1 > 128).
+// Stride of the last stage of PlanStd for length n.
+__host__ __device__ constexpr int last_stage_ns(int n) {
+  int ns = 1;
+  while (ns * next_radix(n / ns) < n) ns *= next_radix(n / ns);
+  return ns;
+}
+
+// Last Stockham stage (stride NS, radix R = N / NS, twiddles w^(q idx)) of the
+// transforms of the NPEN pencils of a row-major tile A[p * N + e], into
+// registers: butterfly t = tid + m kBlock (p = t / NS, idx = t % NS) reads
+// and yields the points p N + idx + q NS, stored at y[m R + q] (threads past
+// the last butterfly compute pencil 0 again; their values are not used).
+// Its inputs are exactly its outputs' positions, so a thread that writes
+// back only these positions needs no barrier before it.  Entry must be
+// synchronised.
+template <bool INV, int N, int NS, int NPEN, int XPT>
+__device__ __forceinline__ void last_stage_own(const double2 *A, const double2 *tw, double2 (&y)[XPT]) {
+  constexpr int R = N / NS;
+  constexpr int MBL = XPT / R;
+  constexpr int TL = NPEN * NS;
+#pragma unroll
+  for (int m = 0; m < MBL; m++) {
+    const int t0 = threadIdx.x + m * kBlock;
+    const int t = (TL % kBlock == 0 || t0 < TL) ? t0 : 0;
+    const int p = t / NS, idx = t % NS;
+    double2 v[R], X[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) v[q] = A[p * N + idx + q * NS];
+    if constexpr (NS > 1) {
+#pragma unroll
+      for (int q = 1; q < R; q++) {
+        double2 w = tw[q * idx];
+        if (INV) w.y = -w.y;
+        v[q] = cmul(v[q], w);
+      }
+    }
+    dft_small<INV, R>(v, X);
+#pragma unroll
+    for (int q = 0; q < R; q++) y[m * R + q] = X[q];
+  }
+}
+
+// Rows per workgroup of the R pass: for n1 > 128, 4 (measured: fewer rows
+// lose more to the per-workgroup table staging than they gain in occupancy),
+// or 2 / 1 when more than 9 / 16 omega blocks are held in registers across
+// the transform (4 rows of 10+ blocks spill); for shorter pencils 8, or 2
+// when more than 5 blocks are held.
 __host__ __device__ constexpr int ct_rows(int n1, int nb) {
-  return n1 > 128 ? 4 : (nb <= 5 ? 8 : 2);
+  return n1 > 128 ? (nb <= 9 ? 4 : (nb <= 16 ? 2 : 1)) : (nb <= 5 ? 8 : 2);
 }
 // Columns per workgroup of the r pass: 16 or 8, so the boundary stages of
 // its radix plan have 128..256 butterflies for the 256 threads.
@@ -101,7 +146,33 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
   constexpr int NF = ACF ? 5 : 3;
   constexpr int P = ct_rows(N1, NB);
   constexpr int PN = P * N1;
+  // Points of this thread.  OWN: the outputs of its butterflies in the last
+  // FFT stage (stride NSL, radix RL): point k = m RL + q of butterfly
+  // t = tid + m kBlock (row p = t / NSL, idx = t % NSL) is e = p N1 + idx + q NSL.
+  // Every per-point step (loads, Coriolis mix, energies, stores) uses this
+  // map, so the last stage reads only this thread's points, runs in
+  // registers and writes HBM (forward) or the mix operands (inverse)
+  // directly.  Used where it gives each thread as many points as the plain
+  // map (e = tid + k kBlock), i.e. where the last stage keeps every thread
+  // busy; short pencils with few rows keep the plain map and a full LDS pass.
   constexpr int PPT = (PN + kBlock - 1) / kBlock;
+  constexpr int NSL = last_stage_ns(N1);
+  constexpr int RL = N1 / NSL;
+  constexpr int TL = P * NSL, MBL = (TL + kBlock - 1) / kBlock;
+  constexpr bool OWN = MBL * RL == PPT;
+  constexpr int XPT = PPT;
+  auto ept = [](int k) {
+    if constexpr (OWN) {
+      const int t = threadIdx.x + (k / RL) * kBlock;
+      return (t / NSL) * N1 + t % NSL + (k % RL) * NSL;
+    } else {
+      return (int)threadIdx.x + k * kBlock;
+    }
+  };
+  auto eok = [](int k) {
+    if constexpr (OWN) return TL % kBlock == 0 || (int)threadIdx.x + (k / RL) * kBlock < TL;
+    else return PN % kBlock == 0 || (int)threadIdx.x + k * kBlock < PN;
+  };
   __shared__ double2 A[PN];
   __shared__ double2 tw[N1];
   __shared__ double2 phs[N1 * NB];   // corPhi(l, i, a) of this slab, reused by the P rows
@@ -114,8 +185,10 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
   const int groups = jb.n2 / P;
   const int l = jb.omin + u / groups, g = u % groups;
   const int nb = min(jb.kmax, l) - jb.omin + 1;
-  // uniform over the workgroup and read-only: scalar loads into SGPRs
-  const QD_C double *V = (const QD_C double *)(jb.corV + (long)l * jb.nO * jb.nO);
+  // uniform over the workgroup and read-only: scalar loads into SGPRs for up
+  // to 5 blocks (larger NB^2 tables would spill the SGPRs)
+  using VPtr = std::conditional_t<(NB <= 5), const QD_C double *, const QD_G double *>;
+  const VPtr V = (VPtr)(jb.corV + (long)l * jb.nO * jb.nO);
   QD_G double2 *base = jb.fbr + (long)l * M + (long)g * PN;
   const QD_G double2 *c0 = ACF ? jb.c0 + (long)l * M + (long)g * PN : nullptr;
   for (int e = threadIdx.x; e < N1; e += kBlock) tw[e] = jb.twR[e];
@@ -131,21 +204,18 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
   // every global load of the slab tile in flight at once; loads are
   // unconditional from in-bounds addresses (block nb-1, last point), and
   // values of blocks >= nb and points >= PN are never used
-  double2 x[NB][PPT];
+  double2 x[NB][XPT];
 #pragma unroll
   for (int b = 0; b < NB; b++)
 #pragma unroll
-    for (int k = 0; k < PPT; k++) {
-      const int e = threadIdx.x + k * kBlock;
-      x[b][k] = base[(b < nb ? b : nb - 1) * LM + (PN % kBlock == 0 || e < PN ? e : PN - 1)];
-    }
+    for (int k = 0; k < XPT; k++) x[b][k] = base[(b < nb ? b : nb - 1) * LM + (eok(k) ? ept(k) : 0)];
   FbrSums<NB, NF> sm;
   sm.zero();
   if (!INV) {
 #pragma unroll
-    for (int k = 0; k < PPT; k++) {
-      const int e = threadIdx.x + k * kBlock;
-      if (PN % kBlock == 0 || e < PN) {
+    for (int k = 0; k < XPT; k++) {
+      const int e = ept(k);
+      if (eok(k)) {
         double2 xp[NB];
 #pragma unroll
         for (int b = 0; b < NB; b++) xp[b] = x[b][k];
@@ -159,16 +229,21 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
     for (int b = 0; b < NB; b++) {
       if (b < nb) {
 #pragma unroll
-        for (int k = 0; k < PPT; k++) {
-          const int e = threadIdx.x + k * kBlock;
-          if (PN % kBlock == 0 || e < PN) A[e] = x[b][k];
-        }
+        for (int k = 0; k < XPT; k++)
+          if (eok(k)) A[ept(k)] = x[b][k];
         __syncthreads();
-        stages_ip<false, N1, 1, P, N1, 1, false>(A, tw);
+        if constexpr (OWN) {
+          stages_ip<false, N1, 1, P, N1, 1, false, PlanStd, NSL>(A, tw);
+          double2 y[XPT];
+          last_stage_own<false, N1, NSL, P, XPT>(A, tw, y);
 #pragma unroll
-        for (int k = 0; k < PPT; k++) {
-          const int e = threadIdx.x + k * kBlock;
-          if (PN % kBlock == 0 || e < PN) base[b * LM + e] = A[e];
+          for (int k = 0; k < XPT; k++)
+            if (eok(k)) base[b * LM + ept(k)] = y[k];
+        } else {
+          stages_ip<false, N1, 1, P, N1, 1, false>(A, tw);
+#pragma unroll
+          for (int k = 0; k < XPT; k++)
+            if (eok(k)) base[b * LM + ept(k)] = A[ept(k)];
         }
       }
     }
@@ -181,7 +256,8 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
     // diagonal, corPhi past nO), so the sums over blocks below run over all
     // NB with clamped, in-range table indices and no per-block conditions.
     const int nOc = jb.nO - 1;
-    const QD_C double2 *cm = (const QD_C double2 *)(jb.cmat + (long)l * jb.nO);
+    using CPtr = std::conditional_t<(NB <= 5), const QD_C double2 *, const QD_G double2 *>;
+    const CPtr cm = (CPtr)(jb.cmat + (long)l * jb.nO);
     // Coriolis energy of block b - 1 once block b is out of the transform:
     //   sum I1 (C(b,b) |x_b|^2 + 2 C(b,b+1) Re(conj(x_b) x_{b+1})),
     // wave-reduced at once so one sum is live at a time
@@ -189,9 +265,9 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
       const double2 c = cm[min(b, nOc)];
       double ce = 0.0;
 #pragma unroll
-      for (int k = 0; k < PPT; k++) {
-        const int e = threadIdx.x + k * kBlock;
-        if (PN % kBlock == 0 || e < PN) {
+      for (int k = 0; k < XPT; k++) {
+        const int e = ept(k);
+        if (eok(k)) {
           double en = c.x * cnorm(x[b][k]);
           if (b + 1 < NB) en += 2.0 * c.y * cdot_re(x[b][k], x[b + 1 < NB ? b + 1 : b][k]);
           ce += en * jb.I1[e % N1];
@@ -204,29 +280,32 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
     for (int b = 0; b < NB; b++) {
       if (b < nb) {
 #pragma unroll
-        for (int k = 0; k < PPT; k++) {
-          const int e = threadIdx.x + k * kBlock;
-          if (PN % kBlock == 0 || e < PN) A[e] = x[b][k];
-        }
+        for (int k = 0; k < XPT; k++)
+          if (eok(k)) A[ept(k)] = x[b][k];
         __syncthreads();
-        stages_ip<true, N1, 1, P, N1, 1, false>(A, tw);
+        if constexpr (OWN) {
+          stages_ip<true, N1, 1, P, N1, 1, false, PlanStd, NSL>(A, tw);
+          last_stage_own<true, N1, NSL, P, XPT>(A, tw, x[b]);
+        } else {
+          stages_ip<true, N1, 1, P, N1, 1, false>(A, tw);
 #pragma unroll
-        for (int k = 0; k < PPT; k++) {
-          const int e = threadIdx.x + k * kBlock;
-          x[b][k] = (PN % kBlock == 0 || e < PN) ? cscale(A[e], s) : make_double2(0.0, 0.0);
+          for (int k = 0; k < XPT; k++)
+            if (eok(k)) x[b][k] = A[ept(k)];
         }
+#pragma unroll
+        for (int k = 0; k < XPT; k++) x[b][k] = eok(k) ? cscale(x[b][k], s) : make_double2(0.0, 0.0);
       } else {
 #pragma unroll
-        for (int k = 0; k < PPT; k++) x[b][k] = make_double2(0.0, 0.0);
+        for (int k = 0; k < XPT; k++) x[b][k] = make_double2(0.0, 0.0);
       }
       if (b > 0) coriolis_energy(b - 1);
     }
     coriolis_energy(NB - 1);
     // first stage, per point: t_a = phi(l, i, a) sum_b V_l[a][b] x_b, in place
 #pragma unroll
-    for (int k = 0; k < PPT; k++) {
-      const int e = threadIdx.x + k * kBlock;
-      if (PN % kBlock == 0 || e < PN) {
+    for (int k = 0; k < XPT; k++) {
+      const int e = ept(k);
+      if (eok(k)) {
         const double2 *ph = (const double2 *)phs + (e % N1) * NB;
         double2 t[NB];
 #pragma unroll
@@ -254,9 +333,9 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
       for (int f = 0; f < NF - 1; f++) sb[f] = 0.0;
       if (b < nb) {
 #pragma unroll
-        for (int k = 0; k < PPT; k++) {
-          const int e = threadIdx.x + k * kBlock;
-          if (PN % kBlock == 0 || e < PN) {
+        for (int k = 0; k < XPT; k++) {
+          const int e = ept(k);
+          if (eok(k)) {
             const int i = e % N1, j = g * P + e / N1;
             double2 acc = make_double2(0.0, 0.0);
 #pragma unroll

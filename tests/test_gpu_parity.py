@@ -118,6 +118,16 @@ def test_parity_nineteen_blocks():
     check_parity(a, b)
 
 
+def test_parity_long_R_many_blocks():
+    # 12 blocks on 192-point R pencils: the R pass holds 2 rows of 16 blocks
+    # per workgroup (the c3 path holds 4 rows of 5)
+    spec = inputs.JobSpec(n1=192, n2=32, ntheta=16, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
+                          J=12, parity=0, k_max=11, omega0=0, j0=2, E_coll=0.02)
+    a, b = run_both(spec, 2)
+    assert len(a.omegas) == 12
+    check_parity(a, b)
+
+
 @pytest.mark.parametrize("n2", [48, 96, 128, 192, 256])
 def test_parity_r_pass_lengths(n2):
     # every compile-time length of the r pass (32 and 64 are covered above):
