@@ -17,6 +17,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <limits>
 #include <numeric>
 
 #include "host_internal.h"
@@ -512,6 +513,10 @@ void DeviceJob::flux_after_step(hipStream_t s) {
   if (dev.flux_on && steps - flux_window_start_ >= dev.flux_T) flux_flush(s);
 }
 
+int DeviceJob::steps_to_flux_flush() const {
+  return dev.flux_on ? dev.flux_T - (steps - flux_window_start_) : std::numeric_limits<int>::max();
+}
+
 void DeviceJob::flux_to_crp(double *crp, hipStream_t s) {
   if (!dev.flux_on) return;
   flux_flush(s);
@@ -536,10 +541,12 @@ Engine::~Engine() {
 }
 
 void Engine::drop_graph() {
-  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
-  if (graph_) (void)hipGraphDestroy(graph_);
-  graph_exec_ = nullptr;
-  graph_ = nullptr;
+  for (int g = 0; g < 2; g++) {
+    if (graph_exec_[g]) (void)hipGraphExecDestroy(graph_exec_[g]);
+    if (graph_[g]) (void)hipGraphDestroy(graph_[g]);
+    graph_exec_[g] = nullptr;
+    graph_[g] = nullptr;
+  }
 }
 
 void Engine::set_jobs(const std::vector<DeviceJob *> &jobs) {
@@ -654,22 +661,24 @@ void Engine::step(hipEvent_t *ev) {
   enqueue(ev);
 }
 
-void Engine::step_graph() {
+void Engine::step_graph(int k) {
   if (jobs_.empty()) return;
-  if (!graph_exec_) {
+  QD_REQUIRE(k == 1 || k == kGraphSteps, "step graphs hold 1 or kGraphSteps steps");
+  const int g = k == 1 ? 0 : 1;
+  if (!graph_exec_[g]) {
     QD_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     try {
-      enqueue(nullptr);
+      for (int i = 0; i < k; i++) enqueue(nullptr);
     } catch (...) {
-      hipGraph_t g = nullptr;
-      (void)hipStreamEndCapture(stream_, &g);
-      if (g) (void)hipGraphDestroy(g);
+      hipGraph_t gr = nullptr;
+      (void)hipStreamEndCapture(stream_, &gr);
+      if (gr) (void)hipGraphDestroy(gr);
       throw;
     }
-    QD_HIP(hipStreamEndCapture(stream_, &graph_));
-    QD_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+    QD_HIP(hipStreamEndCapture(stream_, &graph_[g]));
+    QD_HIP(hipGraphInstantiate(&graph_exec_[g], graph_[g], nullptr, nullptr, 0));
   }
-  QD_HIP(hipGraphLaunch(graph_exec_, stream_));
+  QD_HIP(hipGraphLaunch(graph_exec_[g], stream_));
 }
 
 }  // namespace qd

@@ -7,6 +7,7 @@
 // and the observables back into caller memory, and frees the device.
 // No C++ exception or HIP error crosses this boundary: every failure becomes a
 // qd_status with a message.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -142,17 +143,34 @@ qd_status qd_run(qd_handle *h, int32_t n_steps) {
     DeviceJob &j = *h->job;
     const qd_options &op = h->in->options;
     const bool copy = (op.wave_to_matlab || op.wave_callback) && op.steps_to_copy_psi_from_device_to_host > 0;
-    for (int s = 0; s < n_steps; s++) {
-      const int steps_before = j.steps;
+    const int N = op.steps_to_copy_psi_from_device_to_host;
+    for (int s = 0; s < n_steps;) {
       j.prephase_if_first(h->stream);   // V(-dt/2) of step 0 (cudaOpenMP2.cu:344)
-      if (h->use_graph) h->engine->step_graph();
+      // Steps in this launch: kGraphSteps as one graph when no host-side
+      // event falls inside them (a copy-back follows the step whose
+      // pre-step counter is a multiple of N, the flux window fills, the
+      // observable ring fills), otherwise one.
+      int k = 1;
+      if (h->use_graph) {
+        int lim = std::min(n_steps - s, Engine::kGraphSteps);
+        lim = std::min(lim, j.steps_to_flux_flush());
+        lim = std::min(lim, j.dev.ring - (h->records_enqueued - j.records_done));
+        if (copy) {
+          const int r = j.steps % N;
+          lim = std::min(lim, r == 0 ? 1 : N - r + 1);
+        }
+        if (lim == Engine::kGraphSteps) k = lim;
+      }
+      if (h->use_graph) h->engine->step_graph(k);
       else h->engine->step();
-      h->records_enqueued++;
-      j.steps++;
+      h->records_enqueued += k;
+      j.steps += k;
+      s += k;
+      const int steps_before = j.steps - 1;   // pre-step counter of the launch's last step
       j.flux_after_step(h->stream);
       const bool ring_full = h->records_enqueued - j.records_done >= j.dev.ring;
-      const bool copy_now = copy && steps_before % op.steps_to_copy_psi_from_device_to_host == 0;
-      if (ring_full || copy_now || s == n_steps - 1) {
+      const bool copy_now = copy && steps_before % N == 0;
+      if (ring_full || copy_now || s == n_steps) {
         qd_status st = sync_and_check(h);
         if (st != QD_OK) return st;
       }

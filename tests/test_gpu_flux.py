@@ -65,6 +65,30 @@ def test_flux_multiblock_parity():
     assert np.abs(b.psi - a.psi).max() < 1e-10 * np.abs(a.psi).max()
 
 
+def test_multi_step_graphs_match_single_launches(monkeypatch):
+    """Runs of 100 and then 37 more steps replay 8-step graphs between the
+    host-side events (64-step flux window flushes, the end of a run); the
+    wavepackets, observables and P(E) equal a run that launches every step
+    on its own (no graphs) bit for bit."""
+    job = _multiblock_job()
+    _say("case flux 64x32x24 J=5 p=1 k_max=3 steps=100+37 graphs vs plain launches")
+    res = []
+    for no_graph in ("0", "1"):
+        monkeypatch.setenv("QD_NO_GRAPH", no_graph)
+        b = abi.Inputs(job, total_steps=137)
+        with api.Propagator(b) as p:
+            p.run(100)
+            p.run(37)
+            res.append((p.wavepackets(), p.observables(), p.crp(), int(b.steps[0])))
+    (pa, oa, ca, sa), (pb, ob, cb, sb) = res
+    _say(f"  max |dpsi| {np.abs(pa - pb).max():.3e}  max |dobs| {np.abs(oa - ob).max():.3e}  "
+         f"max |dP| {np.abs(ca - cb).max():.3e}")
+    assert sa == sb
+    assert np.array_equal(pa, pb)
+    assert np.array_equal(oa, ob)
+    assert np.array_equal(ca, cb)
+
+
 def test_crp_readout_mid_run():
     """qd_get_crp folds the open window; reading it after 50 steps and again
     after 100 gives the 100-step one-shot result."""
