@@ -125,13 +125,13 @@ __device__ __forceinline__ void last_stage_own(const double2 *A, const double2 *
   }
 }
 
-// Rows per workgroup of the R pass: for n1 > 128, 4 (measured: fewer rows
-// lose more to the per-workgroup table staging than they gain in occupancy),
-// or 2 / 1 when more than 9 / 16 omega blocks are held in registers across
-// the transform (4 rows of 10+ blocks spill); for shorter pencils 8, or 2
-// when more than 5 blocks are held.
+// Rows per workgroup of the R pass: for n1 > 128, 4 (measured with 5 blocks:
+// fewer rows lose more to the per-workgroup table staging than they gain in
+// occupancy), or 2 / 1 when more than 5 / 16 omega blocks are held in
+// registers across the transform (4 rows of 9 blocks leave one wave per SIMD);
+// for shorter pencils 8, or 2 when more than 5 blocks are held.
 __host__ __device__ constexpr int ct_rows(int n1, int nb) {
-  return n1 > 128 ? (nb <= 9 ? 4 : (nb <= 16 ? 2 : 1)) : (nb <= 5 ? 8 : 2);
+  return n1 > 128 ? (nb <= 5 ? 4 : (nb <= 16 ? 2 : 1)) : (nb <= 5 ? 8 : 2);
 }
 // Columns per workgroup of the r pass: 16 or 8, so the boundary stages of
 // its radix plan have 128..256 butterflies for the 256 threads.
@@ -177,6 +177,10 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
   __shared__ double2 tw[N1];
   __shared__ double2 phs[N1 * NB];   // corPhi(l, i, a) of this slab, reused by the P rows
   __shared__ double red[4 * NF * NB];
+  // more than 5 blocks: the slab's eigenvectors and C_l entries in LDS (too
+  // many for SGPRs; from global memory they are reloaded after every store)
+  __shared__ double Vs[NB > 5 ? NB * NB : 1];
+  __shared__ double2 cms[NB > 5 ? NB : 1];
   const int job = locate_job(a.unit_base, a.n_jobs, blockIdx.x);
   const JobDev &jb = a.jobs[job];
   const int u = blockIdx.x - a.unit_base[job];
@@ -186,12 +190,18 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
   const int l = jb.omin + u / groups, g = u % groups;
   const int nb = min(jb.kmax, l) - jb.omin + 1;
   // uniform over the workgroup and read-only: scalar loads into SGPRs for up
-  // to 5 blocks (larger NB^2 tables would spill the SGPRs)
-  using VPtr = std::conditional_t<(NB <= 5), const QD_C double *, const QD_G double *>;
-  const VPtr V = (VPtr)(jb.corV + (long)l * jb.nO * jb.nO);
+  // to 5 blocks, an LDS copy (staged below; nO <= NB) for more
+  const auto V = [&]() {
+    if constexpr (NB <= 5) return (const QD_C double *)(jb.corV + (long)l * jb.nO * jb.nO);
+    else return (const double *)Vs;
+  }();
   QD_G double2 *base = jb.fbr + (long)l * M + (long)g * PN;
   const QD_G double2 *c0 = ACF ? jb.c0 + (long)l * M + (long)g * PN : nullptr;
   for (int e = threadIdx.x; e < N1; e += kBlock) tw[e] = jb.twR[e];
+  if constexpr (NB > 5) {
+    for (int e = threadIdx.x; e < jb.nO * jb.nO; e += kBlock) Vs[e] = jb.corV[(long)l * jb.nO * jb.nO + e];
+    for (int e = threadIdx.x; e < jb.nO; e += kBlock) cms[e] = jb.cmat[(long)l * jb.nO + e];
+  }
   {
     const QD_G double2 *cp = jb.corPhi + (long)l * N1 * jb.nO;
     for (int e = threadIdx.x; e < N1 * NB; e += kBlock) {
@@ -256,8 +266,10 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
     // diagonal, corPhi past nO), so the sums over blocks below run over all
     // NB with clamped, in-range table indices and no per-block conditions.
     const int nOc = jb.nO - 1;
-    using CPtr = std::conditional_t<(NB <= 5), const QD_C double2 *, const QD_G double2 *>;
-    const CPtr cm = (CPtr)(jb.cmat + (long)l * jb.nO);
+    const auto cm = [&]() {
+      if constexpr (NB <= 5) return (const QD_C double2 *)(jb.cmat + (long)l * jb.nO);
+      else return (const double2 *)cms;
+    }();
     // Coriolis energy of block b - 1 once block b is out of the transform:
     //   sum I1 (C(b,b) |x_b|^2 + 2 C(b,b+1) Re(conj(x_b) x_{b+1})),
     // wave-reduced at once so one sum is live at a time
