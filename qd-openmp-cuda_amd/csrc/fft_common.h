@@ -210,7 +210,7 @@ __device__ __forceinline__ void store_fbr_partials(const JobDev &jb, int u, cons
   if (threadIdx.x < F * jb.nO) {
     const int q = threadIdx.x;   // q = F b + f, stored at field f of block b ([b][5][units])
     const double sum = (q < NV) ? (red[q] + red[NV + q]) + (red[2 * NV + q] + red[3 * NV + q]) : 0.0;
-    jb.part_cor[(long)(5 * (q / F) + q % F) * jb.units[KK_FFT_R] + u] = sum;
+    jb.part_cor[(long)(5 * (q / F) + q % F) * jb.units[KK_FFT_RI] + u] = sum;
   }
 }
 

@@ -125,13 +125,15 @@ __device__ __forceinline__ void last_stage_own(const double2 *A, const double2 *
   }
 }
 
-// Rows per workgroup of the R pass: for n1 > 128, 4 (measured with 5 blocks:
-// fewer rows lose more to the per-workgroup table staging than they gain in
-// occupancy), or 2 / 1 when more than 5 / 16 omega blocks are held in
-// registers across the transform (4 rows of 9 blocks leave one wave per SIMD);
-// for shorter pencils 8, or 2 when more than 5 blocks are held.
-__host__ __device__ constexpr int ct_rows(int n1, int nb) {
-  return n1 > 128 ? (nb <= 5 ? 4 : (nb <= 16 ? 2 : 1)) : (nb <= 5 ? 8 : 2);
+// Rows per workgroup of the R passes: for n1 > 128, 4 (fewer rows lose more
+// to the per-workgroup table staging than they gain in occupancy), or 2 / 1
+// when more than 9 / 16 omega blocks (the inverse pass: more than 5) are held
+// in registers across the transform (the inverse pass with 4 rows of 9
+// blocks has one wave per SIMD); for shorter pencils 8, or 2 when more than
+// 5 blocks are held.  Measured on 9 blocks of 192 x 128 x 64: forward 121 us
+// with 4 rows and 160 us with 2, inverse 289 us with 4 and 174 us with 2.
+__host__ __device__ constexpr int ct_rows(int n1, int nb, bool inv) {
+  return n1 > 128 ? (nb <= (inv ? 5 : 9) ? 4 : (nb <= 16 ? 2 : 1)) : (nb <= 5 ? 8 : 2);
 }
 // Columns per workgroup of the r pass: 16 or 8, so the boundary stages of
 // its radix plan have 128..256 butterflies for the 256 threads.
@@ -144,7 +146,7 @@ __host__ __device__ constexpr int ct_cols(int n2) { return (n2 <= 64 || n2 == 12
 template <bool INV, int NB, int N1, bool ACF>
 __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
   constexpr int NF = ACF ? 5 : 3;
-  constexpr int P = ct_rows(N1, NB);
+  constexpr int P = ct_rows(N1, NB, INV);
   constexpr int PN = P * N1;
   // Points of this thread.  OWN: the outputs of its butterflies in the last
   // FFT stage (stride NSL, radix RL): point k = m RL + q of butterfly
@@ -177,10 +179,13 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
   __shared__ double2 tw[N1];
   __shared__ double2 phs[N1 * NB];   // corPhi(l, i, a) of this slab, reused by the P rows
   __shared__ double red[4 * NF * NB];
-  // more than 5 blocks: the slab's eigenvectors and C_l entries in LDS (too
-  // many for SGPRs; from global memory they are reloaded after every store)
-  __shared__ double Vs[NB > 5 ? NB * NB : 1];
-  __shared__ double2 cms[NB > 5 ? NB : 1];
+  // inverse pass with more than 5 blocks: the slab's eigenvectors and C_l
+  // entries in LDS (too many for SGPRs; from global memory they would be
+  // reloaded after every store of the mix).  The forward mix has no stores
+  // and reads them from global memory, where the compiler keeps them scalar.
+  constexpr bool VLDS = INV && NB > 5;
+  __shared__ double Vs[VLDS ? NB * NB : 1];
+  __shared__ double2 cms[VLDS ? NB : 1];
   const int job = locate_job(a.unit_base, a.n_jobs, blockIdx.x);
   const JobDev &jb = a.jobs[job];
   const int u = blockIdx.x - a.unit_base[job];
@@ -190,15 +195,16 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
   const int l = jb.omin + u / groups, g = u % groups;
   const int nb = min(jb.kmax, l) - jb.omin + 1;
   // uniform over the workgroup and read-only: scalar loads into SGPRs for up
-  // to 5 blocks, an LDS copy (staged below; nO <= NB) for more
+  // to 5 blocks, an LDS copy (staged below; nO <= NB) for more in the inverse
   const auto V = [&]() {
     if constexpr (NB <= 5) return (const QD_C double *)(jb.corV + (long)l * jb.nO * jb.nO);
-    else return (const double *)Vs;
+    else if constexpr (VLDS) return (const double *)Vs;
+    else return (const QD_G double *)(jb.corV + (long)l * jb.nO * jb.nO);
   }();
   QD_G double2 *base = jb.fbr + (long)l * M + (long)g * PN;
   const QD_G double2 *c0 = ACF ? jb.c0 + (long)l * M + (long)g * PN : nullptr;
   for (int e = threadIdx.x; e < N1; e += kBlock) tw[e] = jb.twR[e];
-  if constexpr (NB > 5) {
+  if constexpr (VLDS) {
     for (int e = threadIdx.x; e < jb.nO * jb.nO; e += kBlock) Vs[e] = jb.corV[(long)l * jb.nO * jb.nO + e];
     for (int e = threadIdx.x; e < jb.nO; e += kBlock) cms[e] = jb.cmat[(long)l * jb.nO + e];
   }
@@ -376,7 +382,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
     if (threadIdx.x < NF * jb.nO) {
       const int q = threadIdx.x;   // q = NF b + f, stored at field f of block b ([b][5][units])
       const double sum = (q < NV) ? (red[q] + red[NV + q]) + (red[2 * NV + q] + red[3 * NV + q]) : 0.0;
-      jb.part_cor[(long)(5 * (q / NF) + q % NF) * jb.units[KK_FFT_R] + u] = sum;
+      jb.part_cor[(long)(5 * (q / NF) + q % NF) * jb.units[KK_FFT_RI] + u] = sum;
     }
   }
 }

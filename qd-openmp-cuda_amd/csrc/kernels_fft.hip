@@ -86,7 +86,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_R(StepArgs a) {
   const int job = locate_job(a.unit_base, a.n_jobs, blockIdx.x);
   const JobDev &jb = a.jobs[job];
   const int u = blockIdx.x - a.unit_base[job];
-  const int n1 = jb.n1, P = jb.fft_rows, M = jb.M, L = jb.L;
+  const int n1 = jb.n1, P = INV ? jb.fft_rows_inv : jb.fft_rows, M = jb.M, L = jb.L;
   const long LM = (long)L * M;
   const int groups = jb.n2 / P;
   const int l = jb.omin + u / groups, g = u % groups;

@@ -29,7 +29,7 @@ bool fft_ct_supported(int n) {
     default: return false;
   }
 }
-int fft_ct_rows(int n1, int n_blocks) { return ct_rows(n1, n_blocks); }
+int fft_ct_rows(int n1, int n_blocks, bool inverse) { return ct_rows(n1, n_blocks, inverse); }
 int fft_ct_cols(int n2) { return ct_cols(n2); }
 
 void launch_fft_R_ct(const StepArgs &a, int total_units, int inverse, int acf, int n1, int nb_max, hipStream_t s) {

@@ -88,7 +88,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce(StepArgs a) {
   } else if (f < 6) {
     // second FBR half step partials of the inverse-R pass: [b][5][units] =
     // Coriolis energy, rotational energy, norm, autocorrelation re / im
-    n = jb.units[KK_FFT_R];
+    n = jb.units[KK_FFT_RI];
     src = jb.part_cor + (5L * b + (f - 1)) * n;
   } else {        // potential energy partials of the backward Legendre epilogue: [b][group]
     n = jb.gemm_bwd_groups;

@@ -250,8 +250,12 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   static const int cols_c[] = {16, 8, 4, 2, 1};
   // compile-time FFT kernels where the lengths allow; otherwise the runtime-size
   // R pass keeps nb(l) <= nO blocks of fft_rows rows plus one scratch tile in LDS
-  if (fft_ct_supported(n1) && n2 % fft_ct_rows(n1, nO) == 0) d.fft_rows = fft_ct_rows(n1, nO);
-  else d.fft_rows = pick_divisor(n2, rows_c, 4, 5120 / (nO + 1), n1);
+  if (fft_ct_supported(n1) && n2 % fft_ct_rows(n1, nO, false) == 0 && n2 % fft_ct_rows(n1, nO, true) == 0) {
+    d.fft_rows = fft_ct_rows(n1, nO, false);
+    d.fft_rows_inv = fft_ct_rows(n1, nO, true);
+  } else {
+    d.fft_rows = d.fft_rows_inv = pick_divisor(n2, rows_c, 4, 5120 / (nO + 1), n1);
+  }
   if (fft_ct_supported(n2) && n1 % fft_ct_cols(n2) == 0) d.fft_cols = fft_ct_cols(n2);
   else d.fft_cols = pick_divisor(n1, cols_c, 5, 2048, n2);
   long slabs = 0;
@@ -259,6 +263,7 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   d.units[KK_LEG_FWD] = nO * d.gemm_fwd_groups;
   d.units[KK_LEG_BWD] = nO * d.gemm_bwd_groups;
   d.units[KK_FFT_R] = (L - sh.omin) * (n2 / d.fft_rows);
+  d.units[KK_FFT_RI] = (L - sh.omin) * (n2 / d.fft_rows_inv);
   d.units[KK_FFT_r] = (int)(slabs * (n1 / d.fft_cols));
   d.units[KK_REDUCE] = nO * kRedFields;
   d.units[KK_FLUX] = 0;
@@ -385,7 +390,7 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   d.cmat = upload(cmat, st);
   d.twR = upload(twiddles(n1), st);
   d.twr = upload(twiddles(n2), st);
-  d.part_cor = (double *)alloc(sizeof(double) * (size_t)d.units[KK_FFT_R] * nO * 5);
+  d.part_cor = (double *)alloc(sizeof(double) * (size_t)d.units[KK_FFT_RI] * nO * 5);
   d.part_ke = (double *)alloc(sizeof(double) * (size_t)d.units[KK_FFT_r]);
   d.part_obs = (double *)alloc(sizeof(double) * (size_t)nO * d.gemm_bwd_groups);
   if (sh.acf) {
@@ -592,10 +597,12 @@ void Engine::set_jobs(const std::vector<DeviceJob *> &jobs) {
     any_flux_ |= d.flux_on;
     any_acf_ |= d.c0 != nullptr;
   }
-  // the compile-time R pass takes fft_ct_rows(n1, nb_max) rows per workgroup:
+  // the compile-time R passes take fft_ct_rows(n1, nb_max, inverse) rows per workgroup:
   // every job must have been tiled that way
   for (int q = 0; q < n && ct_n1_; q++)
-    if (jobs[q]->dev.fft_rows != fft_ct_rows(ct_n1_, nb_max_)) ct_n1_ = 0;
+    if (jobs[q]->dev.fft_rows != fft_ct_rows(ct_n1_, nb_max_, false) ||
+        jobs[q]->dev.fft_rows_inv != fft_ct_rows(ct_n1_, nb_max_, true))
+      ct_n1_ = 0;
   for (int k = 0; k < KK_COUNT; k++) total_[k] = bases[(size_t)k * (n + 1) + n];
   QD_HIP(hipMemcpyAsync(d_jobs_, descs.data(), sizeof(JobDev) * n, hipMemcpyHostToDevice, stream_));
   QD_HIP(hipMemcpyAsync(d_bases_, bases.data(), sizeof(int) * bases.size(), hipMemcpyHostToDevice, stream_));
@@ -632,8 +639,8 @@ void Engine::enqueue(hipEvent_t *ev) {
   else launch_fft_r_rt(args(KK_FFT_r), total_[KK_FFT_r], lds_fftr_, s);
   end(QD_K_FFT_r);
   begin(QD_K_FFT_R_INV);
-  if (ct_n1_) launch_fft_R_ct(args(KK_FFT_R), total_[KK_FFT_R], 1, any_acf_, ct_n1_, nb_max_, s);
-  else launch_fft_R_rt(args(KK_FFT_R), total_[KK_FFT_R], 1, any_acf_, nb_max_, lds_fftR_, s);
+  if (ct_n1_) launch_fft_R_ct(args(KK_FFT_RI), total_[KK_FFT_RI], 1, any_acf_, ct_n1_, nb_max_, s);
+  else launch_fft_R_rt(args(KK_FFT_RI), total_[KK_FFT_RI], 1, any_acf_, nb_max_, lds_fftR_, s);
   end(QD_K_FFT_R_INV);
   begin(QD_K_LEG_BWD);
   launch_leg_bwd(args(KK_LEG_BWD), total_[KK_LEG_BWD], bwd_tiles_, bwd_kp_, mixed_, s);

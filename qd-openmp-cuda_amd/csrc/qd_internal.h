@@ -35,7 +35,8 @@ constexpr int kRedFields = 7;     // fields of the final reduction: ke, ce, re, 
 // Coriolis half steps and the DVR observables are fused into these
 enum KernelKind {
   KK_LEG_FWD = 0,
-  KK_FFT_R,
+  KK_FFT_R,     // forward R pass
+  KK_FFT_RI,    // inverse R pass (its own rows per workgroup)
   KK_FFT_r,
   KK_LEG_BWD,
   KK_REDUCE,
@@ -54,7 +55,7 @@ struct JobDev {
   int gemm_bwd_groups;       // workgroups per omega block (backward GEMM)
   int fwd_kp, fwd_np;        // padded K (mult of 4) and N (mult of 16), forward
   int bwd_kp, bwd_np;        // padded K and N, backward
-  int fft_rows;              // pencils per workgroup along R
+  int fft_rows, fft_rows_inv; // pencils per workgroup along R (forward, inverse R pass)
   int fft_cols;              // pencils per workgroup along r
   int units[KK_COUNT];       // work units (workgroups) per kernel for this job
   // physics
@@ -190,12 +191,12 @@ void launch_leg_bwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max
 // transform, after the inverse one; the inverse pass also forms the energies),
 // and the r pass with the kinetic phase.  _rt: any length factoring into
 // 2, 3, 5 (dynamic LDS); _ct: compile-time lengths (fft_ct_supported), with
-// fft_ct_rows(n1, blocks) rows / fft_ct_cols(n2) columns per workgroup (the
-// rows depend on whether the job has more than 5 omega blocks).
+// fft_ct_rows(n1, blocks, inverse) rows / fft_ct_cols(n2) columns per
+// workgroup (the rows depend on the number of omega blocks and on the pass).
 void launch_fft_R_rt(const StepArgs &a, int total_units, int inverse, int acf, int nb_max, size_t lds, hipStream_t stream);
 void launch_fft_r_rt(const StepArgs &a, int total_units, size_t lds, hipStream_t stream);
 bool fft_ct_supported(int n);
-int fft_ct_rows(int n1, int n_blocks);
+int fft_ct_rows(int n1, int n_blocks, bool inverse);
 int fft_ct_cols(int n2);
 void launch_fft_R_ct(const StepArgs &a, int total_units, int inverse, int acf, int n1, int nb_max, hipStream_t stream);
 void launch_fft_r_ct(const StepArgs &a, int total_units, int n2, hipStream_t stream);
