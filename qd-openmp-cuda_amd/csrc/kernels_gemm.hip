@@ -40,7 +40,8 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 __host__ __device__ constexpr int lds_stride(int nt_tiles) { return nt_tiles * 16 + ((nt_tiles & 1) ? 0 : 16); }
 
 // k-steps (of 4 k each) per pipelined tile: a template parameter KC in
-// {8, 12, 16}; KC = 8 runs three tiles deep, 12 and 16 two deep.  Deeper
+// {8, 12, 16}; KC = 8 runs three tiles deep (up to 6 column tiles; the mixed
+// form up to 4), 12 and 16 two deep.  Deeper
 // pipelines keep more bytes in flight per wave but cost registers; the choice
 // keeps two waves per SIMD (VGPRs + AGPRs <= 256) where the tile allows it.
 // The forward pass also holds the phase operands and runs KC = 8 three deep;
@@ -352,7 +353,7 @@ __global__ __launch_bounds__(kBlock) void k_leg(StepArgs a) {
   gw.wpot = jb.wpot; gw.M = M;
   double ob = 0.0;
   const int S = jb.gemm_strips;
-  gemm_wave<NT, KC, (KC == 8 && NT <= 4) ? 3 : 2, MIXED, FWD, !FWD>(gw, (long)g * S * 4 + wave, S, 4, ob);
+  gemm_wave<NT, KC, (KC == 8 && (NT <= 4 || (NT <= 6 && !MIXED))) ? 3 : 2, MIXED, FWD, !FWD>(gw, (long)g * S * 4 + wave, S, 4, ob);
   // first kernel of the step: apply the previous step's counter advance (at
   // the end, so the store cannot hold back the descriptor loads above)
   if (FWD && u == 0 && threadIdx.x == 0) advance_counters(jb);
