@@ -124,6 +124,7 @@ class Engine {
   size_t lds_fftR_ = 0, lds_fftr_ = 0;
   int ct_n1_ = 0, ct_n2_ = 0;   // lengths of the compile-time FFT kernels, 0 = runtime path
   int fwd_tiles_ = 0, bwd_tiles_ = 0, fwd_kp_ = 0, bwd_kp_ = 0, nb_max_ = 1;
+  int fwd_waves_ = 4, bwd_waves_ = 4;   // waves per GEMM workgroup (the same for every job)
   int mixed_ = 0, any_flux_ = 0, any_acf_ = 0;
   hipGraph_t graph_[2] = {};           // [0] one step, [1] kGraphSteps steps
   hipGraphExec_t graph_exec_[2] = {};

@@ -299,10 +299,13 @@ __device__ __forceinline__ void gemm_wave(const GemmWave &gw, long strip0, int n
   }
 }
 
-template <int NT, int KC, bool FWD, bool MIXED>
-__global__ __launch_bounds__(kBlock) void k_leg(StepArgs a) {
+// WAVES waves per workgroup share one LDS copy of B: 8 where B is too large
+// for two 4-wave workgroups per CU.
+template <int NT, int KC, bool FWD, bool MIXED, int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void k_leg(StepArgs a) {
+  constexpr int TB = WAVES * 64;
   extern __shared__ double lds_d[];
-  __shared__ double red[4];
+  __shared__ double red[WAVES];
   const int lu = xcd_logical_unit(blockIdx.x, gridDim.x);
   const int job = locate_job(a.unit_base, a.n_jobs, lu);
   const JobDev &jb = a.jobs[job];
@@ -326,14 +329,14 @@ __global__ __launch_bounds__(kBlock) void k_leg(StepArgs a) {
   float *Bh = (float *)lds_d, *Bl = Bh + kp * Ws;
   if (!MIXED) {
     const double *Bg = (FWD ? jb.wP : jb.Pt) + (long)b * kp * np;
-    for (int e = threadIdx.x; e < kp * W; e += kBlock) {
+    for (int e = threadIdx.x; e < kp * W; e += TB) {
       const int k = e / W, c = e % W;
       Bs[k * Ws + c] = c < np ? Bg[k * np + c] : 0.0;
     }
   } else {
     const float *Bgh = (FWD ? jb.wP_hi : jb.Pt_hi) + (long)b * kp * np;
     const float *Bgl = (FWD ? jb.wP_lo : jb.Pt_lo) + (long)b * kp * np;
-    for (int e = threadIdx.x; e < kp * W; e += kBlock) {
+    for (int e = threadIdx.x; e < kp * W; e += TB) {
       const int k = e / W, c = e % W;
       Bh[k * Ws + c] = c < np ? Bgh[k * np + c] : 0.f;
       Bl[k * Ws + c] = c < np ? Bgl[k * np + c] : 0.f;
@@ -352,8 +355,9 @@ __global__ __launch_bounds__(kBlock) void k_leg(StepArgs a) {
   gw.vph = (const QD_G double *)jb.vphase;
   gw.wpot = jb.wpot; gw.M = M;
   double ob = 0.0;
-  const int S = jb.gemm_strips;
-  gemm_wave<NT, KC, (KC == 8 && (NT <= 4 || (NT <= 6 && !MIXED))) ? 3 : 2, MIXED, FWD, !FWD>(gw, (long)g * S * 4 + wave, S, 4, ob);
+  const int S = FWD ? jb.gemm_strips : jb.gemm_bwd_strips;
+  gemm_wave<NT, KC, (KC == 8 && (NT <= 4 || (NT <= 6 && !MIXED))) ? 3 : 2, MIXED, FWD, !FWD>(
+      gw, (long)g * S * WAVES + wave, S, WAVES, ob);
   // first kernel of the step: apply the previous step's counter advance (at
   // the end, so the store cannot hold back the descriptor loads above)
   if (FWD && u == 0 && threadIdx.x == 0) advance_counters(jb);
@@ -361,15 +365,19 @@ __global__ __launch_bounds__(kBlock) void k_leg(StepArgs a) {
     ob = wave_sum64(ob);
     if (lane == 0) red[wave] = ob;
     __syncthreads();
-    if (threadIdx.x == 0) jb.part_obs[(long)b * groups + g] = (red[0] + red[1]) + (red[2] + red[3]);
+    if (threadIdx.x == 0) {
+      double sum = (red[0] + red[1]) + (red[2] + red[3]);
+      if constexpr (WAVES == 8) sum += (red[4] + red[5]) + (red[6] + red[7]);
+      jb.part_obs[(long)b * groups + g] = sum;
+    }
   }
 }
 
-template <int NT, int KC, bool FWD>
+template <int NT, int KC, bool FWD, int WAVES>
 static void launch_leg_k(const StepArgs &a, int total_units, int kp_max, int mixed, hipStream_t s) {
   const size_t lds = (size_t)kp_max * lds_stride(NT) * (mixed ? 2 * sizeof(float) : sizeof(double));
-  if (mixed) hipLaunchKernelGGL((k_leg<NT, KC, FWD, true>), dim3(total_units), dim3(kBlock), lds, s, a);
-  else hipLaunchKernelGGL((k_leg<NT, KC, FWD, false>), dim3(total_units), dim3(kBlock), lds, s, a);
+  if (mixed) hipLaunchKernelGGL((k_leg<NT, KC, FWD, true, WAVES>), dim3(total_units), dim3(WAVES * 64), lds, s, a);
+  else hipLaunchKernelGGL((k_leg<NT, KC, FWD, false, WAVES>), dim3(total_units), dim3(WAVES * 64), lds, s, a);
 }
 
 // QD_LEG_KC=8|12|16 overrides the k-chunk choice (tuning sweeps)
@@ -383,29 +391,34 @@ static int leg_chunk_override() {
 }
 
 template <int NT, bool FWD>
-static void launch_leg_nt(const StepArgs &a, int total_units, int kp_max, int mixed, hipStream_t s) {
+static void launch_leg_nt(const StepArgs &a, int total_units, int kp_max, int mixed, int waves, hipStream_t s) {
+  if (waves == 8) {   // large B: KC = 8 only
+    launch_leg_k<NT, 8, FWD, 8>(a, total_units, kp_max, mixed, s);
+    return;
+  }
   const int kc = leg_chunk_override() ? leg_chunk_override() : leg_chunk(kp_max / 4, FWD, NT, mixed != 0);
   switch (NT <= 8 ? kc : 8) {
-    case 16: launch_leg_k<NT, NT <= 8 ? 16 : 8, FWD>(a, total_units, kp_max, mixed, s); break;
-    case 12: launch_leg_k<NT, NT <= 8 ? 12 : 8, FWD>(a, total_units, kp_max, mixed, s); break;
-    default: launch_leg_k<NT, 8, FWD>(a, total_units, kp_max, mixed, s); break;
+    case 16: launch_leg_k<NT, NT <= 8 ? 16 : 8, FWD, 4>(a, total_units, kp_max, mixed, s); break;
+    case 12: launch_leg_k<NT, NT <= 8 ? 12 : 8, FWD, 4>(a, total_units, kp_max, mixed, s); break;
+    default: launch_leg_k<NT, 8, FWD, 4>(a, total_units, kp_max, mixed, s); break;
   }
 }
 
 template <bool FWD>
-static void launch_leg(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, hipStream_t s) {
+static void launch_leg(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, int waves,
+                       hipStream_t s) {
   switch (nt_tiles) {
-    case 1: launch_leg_nt<1, FWD>(a, total_units, kp_max, mixed, s); break;
-    case 2: launch_leg_nt<2, FWD>(a, total_units, kp_max, mixed, s); break;
-    case 3: launch_leg_nt<3, FWD>(a, total_units, kp_max, mixed, s); break;
-    case 4: launch_leg_nt<4, FWD>(a, total_units, kp_max, mixed, s); break;
-    case 5: launch_leg_nt<5, FWD>(a, total_units, kp_max, mixed, s); break;
-    case 6: launch_leg_nt<6, FWD>(a, total_units, kp_max, mixed, s); break;
-    case 7: launch_leg_nt<7, FWD>(a, total_units, kp_max, mixed, s); break;
-    case 8: launch_leg_nt<8, FWD>(a, total_units, kp_max, mixed, s); break;
-    case 10: launch_leg_nt<10, FWD>(a, total_units, kp_max, mixed, s); break;
-    case 12: launch_leg_nt<12, FWD>(a, total_units, kp_max, mixed, s); break;
-    case 15: launch_leg_nt<15, FWD>(a, total_units, kp_max, mixed, s); break;
+    case 1: launch_leg_nt<1, FWD>(a, total_units, kp_max, mixed, waves, s); break;
+    case 2: launch_leg_nt<2, FWD>(a, total_units, kp_max, mixed, waves, s); break;
+    case 3: launch_leg_nt<3, FWD>(a, total_units, kp_max, mixed, waves, s); break;
+    case 4: launch_leg_nt<4, FWD>(a, total_units, kp_max, mixed, waves, s); break;
+    case 5: launch_leg_nt<5, FWD>(a, total_units, kp_max, mixed, waves, s); break;
+    case 6: launch_leg_nt<6, FWD>(a, total_units, kp_max, mixed, waves, s); break;
+    case 7: launch_leg_nt<7, FWD>(a, total_units, kp_max, mixed, waves, s); break;
+    case 8: launch_leg_nt<8, FWD>(a, total_units, kp_max, mixed, waves, s); break;
+    case 10: launch_leg_nt<10, FWD>(a, total_units, kp_max, mixed, waves, s); break;
+    case 12: launch_leg_nt<12, FWD>(a, total_units, kp_max, mixed, waves, s); break;
+    case 15: launch_leg_nt<15, FWD>(a, total_units, kp_max, mixed, waves, s); break;
     default: break;   // rejected by leg_tiles_supported at validation
   }
 }
@@ -417,11 +430,13 @@ bool leg_tiles_supported(int nt_tiles) {
   }
 }
 
-void launch_leg_fwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, hipStream_t s) {
-  launch_leg<true>(a, total_units, nt_tiles, kp_max, mixed, s);
+void launch_leg_fwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, int waves,
+                    hipStream_t s) {
+  launch_leg<true>(a, total_units, nt_tiles, kp_max, mixed, waves, s);
 }
-void launch_leg_bwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, hipStream_t s) {
-  launch_leg<false>(a, total_units, nt_tiles, kp_max, mixed, s);
+void launch_leg_bwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, int waves,
+                    hipStream_t s) {
+  launch_leg<false>(a, total_units, nt_tiles, kp_max, mixed, waves, s);
 }
 
 // probe: C (rows x n, ldc) = A (rows x k, lda) . B ([kp][np] row-major), one strip per wave
@@ -467,17 +482,18 @@ void launch_gemm_probe(const double *A, int lda, const double *B, int kp, int np
 #undef QD_PROBE_CASE
 }
 
-template <int NT, int KC>
+template <int NT, int KC, int WAVES = 4>
 static void set_leg_attrs_k(int lim) {
-  (void)hipFuncSetAttribute((const void *)k_leg<NT, KC, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
-  (void)hipFuncSetAttribute((const void *)k_leg<NT, KC, false, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
-  (void)hipFuncSetAttribute((const void *)k_leg<NT, KC, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
-  (void)hipFuncSetAttribute((const void *)k_leg<NT, KC, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+  (void)hipFuncSetAttribute((const void *)k_leg<NT, KC, true, false, WAVES>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+  (void)hipFuncSetAttribute((const void *)k_leg<NT, KC, false, false, WAVES>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+  (void)hipFuncSetAttribute((const void *)k_leg<NT, KC, true, true, WAVES>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+  (void)hipFuncSetAttribute((const void *)k_leg<NT, KC, false, true, WAVES>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
 }
 
 template <int NT>
 static void set_leg_attrs(int lim) {
   set_leg_attrs_k<NT, 8>(lim);
+  set_leg_attrs_k<NT, 8, 8>(lim);
   if (NT <= 8) {
     set_leg_attrs_k<NT, NT <= 8 ? 12 : 8>(lim);
     set_leg_attrs_k<NT, NT <= 8 ? 16 : 8>(lim);

@@ -46,6 +46,9 @@ size_t gemm_lds_bytes(int kp, int np, bool mixed) {
   return (size_t)kp * gemm_lds_stride(nt) * (mixed ? 2 * sizeof(float) : sizeof(double));
 }
 
+constexpr size_t kLegLdsTwoGroups = 80 * 1024;
+constexpr long kCUs = 256;   // compute units of an MI355X
+
 int pick_divisor(int n, const int *cands, int ncand, int limit_elems, int other) {
   for (int q = 0; q < ncand; q++)
     if (n % cands[q] == 0 && cands[q] * other <= limit_elems) return cands[q];
@@ -243,9 +246,24 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   const long strips = (2L * M + 15) / 16;
   // long-lived workgroups: about two per CU in one round, so each wave runs
   // many pipelined tiles after paying the B staging and pipeline fill once
-  d.gemm_strips = (int)std::max(1L, (strips * nO + 4L * 512 - 1) / (4L * 512));
-  d.gemm_fwd_groups = (int)((strips + 4L * d.gemm_strips - 1) / (4L * d.gemm_strips));
-  d.gemm_bwd_groups = d.gemm_fwd_groups;
+  // A B table over 80 KB leaves room for one workgroup per CU: 8 waves then
+  // share it (two per SIMD, as two 4-wave workgroups would be).  The strips
+  // per wave make the workgroups of all blocks fit one round of the CUs
+  // (2 four-wave or 1 eight-wave workgroup per CU): each wave then runs many
+  // pipelined tiles after paying the B staging and the pipeline fill once.
+  d.gemm_fwd_waves = gemm_lds_bytes(d.fwd_kp, d.fwd_np, sh.mixed) > kLegLdsTwoGroups ? 8 : 4;
+  d.gemm_bwd_waves = gemm_lds_bytes(d.bwd_kp, d.bwd_np, sh.mixed) > kLegLdsTwoGroups ? 8 : 4;
+  auto strips_per_wave = [&](int waves) {
+    const long slots = waves == 8 ? kCUs : 2L * kCUs;
+    long S = std::max(1L, (strips * nO + waves * slots - 1) / (waves * slots));
+    while (nO * ((strips + waves * S - 1) / (waves * S)) > slots) S++;
+    return (int)S;
+  };
+  d.gemm_strips = strips_per_wave(d.gemm_fwd_waves);
+  d.gemm_bwd_strips = strips_per_wave(d.gemm_bwd_waves);
+  d.gemm_fwd_groups = (int)((strips + (long)d.gemm_fwd_waves * d.gemm_strips - 1) / ((long)d.gemm_fwd_waves * d.gemm_strips));
+  d.gemm_bwd_groups =
+      (int)((strips + (long)d.gemm_bwd_waves * d.gemm_bwd_strips - 1) / ((long)d.gemm_bwd_waves * d.gemm_bwd_strips));
   static const int rows_c[] = {8, 4, 2, 1};
   static const int cols_c[] = {16, 8, 4, 2, 1};
   // compile-time FFT kernels where the lengths allow; otherwise the runtime-size
@@ -577,11 +595,15 @@ void Engine::set_jobs(const std::vector<DeviceJob *> &jobs) {
   fwd_tiles_ = bwd_tiles_ = fwd_kp_ = bwd_kp_ = 0;
   nb_max_ = 1;
   mixed_ = jobs[0]->dev.mixed;
+  fwd_waves_ = jobs[0]->dev.gemm_fwd_waves;
+  bwd_waves_ = jobs[0]->dev.gemm_bwd_waves;
   any_flux_ = 0;
   any_acf_ = 0;
   for (int q = 0; q < n; q++) {
     const JobDev &d = jobs[q]->dev;
     QD_REQUIRE(d.mixed == mixed_, "all jobs of one engine must share a precision");
+    QD_REQUIRE(d.gemm_fwd_waves == fwd_waves_ && d.gemm_bwd_waves == bwd_waves_,
+               "all jobs of one engine must share the GEMM workgroup size");
     descs[q] = d;
     for (int k = 0; k < KK_COUNT; k++) bases[(size_t)k * (n + 1) + q + 1] = bases[(size_t)k * (n + 1) + q] + d.units[k];
     lds_fftR_ = std::max(lds_fftR_, (size_t)((d.nO + 1) * d.fft_rows * d.n1 + d.n1) * 16);
@@ -628,7 +650,7 @@ void Engine::enqueue(hipEvent_t *ev) {
     if (ev) QD_HIP(hipEventRecord(ev[2 * k + 1], s));
   };
   begin(QD_K_LEG_FWD);
-  launch_leg_fwd(args(KK_LEG_FWD), total_[KK_LEG_FWD], fwd_tiles_, fwd_kp_, mixed_, s);
+  launch_leg_fwd(args(KK_LEG_FWD), total_[KK_LEG_FWD], fwd_tiles_, fwd_kp_, mixed_, fwd_waves_, s);
   end(QD_K_LEG_FWD);
   begin(QD_K_FFT_R_FWD);
   if (ct_n1_) launch_fft_R_ct(args(KK_FFT_R), total_[KK_FFT_R], 0, 0, ct_n1_, nb_max_, s);
@@ -643,7 +665,7 @@ void Engine::enqueue(hipEvent_t *ev) {
   else launch_fft_R_rt(args(KK_FFT_RI), total_[KK_FFT_RI], 1, any_acf_, nb_max_, lds_fftR_, s);
   end(QD_K_FFT_R_INV);
   begin(QD_K_LEG_BWD);
-  launch_leg_bwd(args(KK_LEG_BWD), total_[KK_LEG_BWD], bwd_tiles_, bwd_kp_, mixed_, s);
+  launch_leg_bwd(args(KK_LEG_BWD), total_[KK_LEG_BWD], bwd_tiles_, bwd_kp_, mixed_, bwd_waves_, s);
   end(QD_K_LEG_BWD);
   if (any_flux_) {
     begin(QD_K_OBS);

@@ -50,7 +50,8 @@ struct JobDev {
   int J;
   int mixed;                 // 1: mixed-precision Legendre GEMMs
   // work decomposition
-  int gemm_strips;           // 16-row strips per wave in the Legendre GEMMs
+  int gemm_strips, gemm_bwd_strips;   // 16-row strips per wave in the forward / backward Legendre GEMM
+  int gemm_fwd_waves, gemm_bwd_waves;   // waves per GEMM workgroup (4, or 8 to share a large B in LDS)
   int gemm_fwd_groups;       // workgroups per omega block (forward GEMM)
   int gemm_bwd_groups;       // workgroups per omega block (backward GEMM)
   int fwd_kp, fwd_np;        // padded K (mult of 4) and N (mult of 16), forward
@@ -185,8 +186,10 @@ __device__ __forceinline__ void advance_counters(const JobDev &jb) {
 void launch_prephase(double2 *psi, const double *pot, long G, int nO, double dt, hipStream_t stream);
 // column tile counts (N / 16) with a Legendre GEMM instantiation
 bool leg_tiles_supported(int nt_tiles);
-void launch_leg_fwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, hipStream_t stream);
-void launch_leg_bwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, hipStream_t stream);
+void launch_leg_fwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, int waves,
+                    hipStream_t stream);
+void launch_leg_bwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, int waves,
+                    hipStream_t stream);
 // FFT along R fused with rotational . Coriolis . rotational (before the forward
 // transform, after the inverse one; the inverse pass also forms the energies),
 // and the r pass with the kinetic phase.  _rt: any length factoring into
