@@ -246,6 +246,13 @@ qd_status qd_server_submit(qd_server *s, qd_inputs *in, int32_t *job_id);
 /* admit, advance every running job by one step, retire finished jobs;
  * *n_running / *n_queued (may be NULL) report the state after the call */
 qd_status qd_server_step(qd_server *s, int32_t *n_running, int32_t *n_queued);
+/* qd_server_step that advances each shape class by up to max_steps steps:
+ * by 8 (replayed as one graph, saving the launch gaps between steps) when
+ * max_steps >= 8 and none of the class's jobs reaches time.total_steps or
+ * fills its 64-step flux window within them, otherwise by one.  Results are
+ * bitwise those of single steps; admission and retirement happen between
+ * calls, so larger steps trade arrival latency for throughput. */
+qd_status qd_server_advance(qd_server *s, int32_t max_steps, int32_t *n_running, int32_t *n_queued);
 /* state (QD_JOB_*), steps done, seconds from submit to retirement (or to now),
  * seconds from submit to admission (-1 while queued) */
 qd_status qd_server_job_status(qd_server *s, int32_t job_id, int32_t *state, int32_t *steps_done,

@@ -38,6 +38,7 @@ _DECLS = {
     "qd_server_create": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_void_p)]),
     "qd_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32)]),
     "qd_server_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "qd_server_advance": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "qd_server_job_status": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "qd_server_job_error": (C.c_char_p, [C.c_void_p, C.c_int32]),
@@ -172,9 +173,13 @@ class Server:
         self._inputs[jid.value] = inputs
         return jid.value
 
-    def step(self):
+    def step(self, max_steps: int = 1):
+        """qd_server_step (max_steps = 1) or qd_server_advance."""
         nr, nq = C.c_int32(), C.c_int32()
-        self._check(lib().qd_server_step(self.h, C.byref(nr), C.byref(nq)))
+        if max_steps == 1:
+            self._check(lib().qd_server_step(self.h, C.byref(nr), C.byref(nq)))
+        else:
+            self._check(lib().qd_server_advance(self.h, max_steps, C.byref(nr), C.byref(nq)))
         return nr.value, nq.value
 
     def status(self, job_id: int):

@@ -190,9 +190,12 @@ struct qd_server {
     c.members.swap(keep);
   }
 
-  void step() {
+  // max_steps >= Engine::kGraphSteps lets a class advance by kGraphSteps
+  // steps as one graph when none of its jobs reaches its target or fills its
+  // flux window within them (the ring holds kDrainEvery + kGraphSteps)
+  void step(int max_steps) {
     admit();
-    // enqueue one step of every class first, so the classes' streams run
+    // enqueue the steps of every class first, so the classes' streams run
     // concurrently; then drain / retire where needed
     std::vector<ShapeClass *> settle;
     for (auto &cp : classes) {
@@ -205,12 +208,18 @@ struct qd_server {
         c.dirty = false;
       }
       for (ServerJob *sj : c.members) sj->job->prephase_if_first(c.stream);
-      c.engine->step_graph();
+      int k = 1;
+      if (max_steps >= Engine::kGraphSteps) {
+        k = Engine::kGraphSteps;
+        for (ServerJob *sj : c.members)
+          if (sj->target - sj->enqueued < k || sj->job->steps_to_flux_flush() < k) k = 1;
+      }
+      c.engine->step_graph(k);
       bool finishing = false, drain = false;
       for (ServerJob *sj : c.members) {
         DeviceJob &j = *sj->job;
-        j.steps++;
-        sj->enqueued++;
+        j.steps += k;
+        sj->enqueued += k;
         j.flux_after_step(c.stream);
         finishing |= sj->enqueued >= sj->target;
         drain |= sj->enqueued - j.records_done >= kDrainEvery;
@@ -305,9 +314,14 @@ qd_status qd_server_submit(qd_server *s, qd_inputs *in, int32_t *job_id) {
 }
 
 qd_status qd_server_step(qd_server *s, int32_t *n_running, int32_t *n_queued) {
+  return qd_server_advance(s, 1, n_running, n_queued);
+}
+
+qd_status qd_server_advance(qd_server *s, int32_t max_steps, int32_t *n_running, int32_t *n_queued) {
   if (!s) return sfail(nullptr, QD_ERR_INVALID, "server is NULL");
+  if (max_steps < 1) return sfail(s, QD_ERR_INVALID, "max_steps must be >= 1");
   QD_SERVER_GUARD(s, {
-    s->step();
+    s->step(max_steps);
     if (n_running) *n_running = s->running();
     if (n_queued) *n_queued = (int32_t)s->queue.size();
     return QD_OK;

@@ -37,11 +37,13 @@ def prepare(stream: Sequence[Tuple[float, "inputs.JobSpec"]], precision: int = a
 
 
 def run_stream(prepared: Sequence[Tuple[float, abi.Inputs]], budget_bytes: int = 0, max_running: int = 0,
-               time_scale: float = 1.0) -> dict:
+               time_scale: float = 1.0, max_steps: int = 8) -> dict:
     """Replay the arrivals (seconds, scaled by time_scale) against one server.
 
-    Returns per-job latency and queueing times, p50/p99, and the aggregate
-    throughput over the span from the first submission to the last result."""
+    Each server call advances every shape class by up to max_steps steps
+    (qd_server_advance).  Returns per-job latency and queueing times,
+    p50/p99, and the aggregate throughput over the span from the first
+    submission to the last result."""
     order = sorted(range(len(prepared)), key=lambda q: prepared[q][0])
     with api.Server(budget_bytes, max_running) as srv:
         ids = {}
@@ -58,7 +60,7 @@ def run_stream(prepared: Sequence[Tuple[float, abi.Inputs]], budget_bytes: int =
                 active.add(q)
                 nxt += 1
             if active:
-                srv.step()
+                srv.step(max_steps)
                 steps += 1
                 for q in list(active):
                     if srv.status(ids[q])["state"] in (api.Server.DONE, api.Server.FAILED):

@@ -41,27 +41,31 @@ def _same(a, b):
     assert np.array_equal(a.crp, b.crp)
 
 
-def test_serve_mix_equals_alone():
+@pytest.mark.parametrize("max_steps", [1, 8])
+def test_serve_mix_equals_alone(max_steps):
+    # max_steps 8: qd_server_advance, 8-step graphs between the jobs' flux
+    # window flushes and targets (the first job runs 70 steps: one full
+    # 64-step window)
     specs = [
-        (_spec(64, 32, 24, 5, 1, 3, 30, n_energies=6, flux_r=2.8, energy_range=(0.012, 0.03)), abi.QD_PREC_FP64),
+        (_spec(64, 32, 24, 5, 1, 3, 70, n_energies=6, flux_r=2.8, energy_range=(0.012, 0.03)), abi.QD_PREC_FP64),
         (_spec(48, 32, 20, 2, 0, -1, 20), abi.QD_PREC_FP64),
         (_spec(64, 32, 24, 0, 0, -1, 25), abi.QD_PREC_FP64),
         (_spec(64, 32, 24, 3, 0, 2, 15), abi.QD_PREC_MIXED),
         (_spec(64, 32, 24, 4, 1, 4, 18), abi.QD_PREC_FP64),
     ]
-    _say("case serve mix: 5 jobs, 3 shapes, fp64 + mixed, staggered submission")
+    _say(f"case serve mix: 5 jobs, 3 shapes, fp64 + mixed, staggered submission, max_steps {max_steps}")
     pairs = [_pair(s, p) for s, p in specs]
     with api.Server() as srv:
         ids = [srv.submit(pairs[0][1]), srv.submit(pairs[1][1])]
         for _ in range(5):
-            srv.step()
+            srv.step(max_steps)
         ids.append(srv.submit(pairs[2][1]))
         ids.append(srv.submit(pairs[3][1]))
         for _ in range(3):
-            srv.step()
+            srv.step(max_steps)
         ids.append(srv.submit(pairs[4][1]))
         for _ in range(200):
-            nr, nq = srv.step()
+            nr, nq = srv.step(max_steps)
             if nr == 0 and nq == 0:
                 break
         st = [srv.status(i) for i in ids]
