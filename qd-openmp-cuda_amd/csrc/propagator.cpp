@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <numeric>
@@ -246,13 +247,26 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   const long strips = (2L * M + 15) / 16;
   // long-lived workgroups: about two per CU in one round, so each wave runs
   // many pipelined tiles after paying the B staging and pipeline fill once
-  // A B table over 80 KB leaves room for one workgroup per CU: 8 waves then
-  // share it (two per SIMD, as two 4-wave workgroups would be).  The strips
-  // per wave make the workgroups of all blocks fit one round of the CUs
-  // (2 four-wave or 1 eight-wave workgroup per CU): each wave then runs many
-  // pipelined tiles after paying the B staging and the pipeline fill once.
-  d.gemm_fwd_waves = gemm_lds_bytes(d.fwd_kp, d.fwd_np, sh.mixed) > kLegLdsTwoGroups ? 8 : 4;
-  d.gemm_bwd_waves = gemm_lds_bytes(d.bwd_kp, d.bwd_np, sh.mixed) > kLegLdsTwoGroups ? 8 : 4;
+  // Waves per GEMM workgroup.  8 waves share one LDS copy of B per CU (two
+  // per SIMD, as two 4-wave workgroups would be, with half the B staging):
+  // always when B is over 80 KB (one workgroup per CU fits), and otherwise
+  // when there are strips enough to give every CU one 8-wave workgroup
+  // (the J=20 workload: 72 / 74 us instead of 74 / 80 us).  Not where the
+  // 8-wave build spills (more than 8 column tiles; mixed: more than 5).  The
+  // strips per wave make the workgroups of all blocks fit one round of the
+  // CUs (2 four-wave or 1 eight-wave workgroup per CU): each wave then runs
+  // many pipelined tiles after paying the B staging and pipeline fill once.
+  auto pick_waves = [&](int kp, int np) {
+    if (np / 16 > (sh.mixed ? 5 : 8)) return 4;
+    if (gemm_lds_bytes(kp, np, sh.mixed) > kLegLdsTwoGroups) return 8;
+    return strips * nO >= 8L * kCUs ? 8 : 4;
+  };
+  d.gemm_fwd_waves = pick_waves(d.fwd_kp, d.fwd_np);
+  d.gemm_bwd_waves = pick_waves(d.bwd_kp, d.bwd_np);
+  if (const char *e = std::getenv("QD_LEG_WAVES")) {   // tuning sweeps: 4 or 8 for both GEMMs
+    const int w = std::atoi(e);
+    if (w == 4 || w == 8) d.gemm_fwd_waves = d.gemm_bwd_waves = w;
+  }
   auto strips_per_wave = [&](int waves) {
     const long slots = waves == 8 ? kCUs : 2L * kCUs;
     long S = std::max(1L, (strips * nO + waves * slots - 1) / (waves * slots));
