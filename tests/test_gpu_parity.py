@@ -118,6 +118,16 @@ def test_parity_nineteen_blocks():
     check_parity(a, b)
 
 
+@pytest.mark.parametrize("waves", ["4", "8"])
+def test_parity_gemm_workgroup_waves(monkeypatch, waves):
+    # the Legendre GEMMs in 4- and 8-wave workgroups (large jobs take 8) on the
+    # c3 block structure: 5 blocks, 32 theta points, 2 and 1 strips per wave
+    monkeypatch.setenv("QD_LEG_WAVES", waves)
+    spec = dataclasses.replace(inputs.config_spec("c3"), n1=96, n2=64, ntheta=32)
+    a, b = run_both(spec, 3)
+    check_parity(a, b)
+
+
 def test_parity_long_R_many_blocks():
     # 12 blocks on 192-point R pencils: the R pass holds 2 rows of 16 blocks
     # per workgroup (the c3 path holds 4 rows of 5)
@@ -149,6 +159,15 @@ def test_parity_runtime_fft_lengths():
 def test_parity_c1_shape():
     spec = inputs.config_spec("c1")
     a, b = run_both(spec, 4)
+    check_parity(a, b)
+
+
+def test_parity_c3_full_shape():
+    # the benchmark workload itself (192 x 128 x 64, J = 20, K_max = 4, 5 blocks):
+    # 8-wave GEMMs, the 192-point R passes and the 8-2-8 r pass at full size
+    spec = inputs.config_spec("c3")
+    a, b = run_both(spec, 2)
+    assert len(a.omegas) == 5
     check_parity(a, b)
 
 
