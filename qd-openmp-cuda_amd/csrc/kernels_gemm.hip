@@ -258,7 +258,10 @@ __device__ __forceinline__ void gemm_wave(const GemmWave &gw, long strip0, int n
           const int col = acc_col<MIXED>(t, lk, it);
           const bool ok = rok && col < gw.N;
           const double v = acc[t][it];
-          if (ok) gw.C[(long)col * gw.ldc + r] = v;
+          // streaming store: the output is read once, by the next pass; kept
+          // out of L2 it leaves room for the reused tables (backward GEMM on
+          // the J=20 workload: 74 -> 66 us)
+          if (ok) __builtin_nontemporal_store(v, &gw.C[(long)col * gw.ldc + r]);
           if (OBS) {
             const double wv = ((it >> 1) == (int)odd) ? vh[t][it & 1] : vo[t][it & 1];   // 0 for col >= N
             ob += (rok ? v * v : 0.0) * wv;
