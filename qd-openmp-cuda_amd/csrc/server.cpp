@@ -33,7 +33,7 @@ double now_s() {
 // for them: grid lengths (FFT kernels and tilings, including the R-pass rows,
 // which depend on whether a job has more than 5 omega blocks), Legendre tile
 // shapes and the precision.
-using ClassKey = std::tuple<int, int, int, int, int, int, int, int, int, int>;
+using ClassKey = std::tuple<int, int, int, int, int, int, int, int, int, int, int, int>;
 
 struct ServerJob {
   int id = 0;
@@ -74,7 +74,8 @@ struct qd_server {
 
   ShapeClass &class_for(const DeviceJob &j) {
     const JobDev &d = j.dev;
-    const ClassKey key{d.n1, d.n2, d.nt, d.fwd_np, d.bwd_np, d.fwd_kp, d.bwd_kp, d.mixed, d.fft_rows, d.fft_rows_inv};
+    const ClassKey key{d.n1,      d.n2,     d.nt,       d.fwd_np,       d.bwd_np,           d.fwd_kp,
+                       d.bwd_kp,  d.mixed,  d.fft_rows, d.fft_rows_inv, d.gemm_fwd_waves,   d.gemm_bwd_waves};
     for (auto &c : classes)
       if (c->key == key) return *c;
     std::unique_ptr<ShapeClass> c(new ShapeClass());

@@ -76,6 +76,26 @@ def test_serve_mix_equals_alone(max_steps):
         _same(alone, mixed)
 
 
+def test_serve_same_shape_different_gemm_workgroups():
+    # 96 x 64 x 16 with 2 and 3 blocks: the 3-block job has strips enough for
+    # 8-wave GEMM workgroups, the 2-block job takes 4-wave ones, so they run in
+    # different shape classes although their grids and FFT tilings agree
+    specs = [_spec(96, 64, 16, 1, 1, -1, 12), _spec(96, 64, 16, 2, 0, -1, 12)]
+    _say("case serve: same grid, 2 and 3 blocks, different GEMM workgroup sizes")
+    pairs = [_pair(s) for s in specs]
+    assert [len(a.omegas) for a, _ in pairs] == [2, 3]
+    with api.Server() as srv:
+        ids = [srv.submit(m) for _, m in pairs]
+        for _ in range(100):
+            nr, nq = srv.step(8)
+            if nr == 0 and nq == 0:
+                break
+        st = [srv.status(i) for i in ids]
+        assert all(s["state"] == api.Server.DONE for s in st), [srv.error(i) for i in ids]
+    for alone, mixed in pairs:
+        _same(alone, mixed)
+
+
 def test_serve_admission_by_memory_budget():
     spec = _spec(64, 32, 24, 5, 1, 3, 12)
     _say("case serve budget: 3 jobs of 64x32x24 J=5, budget for one")
