@@ -66,9 +66,9 @@ void launch_prephase(double2 *psi, const double *pot, long G, int nO, double dt,
 // turn; then the fixed block reduction) and writes that field of the step
 // record: no workgroup waits for another, so the kernel needs no
 // device-scope fence (each of which writes back an XCD's L2 on this chip).
-// Workgroup (0, 0) sets the pending-advance flag; the first workgroup of the
-// next step's first kernel advances the step counters (advance_counters), so
-// no workgroup here can read a counter that another has already advanced.
+// Workgroup (0, 0) sets the pending-advance flag; one workgroup of the next
+// step's first kernel advances the step counters (advance_counters), so no
+// workgroup here can read a counter that another has already advanced.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_reduce(StepArgs a) {
   __shared__ double red[4];

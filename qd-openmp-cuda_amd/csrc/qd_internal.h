@@ -96,7 +96,7 @@ struct JobDev {
   QD_G int *counters;             // device: [0] step counter (the reference's time.steps),
                              //         [1] records written, [2] advance pending: set by
                              //         the step's reduction, applied by advance_counters
-                             //         at the start of the next step's first kernel
+                             //         at the end of the next step's first kernel
   // flux at the dividing surface r2 = r_s (index flux_ns): per step the
   // surface kernel writes the values and the ng-point r-derivative of the
   // state with V(dt/2) restored into window slot (step % flux_T); every
@@ -170,9 +170,10 @@ __device__ __forceinline__ double wave_sum64(double x) {
 #endif
 
 // Applies the step-counter advance flagged by the previous step's reduction;
-// called by one thread of the job's first workgroup of each step's first
-// kernel (kernel boundaries order it after that reduction and before this
-// step's readers of the counters).
+// called, after its own work, by one thread of the job's first workgroup of
+// each step's first kernel (the forward Legendre GEMM, which reads no
+// counter).  Kernel boundaries order it after that reduction and before this
+// step's readers of the counters (flux surface kernel, reduction).
 __device__ __forceinline__ void advance_counters(const JobDev &jb) {
   if (jb.counters[2]) {
     jb.counters[0] += 1;
