@@ -162,6 +162,15 @@ def test_parity_c1_shape():
     check_parity(a, b)
 
 
+def test_parity_c1_long_run():
+    # 600 steps of the CPU-runnable workload (128 x 64 x 48, J = 0; the packet
+    # is still mostly on the grid): 8-step graphs throughout, and rounding
+    # must not grow past the per-step bar over the run
+    spec = inputs.config_spec("c1")
+    a, b = run_both(spec, 600)
+    check_parity(a, b)
+
+
 def test_parity_c3_full_shape():
     # the benchmark workload itself (192 x 128 x 64, J = 20, K_max = 4, 5 blocks):
     # 8-wave GEMMs, the 192-point R passes and the 8-2-8 r pass at full size
