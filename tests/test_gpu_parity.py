@@ -128,6 +128,17 @@ def test_parity_gemm_workgroup_waves(monkeypatch, waves):
     check_parity(a, b)
 
 
+def test_parity_theta_not_multiple_of_4():
+    # n_theta = 30: both Legendre GEMMs run on k-padded operands (forward K = 30
+    # in 32, backward K = 29 - Omega in 32), the forward through range-checked
+    # buffer loads and the zeroed psi slabs past the last block
+    spec = inputs.JobSpec(n1=64, n2=32, ntheta=30, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
+                          J=2, parity=0, k_max=-1, omega0=0, j0=1, E_coll=0.02)
+    a, b = run_both(spec, 3)
+    assert len(a.omegas) == 3
+    check_parity(a, b)
+
+
 def test_parity_long_R_many_blocks():
     # 12 blocks on 192-point R pencils: the R pass holds 2 rows of 16 blocks
     # per workgroup (the c3 path holds 4 rows of 5)
