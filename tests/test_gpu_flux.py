@@ -159,3 +159,20 @@ def test_mixed_crp_error_c4_shape():
     ok = eta >= 1e-2
     _say(f"  max |dP| {err[ok].max():.3e} over {int(ok.sum())} energies with eta >= 1e-2 max")
     assert err[ok].max() <= 1e-6
+
+
+def test_reaction_probability_within_bounds_at_dt_5():
+    """0 <= P(E) <= 1 for the 128x64x48 J=0 F + H2 job propagated with dt = 5 au
+    over 20000 au (4000 steps; the norm left on the grid is 2e-5), wherever the
+    initial packet carries energy (eta_sq >= 1e-3 of its maximum).  At dt = 10 au
+    the split-operator step error alone moves P(E) by up to 0.1 and past 1 at
+    some energies, so the bound is checked at the smaller step."""
+    spec = inputs.config_spec("c1", dt=5.0, total_steps=4000, n_energies=24, energy_range=(0.02, 0.09))
+    job = inputs.build_job(spec)
+    _say("case flux bounds 128x64x48 J=0 dt=5 steps=4000 energies=24")
+    b = abi.Inputs(job, want_acf=False, want_observables=False, final_copy=False)
+    api.evolve(b)
+    ok = _resolved(job, 1e-3)
+    _say(f"  P over {int(ok.sum())} energies: min {b.crp[ok].min():.4f} max {b.crp[ok].max():.4f}")
+    assert b.crp[ok].min() >= 0.0
+    assert b.crp[ok].max() <= 1.0
