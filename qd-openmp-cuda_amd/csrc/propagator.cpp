@@ -498,18 +498,29 @@ void DeviceJob::drain(int records_total) {
   const int nO = shape.nO;
   if (records_total <= records_done) return;
   QD_REQUIRE(records_total - records_done <= dev.ring, "observable ring overflow");
+  const int n = records_total - records_done;
   obs.resize((size_t)records_total * nO * 5);
   acf.resize((size_t)records_total * 2);
-  for (int r = records_done; r < records_total; r++) {
+  std::vector<double2> part((size_t)n * nO);
+  // the new records fill ring slots from records_done % ring on: at most two
+  // contiguous spans, one copy each per ring (not one per record)
+  for (int r = records_done; r < records_total;) {
     const int slot = r % dev.ring;
+    const int len = std::min(records_total - r, dev.ring - slot);
     QD_HIP(hipMemcpy(obs.data() + (size_t)r * nO * 5, dev.obs_ring + (size_t)slot * nO * 5,
-                     sizeof(double) * nO * 5, hipMemcpyDeviceToHost));
-    std::vector<double2> part(nO);
-    QD_HIP(hipMemcpy(part.data(), dev.acf_ring + (size_t)slot * nO, sizeof(double2) * nO, hipMemcpyDeviceToHost));
+                     sizeof(double) * nO * 5 * len, hipMemcpyDeviceToHost));
+    QD_HIP(hipMemcpy(part.data() + (size_t)(r - records_done) * nO, dev.acf_ring + (size_t)slot * nO,
+                     sizeof(double2) * nO * len, hipMemcpyDeviceToHost));
+    r += len;
+  }
+  for (int q = 0; q < n; q++) {
     double re = 0.0, im = 0.0;
-    for (int b = 0; b < nO; b++) { re += part[b].x; im += part[b].y; }
-    acf[(size_t)r * 2] = re;
-    acf[(size_t)r * 2 + 1] = im;
+    for (int b = 0; b < nO; b++) {
+      re += part[(size_t)q * nO + b].x;
+      im += part[(size_t)q * nO + b].y;
+    }
+    acf[(size_t)(records_done + q) * 2] = re;
+    acf[(size_t)(records_done + q) * 2 + 1] = im;
   }
   records_done = records_total;
 }

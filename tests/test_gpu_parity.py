@@ -219,6 +219,33 @@ def test_resume_equals_single_run():
     assert e <= 1e-13
 
 
+def test_observable_ring_wraps():
+    """Per-step records read back across the end of the 1024-slot device ring.
+    A read at step 100 and a run on to 1100 leave records 100..1099 in slots
+    100..1023 and 0..75; they equal those of one uninterrupted run, bit for bit,
+    and the uninterrupted run's own norm and energies match the oracle."""
+    spec = inputs.JobSpec(n1=64, n2=32, ntheta=16, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
+                          J=2, parity=0, k_max=-1, j0=1, E_coll=0.02)
+    job = inputs.build_job(spec)
+    _say("case ring wrap 64x32x16 J=2 steps=1100")
+    one = abi.Inputs(job, total_steps=1100, final_copy=False)
+    api.evolve(one)
+    two = abi.Inputs(job, total_steps=1100, final_copy=False)
+    with api.Propagator(two) as p:
+        p.run(100)
+        head = p.observables()
+        p.run(1000)
+        obs = p.observables()
+        acf = p.autocorrelation()
+    assert np.array_equal(head, one.obs[:100])
+    assert np.array_equal(obs, one.obs)
+    assert np.array_equal(acf, one.acf)
+    ref = abi.Inputs(job, total_steps=40, final_copy=False)
+    oracle.evolve(ref, 8)
+    assert np.abs(one.obs[:40] - ref.obs).max() <= OBS_TOL * max(1.0, float(np.abs(ref.obs).max()))
+    assert np.abs(one.acf[:40] - ref.acf).max() <= PSI_TOL
+
+
 def test_fp64_bitwise_reproducible():
     spec = inputs.JobSpec(n1=64, n2=32, ntheta=24, J=2, parity=0, k_max=-1, j0=1, E_coll=0.02,
                           R_range=(2.0, 12.0), r_range=(0.5, 5.0))
