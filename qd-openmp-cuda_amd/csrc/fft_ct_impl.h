@@ -203,21 +203,8 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
   }();
   QD_G double2 *base = jb.fbr + (long)l * M + (long)g * PN;
   const QD_G double2 *c0 = ACF ? jb.c0 + (long)l * M + (long)g * PN : nullptr;
-  for (int e = threadIdx.x; e < N1; e += kBlock) tw[e] = jb.twR[e];
-  if constexpr (VLDS) {
-    for (int e = threadIdx.x; e < jb.nO * jb.nO; e += kBlock) Vs[e] = jb.corV[(long)l * jb.nO * jb.nO + e];
-    for (int e = threadIdx.x; e < jb.nO; e += kBlock) cms[e] = jb.cmat[(long)l * jb.nO + e];
-  }
-  {
-    const QD_G double2 *cp = jb.corPhi + (long)l * N1 * jb.nO;
-    for (int e = threadIdx.x; e < N1 * NB; e += kBlock) {
-      const int i = e / NB, a2 = e % NB;
-      const double2 v = cp[i * jb.nO + min(a2, jb.nO - 1)];
-      phs[e] = a2 < jb.nO ? v : make_double2(0.0, 0.0);
-    }
-  }
-  __syncthreads();
-  // every global load of the slab tile in flight at once; loads are
+  // every global load of the slab tile in flight at once, issued before the
+  // table staging below so that the round trips overlap; loads are
   // unconditional from in-bounds addresses (block nb-1, last point), and
   // values of blocks >= nb and points >= PN are never used
   double2 x[NB][XPT];
@@ -225,6 +212,35 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
   for (int b = 0; b < NB; b++)
 #pragma unroll
     for (int k = 0; k < XPT; k++) x[b][k] = base[(b < nb ? b : nb - 1) * LM + (eok(k) ? ept(k) : 0)];
+  {
+    // the twiddles and the slab's phases: every load first (clamped in-range
+    // indices), then the LDS writes, so the staging costs one round trip
+    // rather than one per loop trip
+    constexpr int NTW = (N1 + kBlock - 1) / kBlock;
+    constexpr int NPH = (N1 * NB + kBlock - 1) / kBlock;
+    const QD_G double2 *cp = jb.corPhi + (long)l * N1 * jb.nO;
+    double2 tv[NTW], pv[NPH];
+#pragma unroll
+    for (int it = 0; it < NTW; it++) tv[it] = jb.twR[min((int)threadIdx.x + it * kBlock, N1 - 1)];
+#pragma unroll
+    for (int it = 0; it < NPH; it++) {
+      const int e = min((int)threadIdx.x + it * kBlock, N1 * NB - 1);
+      pv[it] = cp[(e / NB) * jb.nO + min(e % NB, jb.nO - 1)];
+    }
+    if constexpr (VLDS) {
+      for (int e = threadIdx.x; e < jb.nO * jb.nO; e += kBlock) Vs[e] = jb.corV[(long)l * jb.nO * jb.nO + e];
+      for (int e = threadIdx.x; e < jb.nO; e += kBlock) cms[e] = jb.cmat[(long)l * jb.nO + e];
+    }
+#pragma unroll
+    for (int it = 0; it < NTW; it++)
+      if ((int)threadIdx.x + it * kBlock < N1) tw[threadIdx.x + it * kBlock] = tv[it];
+#pragma unroll
+    for (int it = 0; it < NPH; it++) {
+      const int e = threadIdx.x + it * kBlock;
+      if (e < N1 * NB) phs[e] = e % NB < jb.nO ? pv[it] : make_double2(0.0, 0.0);
+    }
+  }
+  __syncthreads();
   FbrSums<NB, NF> sm;
   sm.zero();
   if (!INV) {
@@ -414,7 +430,12 @@ __global__ __launch_bounds__(kBlock) void k_fft_r_ct(StepArgs a) {
   int b, l, g;
   decode_slab_unit(jb, u, n1 / Q, b, l, g);
   QD_G double2 *base = jb.fbr + ((long)b * jb.L + l) * jb.M + (long)g * Q;   // (p, j) at base[j n1 + p]
-  for (int e = threadIdx.x; e < N2; e += kBlock) tw[e] = jb.twr[e];
+  // twiddles loaded here and written to LDS after the pencil loads below, so
+  // that both round trips overlap
+  constexpr int NTW = (N2 + kBlock - 1) / kBlock;
+  double2 tv[NTW];
+#pragma unroll
+  for (int it = 0; it < NTW; it++) tv[it] = jb.twr[min((int)threadIdx.x + it * kBlock, N2 - 1)];
   double2 X[MB][R1];
   // forward first stage (stride 1) on points read from HBM
 #pragma unroll
@@ -430,6 +451,9 @@ __global__ __launch_bounds__(kBlock) void k_fft_r_ct(StepArgs a) {
       for (int q = 0; q < R1; q++) A[p + (idx * R1 + q) * Q] = X[m][q];
     }
   }
+#pragma unroll
+  for (int it = 0; it < NTW; it++)
+    if ((int)threadIdx.x + it * kBlock < N2) tw[threadIdx.x + it * kBlock] = tv[it];
   __syncthreads();
   stages_ip<false, N2, R1, Q, 1, Q, true, PlanEnds, NSL>(A, tw);
   // forward last stage (stride NSL, twiddles w^(q idx)) into registers:
