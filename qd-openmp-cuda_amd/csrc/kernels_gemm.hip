@@ -95,6 +95,33 @@ __device__ __forceinline__ double buffer_load_f64(__amdgpu_buffer_rsrc_t r, uint
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, 0, 0));
 }
 
+// B ([kp][np] row-major, global) into LDS rows of stride Ws, columns np..W-1
+// zeroed.  Each thread issues up to 8 loads before its first LDS write, so the
+// staging costs one round trip per 8 elements per thread rather than one per
+// element.
+template <int TB, int W, typename T, typename Src>
+__device__ __forceinline__ void stage_b(T *Bs, Src Bg, int kp, int np, int Ws) {
+  constexpr int CH = 8;
+  const int total = kp * W;
+  for (int e0 = 0; e0 < total; e0 += CH * TB) {
+    T v[CH];
+#pragma unroll
+    for (int q = 0; q < CH; q++) {
+      const int e = e0 + q * TB + (int)threadIdx.x;
+      const int k = e / W, c = e - k * W;
+      v[q] = (e < total && c < np) ? Bg[k * np + c] : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < CH; q++) {
+      const int e = e0 + q * TB + (int)threadIdx.x;
+      if (e < total) {
+        const int k = e / W;
+        Bs[k * Ws + (e - k * W)] = v[q];
+      }
+    }
+  }
+}
+
 // Value of the neighbouring lane (lane ^ 1) through a DPP quad permutation
 // (1, 0, 3, 2): a VALU move, no LDS round trip.
 __device__ __forceinline__ double swap_adjacent(double x) {
@@ -331,19 +358,10 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg(StepArgs a) {
   double *Bs = lds_d;
   float *Bh = (float *)lds_d, *Bl = Bh + kp * Ws;
   if (!MIXED) {
-    const double *Bg = (FWD ? jb.wP : jb.Pt) + (long)b * kp * np;
-    for (int e = threadIdx.x; e < kp * W; e += TB) {
-      const int k = e / W, c = e % W;
-      Bs[k * Ws + c] = c < np ? Bg[k * np + c] : 0.0;
-    }
+    stage_b<TB, W>(Bs, (FWD ? jb.wP : jb.Pt) + (long)b * kp * np, kp, np, Ws);
   } else {
-    const float *Bgh = (FWD ? jb.wP_hi : jb.Pt_hi) + (long)b * kp * np;
-    const float *Bgl = (FWD ? jb.wP_lo : jb.Pt_lo) + (long)b * kp * np;
-    for (int e = threadIdx.x; e < kp * W; e += TB) {
-      const int k = e / W, c = e % W;
-      Bh[k * Ws + c] = c < np ? Bgh[k * np + c] : 0.f;
-      Bl[k * Ws + c] = c < np ? Bgl[k * np + c] : 0.f;
-    }
+    stage_b<TB, W>(Bh, (FWD ? jb.wP_hi : jb.Pt_hi) + (long)b * kp * np, kp, np, Ws);
+    stage_b<TB, W>(Bl, (FWD ? jb.wP_lo : jb.Pt_lo) + (long)b * kp * np, kp, np, Ws);
   }
   __syncthreads();
   GemmWave gw;
@@ -448,10 +466,7 @@ __global__ __launch_bounds__(kBlock) void k_gemm_probe(const double *A, int lda,
                                                        double *C, int ldc, int rows, int K, int N) {
   extern __shared__ double Bs[];
   constexpr int Ws = lds_stride(NT), W = NT * 16;
-  for (int e = threadIdx.x; e < kp * W; e += kBlock) {
-    const int k = e / W, c = e % W;
-    Bs[k * Ws + c] = c < np ? Bg[k * np + c] : 0.0;
-  }
+  stage_b<kBlock, W>(Bs, Bg, kp, np, Ws);
   __syncthreads();
   const int r0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
   GemmWave gw{};
