@@ -153,8 +153,12 @@ __device__ __forceinline__ int acc_col(int t, int lk, int it) {
 //   PHASE: multiply each A element by d1 d2 exp(-i dt V) (forward prologue);
 //          a lane holds one real component, its partner lane (r ^ 1) the other.
 //   OBS:   accumulate w V |psi|^2 of the output (backward epilogue) into ob.
-template <int NT, int kCh, int DEPTH, bool MIXED, bool PHASE, bool OBS>
-__device__ __forceinline__ void gemm_wave(const GemmWave &gw, long strip0, int nstrips, int sstride, double &ob) {
+// `stage` (the workgroup's staging of B into LDS and its barrier) runs once on
+// every path, after the first tiles' loads are issued, so that both round
+// trips overlap.
+template <int NT, int kCh, int DEPTH, bool MIXED, bool PHASE, bool OBS, typename Stage>
+__device__ __forceinline__ void gemm_wave(const GemmWave &gw, long strip0, int nstrips, int sstride, double &ob,
+                                          Stage &&stage) {
   const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
   const bool odd = lr & 1;
   const int nch = (gw.ksteps + kCh - 1) / kCh;
@@ -162,7 +166,10 @@ __device__ __forceinline__ void gemm_wave(const GemmWave &gw, long strip0, int n
   for (int s = 0; s < nstrips; s++)
     if ((strip0 + (long)s * sstride) * 16 < gw.rows) ns = s + 1;
   const int ntiles = ns * nch;
-  if (ntiles <= 0) return;
+  if (ntiles <= 0) {
+    stage();
+    return;
+  }
   double abuf[DEPTH][kCh], tbuf[DEPTH][kCh];
   auto row_of = [&](int tile) { return (strip0 + (long)(tile / nch) * sstride) * 16 + lr; };
   // operand rows k < K only: rows K.. of the padded product read as zero
@@ -190,8 +197,8 @@ __device__ __forceinline__ void gemm_wave(const GemmWave &gw, long strip0, int n
         if (PHASE) tp[q] = buffer_load_f64(rT, v0 + q * kstride);   // even lane Re T, odd lane Im T
       }
     } else {
-      // a uniform row base per k (an SGPR pair) plus one 32-bit lane offset:
-      // the loads take the scalar-base form and share one address VGPR.
+      // a uniform row base per k plus one 32-bit lane offset (the compiler
+      // still forms most addresses as 64-bit vector pairs).
       // k-steps past the last (when kCh does not divide ksteps) reread the
       // last one; rows K <= k < 4 ksteps are allocated (zero padding past
       // the last block) and masked where they are used.
@@ -317,6 +324,7 @@ __device__ __forceinline__ void gemm_wave(const GemmWave &gw, long strip0, int n
   };
 #pragma unroll
   for (int d = 0; d < DEPTH - 1; d++) load(d, abuf[d], tbuf[d]);
+  stage();
   for (int t0 = 0; t0 < ntiles; t0 += DEPTH) {
 #pragma unroll
     for (int d = 0; d < DEPTH; d++) {
@@ -357,13 +365,15 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg(StepArgs a) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   double *Bs = lds_d;
   float *Bh = (float *)lds_d, *Bl = Bh + kp * Ws;
-  if (!MIXED) {
-    stage_b<TB, W>(Bs, (FWD ? jb.wP : jb.Pt) + (long)b * kp * np, kp, np, Ws);
-  } else {
-    stage_b<TB, W>(Bh, (FWD ? jb.wP_hi : jb.Pt_hi) + (long)b * kp * np, kp, np, Ws);
-    stage_b<TB, W>(Bl, (FWD ? jb.wP_lo : jb.Pt_lo) + (long)b * kp * np, kp, np, Ws);
-  }
-  __syncthreads();
+  auto stage = [&]() {
+    if (!MIXED) {
+      stage_b<TB, W>(Bs, (FWD ? jb.wP : jb.Pt) + (long)b * kp * np, kp, np, Ws);
+    } else {
+      stage_b<TB, W>(Bh, (FWD ? jb.wP_hi : jb.Pt_hi) + (long)b * kp * np, kp, np, Ws);
+      stage_b<TB, W>(Bl, (FWD ? jb.wP_lo : jb.Pt_lo) + (long)b * kp * np, kp, np, Ws);
+    }
+    __syncthreads();
+  };
   GemmWave gw;
   gw.A = FWD ? (const double *)(jb.psi + b * G) : (const double *)(jb.fbr + slab0);
   gw.C = FWD ? (double *)(jb.fbr + slab0) : (double *)(jb.psi + b * G);
@@ -378,7 +388,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg(StepArgs a) {
   double ob = 0.0;
   const int S = FWD ? jb.gemm_strips : jb.gemm_bwd_strips;
   gemm_wave<NT, KC, (KC == 8 && (NT <= 4 || (NT <= 6 && !MIXED))) ? 3 : 2, MIXED, FWD, !FWD>(
-      gw, (long)g * S * WAVES + wave, S, WAVES, ob);
+      gw, (long)g * S * WAVES + wave, S, WAVES, ob, stage);
   // first kernel of the step: apply the previous step's counter advance (at
   // the end, so the store cannot hold back the descriptor loads above)
   if (FWD && u == 0 && threadIdx.x == 0) advance_counters(jb);
@@ -466,15 +476,16 @@ __global__ __launch_bounds__(kBlock) void k_gemm_probe(const double *A, int lda,
                                                        double *C, int ldc, int rows, int K, int N) {
   extern __shared__ double Bs[];
   constexpr int Ws = lds_stride(NT), W = NT * 16;
-  stage_b<kBlock, W>(Bs, Bg, kp, np, Ws);
-  __syncthreads();
   const int r0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
   GemmWave gw{};
   gw.A = A; gw.lda = lda; gw.K = K; gw.ksteps = kp / 4;
   gw.C = C; gw.ldc = ldc; gw.N = N; gw.rows = rows;
   gw.Bs = Bs; gw.Ws = Ws;
   double ob = 0.0;
-  gemm_wave<NT, 8, 2, false, false, false>(gw, r0 / 16, 1, 1, ob);
+  gemm_wave<NT, 8, 2, false, false, false>(gw, r0 / 16, 1, 1, ob, [&]() {
+    stage_b<kBlock, W>(Bs, Bg, kp, np, Ws);
+    __syncthreads();
+  });
 }
 
 void launch_gemm_probe(const double *A, int lda, const double *B, int kp, int np, double *C, int ldc, int rows,
