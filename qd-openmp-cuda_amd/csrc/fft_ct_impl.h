@@ -358,35 +358,56 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
       }
     }
     // second stage, block by block: y_b = rot2(l, j) sum_a V_l[a][b] t_a,
-    // stored; rotational energy l(l+1)(I1 + I2)|y|^2, |y|^2 and conj(c0) y
+    // stored; rotational energy l(l+1)(I1 + I2)|y|^2, |y|^2 and conj(c0) y.
+    // Everything read from the job descriptor or the per-point tables is
+    // loaded before the first store: as far as the compiler knows the stores
+    // may alias the descriptor, and it would reload its fields (and then the
+    // tables through them) after every store, two dependent round trips per
+    // block.
     const double ll = (double)l * (l + 1);
+    const int nO = jb.nO;
+    const QD_G double2 *r2p = jb.rot2 + (long)l * jb.n2;
+    QD_G double *pcor = jb.part_cor;
+    const long ustride = jb.units[KK_FFT_RI];
+    double2 rv[XPT];
+    double iv[XPT];
+#pragma unroll
+    for (int k = 0; k < XPT; k++) {
+      const int e = eok(k) ? ept(k) : 0;
+      const int i = e % N1, j = g * P + e / N1;
+      rv[k] = r2p[j];
+      iv[k] = ll * (jb.I1[i] + jb.I2[j]);
+    }
 #pragma unroll
     for (int b = 0; b < NB; b++) {
       double sb[NF - 1];
 #pragma unroll
       for (int f = 0; f < NF - 1; f++) sb[f] = 0.0;
       if (b < nb) {
+        double2 z[ACF ? XPT : 1];
+        if constexpr (ACF) {   // this block's c0 values ahead of its stores
+#pragma unroll
+          for (int k = 0; k < XPT; k++) z[k] = c0[b * LM + (eok(k) ? ept(k) : 0)];
+        }
 #pragma unroll
         for (int k = 0; k < XPT; k++) {
           const int e = ept(k);
           if (eok(k)) {
-            const int i = e % N1, j = g * P + e / N1;
             double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
             for (int al = 0; al < NB; al++) {
-              const double v = V[min(al, nOc) * jb.nO + b];
+              const double v = V[min(al, nOc) * nO + b];
               acc.x = fma(v, x[al][k].x, acc.x);
               acc.y = fma(v, x[al][k].y, acc.y);
             }
-            const double2 y = cmul(jb.rot2[(long)l * jb.n2 + j], acc);
+            const double2 y = cmul(rv[k], acc);
             base[b * LM + e] = y;
             const double q = cnorm(y);
-            sb[0] += ll * (jb.I1[i] + jb.I2[j]) * q;
+            sb[0] += iv[k] * q;
             sb[1] += q;
-            if constexpr (NF == 5) {
-              const double2 z = c0[b * LM + e];
-              sb[2] += z.x * y.x + z.y * y.y;
-              sb[3] += z.x * y.y - z.y * y.x;
+            if constexpr (ACF) {
+              sb[2] += z[k].x * y.x + z[k].y * y.y;
+              sb[3] += z[k].x * y.y - z[k].y * y.x;
             }
           }
         }
@@ -395,10 +416,10 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
       for (int f = 0; f < NF - 1; f++) wave_sum_to(sb[f], red + wave * NV + b * NF + 1 + f);
     }
     __syncthreads();
-    if (threadIdx.x < NF * jb.nO) {
+    if (threadIdx.x < NF * nO) {
       const int q = threadIdx.x;   // q = NF b + f, stored at field f of block b ([b][5][units])
       const double sum = (q < NV) ? (red[q] + red[NV + q]) + (red[2 * NV + q] + red[3 * NV + q]) : 0.0;
-      jb.part_cor[(long)(5 * (q / NF) + q % NF) * jb.units[KK_FFT_RI] + u] = sum;
+      pcor[(long)(5 * (q / NF) + q % NF) * ustride + u] = sum;
     }
   }
 }
