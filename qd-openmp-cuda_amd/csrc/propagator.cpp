@@ -331,15 +331,7 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
     }
     return w;
   };
-  // potential phase with the absorber folded in (evolutionCUDAaux.cu:5-11, :243-252)
-  std::vector<double2> vph(G);
   std::vector<double> pot(in->pot, in->pot + G);
-  for (long idx = 0; idx < G; idx++) {
-    const int i = idx % n1, j = (idx / n1) % n2;
-    const double dd = sh.dump ? in->dump1.dump[i] * in->dump2.dump[j] : 1.0;
-    const double2 p = phase(-dt * pot[idx]);
-    vph[idx] = make_double2(dd * p.x, dd * p.y);
-  }
   // Legendre tables (omegawavepacket.cu:134-181), padded row-major [k][c]
   std::vector<double> wP((size_t)nO * d.fwd_kp * d.fwd_np, 0.0), Pt((size_t)nO * d.bwd_kp * d.bwd_np, 0.0);
   const double *Pin = in->omega_states.associated_legendres;
@@ -390,8 +382,19 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   d.fbr = (double2 *)alloc(16 * (size_t)M * (L * nO + fbr_pad));
   QD_HIP(hipMemsetAsync(d.fbr, 0, 16 * (size_t)M * (L * nO + fbr_pad), st));
   d.c0 = nullptr;
-  d.vphase = upload(vph, st);
   d.pot = upload(pot, st);
+  {
+    // potential phase with the absorber folded in, built on the device from pot
+    const double *dd1 = nullptr, *dd2 = nullptr;
+    if (sh.dump) {
+      dd1 = upload(std::vector<double>(in->dump1.dump, in->dump1.dump + n1), st);
+      dd2 = upload(std::vector<double>(in->dump2.dump, in->dump2.dump + n2), st);
+    }
+    double2 *vp = (double2 *)alloc(16 * (size_t)G);
+    launch_vphase(vp, d.pot, dd1, dd2, n1, n2, (long)G, dt, st);
+    QD_HIP(hipGetLastError());
+    d.vphase = vp;
+  }
   d.w = upload(std::vector<double>(in->theta.w, in->theta.w + nt), st);
   {
     // quadrature-weighted potential of the potential-energy epilogue,
