@@ -222,8 +222,10 @@ qd_status qd_profile_steps(qd_handle *h, int32_t n_steps, double *us);
  * per process).  A server keeps many partial-wave jobs resident on the current
  * device and advances every running job by one step per qd_server_step:
  *   - qd_server_submit validates a job and queues it (nothing is allocated);
- *   - each qd_server_step first admits queued jobs whose estimated device
- *     footprint fits the byte budget (backfilling past jobs that do not fit),
+ *   - each qd_server_step first admits up to four queued jobs whose
+ *     estimated device footprint fits the byte budget (backfilling past jobs
+ *     that do not fit; a few per step, so that the running classes keep the
+ *     GPU busy while the host builds the next jobs),
  *     then runs one step of every running job (jobs of one grid shape and
  *     precision share each kernel launch and a HIP graph; shape classes run on
  *     their own streams), then retires jobs that reached time.total_steps:

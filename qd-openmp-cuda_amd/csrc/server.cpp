@@ -59,6 +59,11 @@ struct ShapeClass {
 };
 
 constexpr int kDrainEvery = 512;   // records per job between ring drains (ring holds 1024)
+// Jobs admitted per server step.  Building a job (allocation, tables, the
+// wavepacket upload: up to 0.7 GB for 9 blocks of 256 x 192 x 96) is host
+// work; admitting a few per step lets the classes already running keep the
+// GPU busy meanwhile, instead of a burst of arrivals holding it idle.
+constexpr int kAdmitPerStep = 4;
 
 }  // namespace
 
@@ -99,9 +104,11 @@ struct qd_server {
   }
 
   void admit() {
+    int admitted = 0;
     for (auto it = queue.begin(); it != queue.end();) {
       ServerJob *sj = *it;
       if (max_running > 0 && running() >= max_running) break;
+      if (admitted >= kAdmitPerStep) break;
       if (used + sj->estimate > budget) {   // backfill: a later, smaller job may fit
         ++it;
         continue;
@@ -122,6 +129,7 @@ struct qd_server {
       used += sj->charged;
       sj->state = QD_JOB_RUNNING;
       sj->t_admit = now_s();
+      admitted++;
       ShapeClass &c = class_for(*sj->job);
       c.members.push_back(sj);
       c.dirty = true;

@@ -96,6 +96,25 @@ def test_serve_same_shape_different_gemm_workgroups():
         _same(alone, mixed)
 
 
+def test_serve_admits_a_few_jobs_per_step():
+    # six jobs submitted at once, budget for all: each step admits at most
+    # four, so the first step leaves two queued; results are those alone
+    spec = _spec(64, 32, 24, 2, 0, -1, 10)
+    _say("case serve admission per step: 6 jobs of 64x32x24 J=2, 10 steps")
+    pairs = [_pair(spec) for _ in range(6)]
+    with api.Server() as srv:
+        ids = [srv.submit(m) for _, m in pairs]
+        nr, nq = srv.step()
+        assert (nr, nq) == (4, 2)
+        for _ in range(100):
+            nr, nq = srv.step()
+            if nr == 0 and nq == 0:
+                break
+        assert all(srv.status(i)["state"] == api.Server.DONE for i in ids)
+    for alone, mixed in pairs:
+        _same(alone, mixed)
+
+
 def test_serve_admission_by_memory_budget():
     spec = _spec(64, 32, 24, 5, 1, 3, 12)
     _say("case serve budget: 3 jobs of 64x32x24 J=5, budget for one")
