@@ -84,6 +84,22 @@ void launch_vphase(double2 *vph, const double *pot, const double *d1, const doub
   hipLaunchKernelGGL(k_vphase, dim3((unsigned)blocks), dim3(kBlock), 0, stream, vph, pot, d1, d2, n1, n2, G, dt);
 }
 
+// Quadrature-weighted potential of the potential-energy epilogue:
+// wpot[k M + m] = w_k V(k, m) for k < nt, zero rows up to `rows`.
+__global__ __launch_bounds__(kBlock) void k_wpot(double *wpot, const double *pot, const double *w, int nt, long M,
+                                                 long total) {
+  for (long idx = (long)blockIdx.x * kBlock + threadIdx.x; idx < total; idx += (long)gridDim.x * kBlock) {
+    const long k = idx / M;
+    wpot[idx] = k < nt ? w[k] * pot[idx] : 0.0;
+  }
+}
+
+void launch_wpot(double *wpot, const double *pot, const double *w, int nt, long M, int rows, hipStream_t stream) {
+  const long total = (long)rows * M;
+  const long blocks = std::min<long>((total + kBlock - 1) / kBlock, 4096);
+  hipLaunchKernelGGL(k_wpot, dim3((unsigned)blocks), dim3(kBlock), 0, stream, wpot, pot, w, nt, M, total);
+}
+
 // ---------------------------------------------------------------------------
 // Final reduction.  Workgroup (b, f) of a job sums all partials of field f of
 // omega block b in a fixed order (thread t takes elements t, t + 256, ... in

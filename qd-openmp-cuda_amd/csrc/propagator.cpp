@@ -331,7 +331,6 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
     }
     return w;
   };
-  std::vector<double> pot(in->pot, in->pot + G);
   // Legendre tables (omegawavepacket.cu:134-181), padded row-major [k][c]
   std::vector<double> wP((size_t)nO * d.fwd_kp * d.fwd_np, 0.0), Pt((size_t)nO * d.bwd_kp * d.bwd_np, 0.0);
   const double *Pin = in->omega_states.associated_legendres;
@@ -382,7 +381,11 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   d.fbr = (double2 *)alloc(16 * (size_t)M * (L * nO + fbr_pad));
   QD_HIP(hipMemsetAsync(d.fbr, 0, 16 * (size_t)M * (L * nO + fbr_pad), st));
   d.c0 = nullptr;
-  d.pot = upload(pot, st);
+  {
+    double *pp = (double *)alloc(sizeof(double) * (size_t)G);
+    QD_HIP(hipMemcpyAsync(pp, in->pot, sizeof(double) * (size_t)G, hipMemcpyHostToDevice, st));
+    d.pot = pp;
+  }
   {
     // potential phase with the absorber folded in, built on the device from pot
     const double *dd1 = nullptr, *dd2 = nullptr;
@@ -399,10 +402,10 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   {
     // quadrature-weighted potential of the potential-energy epilogue,
     // w_k V(k, m), padded with zero rows to the bwd_np output columns
-    std::vector<double> wpot((size_t)d.bwd_np * M, 0.0);
-    for (int k = 0; k < nt; k++)
-      for (int m = 0; m < M; m++) wpot[(size_t)k * M + m] = in->theta.w[k] * pot[(size_t)k * M + m];
-    d.wpot = upload(wpot, st);
+    double *wp = (double *)alloc(sizeof(double) * (size_t)d.bwd_np * M);
+    launch_wpot(wp, d.pot, d.w, nt, (long)M, d.bwd_np, st);
+    QD_HIP(hipGetLastError());
+    d.wpot = wp;
   }
   d.wP = upload(wP, st);
   d.Pt = upload(Pt, st);
@@ -463,7 +466,7 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
       for (int k = 0; k < nt; k++)
         for (int i = 0; i < n1; i++) {
           const long idx = (long)k * M + (long)(d.flux_ns - half + g) * n1 + i;
-          rph[((size_t)g * nt + k) * n1 + i] = phase(-0.5 * dt * pot[idx]);
+          rph[((size_t)g * nt + k) * n1 + i] = phase(-0.5 * dt * in->pot[idx]);
         }
     d.flux_coef = upload(first_derivative_weights(ng, sh.dr2), st);
     d.flux_rph = upload(rph, st);

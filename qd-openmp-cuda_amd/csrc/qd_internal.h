@@ -185,6 +185,7 @@ __device__ __forceinline__ void advance_counters(const JobDev &jb) {
 // Host-side launchers (each enqueues one kernel on `stream`).
 // step-0 phase exp(+i dt/2 V) on every block of one job (before its first step)
 void launch_prephase(double2 *psi, const double *pot, long G, int nO, double dt, hipStream_t stream);
+void launch_wpot(double *wpot, const double *pot, const double *w, int nt, long M, int rows, hipStream_t stream);
 void launch_vphase(double2 *vph, const double *pot, const double *d1, const double *d2, int n1, int n2, long G,
                    double dt, hipStream_t stream);
 // column tile counts (N / 16) with a Legendre GEMM instantiation
