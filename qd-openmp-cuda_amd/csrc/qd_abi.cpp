@@ -209,7 +209,7 @@ qd_status qd_get_observables(qd_handle *h, int32_t first, int32_t count, double 
     if (first < 0 || count < 0 || first + count > j.records_done)
       return fail(h, QD_ERR_INVALID, "observable records out of range");
     const size_t rec = (size_t)j.shape.nO * 5;
-    std::memcpy(dst, j.obs.data() + (size_t)first * rec, sizeof(double) * rec * count);
+    if (count > 0) std::memcpy(dst, j.obs.data() + (size_t)first * rec, sizeof(double) * rec * count);
     return QD_OK;
   })
 }
@@ -223,7 +223,7 @@ qd_status qd_get_autocorrelation(qd_handle *h, int32_t first, int32_t count, dou
     if (!j.shape.acf) return fail(h, QD_ERR_STATE, "autocorrelation was not requested at create");
     if (first < 0 || count < 0 || first + count > j.records_done)
       return fail(h, QD_ERR_INVALID, "autocorrelation records out of range");
-    std::memcpy(dst, j.acf.data() + (size_t)first * 2, sizeof(double) * 2 * count);
+    if (count > 0) std::memcpy(dst, j.acf.data() + (size_t)first * 2, sizeof(double) * 2 * count);
     return QD_OK;
   })
 }

@@ -167,8 +167,9 @@ struct qd_server {
     qd_inputs *in = sj->in;
     *in->time.steps = j.steps;
     const size_t rec = (size_t)j.shape.nO * 5;
-    if (in->observables) std::memcpy(in->observables, j.obs.data(), sizeof(double) * rec * sj->target);
-    if (in->autocorrelation && j.shape.acf)
+    if (in->observables && sj->target > 0)
+      std::memcpy(in->observables, j.obs.data(), sizeof(double) * rec * sj->target);
+    if (in->autocorrelation && j.shape.acf && sj->target > 0)
       std::memcpy(in->autocorrelation, j.acf.data(), sizeof(double) * 2 * sj->target);
     if (in->final_copy) j.copy_psi(in->omega_states.wave_packets, s);
     if (j.shape.crp) j.flux_to_crp(in->crp.CRP, s);

@@ -139,6 +139,37 @@ def test_parity_theta_not_multiple_of_4():
     check_parity(a, b)
 
 
+def test_zero_steps_returns_the_inputs():
+    # total_steps = 0: nothing runs (no step-0 phase either); the wavepackets
+    # come back as given, time.steps stays 0 and there are no records
+    spec = inputs.JobSpec(n1=64, n2=32, ntheta=24, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
+                          J=3, parity=1, k_max=-1, omega0=1, j0=1, E_coll=0.02)
+    job = inputs.build_job(spec)
+    _say("case zero steps 64x32x24 J=3")
+    b = abi.Inputs(job, total_steps=0)
+    api.evolve(b)
+    assert b.steps[0] == 0
+    assert np.array_equal(b.psi, job["omega_states"]["wave_packets"])
+    assert b.obs.shape[0] == 0 and b.acf.shape[0] == 0
+
+
+def test_parity_max_blocks_24():
+    # 24 omega blocks, the most a job may have (J = 23, p = 1: omega 0..23)
+    spec = inputs.JobSpec(n1=48, n2=32, ntheta=28, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
+                          J=23, parity=1, k_max=-1, omega0=0, j0=2, E_coll=0.02)
+    a, b = run_both(spec, 2)
+    assert len(a.omegas) == 24
+    check_parity(a, b)
+
+
+def test_parity_runtime_R_length_512():
+    # 512-point R pencils: beyond the compile-time lengths, the runtime-size R pass
+    spec = inputs.JobSpec(n1=512, n2=16, ntheta=8, R_range=(2.0, 14.0), r_range=(0.5, 5.0),
+                          J=1, parity=1, k_max=-1, omega0=0, j0=1, E_coll=0.02)
+    a, b = run_both(spec, 2)
+    check_parity(a, b)
+
+
 def test_parity_long_R_many_blocks():
     # 12 blocks on 192-point R pencils: the R pass holds 2 rows of 16 blocks
     # per workgroup (the c3 path holds 4 rows of 5)
