@@ -194,9 +194,9 @@ def main():
                     help="only run a plain copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs one partial-wave job set on one GPU; "
-                                   "--gpus N > 1 is not supported (jobs shard across GPUs by job, "
-                                   "see DESIGN.md)"}))
+        print(json.dumps({"error": "one GPU only: --gpus N > 1 is not supported. Partial-wave jobs are "
+                                   "independent, so a multi-GPU deployment runs one server per GPU and "
+                                   "deals jobs out by memory footprint"}))
         sys.exit(2)
     _load_package()
     from qd_openmp_cuda_amd import abi, api, inputs

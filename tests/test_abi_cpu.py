@@ -1,6 +1,6 @@
 """C-ABI checks that need no GPU: the library loads, exports every symbol
 declared in include/*.h, rejects invalid inputs before touching the device
-(the reference's insist() conditions, SURVEY of cudaMath.h:119,
+(the reference's insist() conditions, the odd-n case of cudaMath.h:119,
 omegawavepacket.cu:46/:142, CoriolisMatrix.C:65, cudaOpenMP2.cu:437-440),
 and its host-side Coriolis eigensolver agrees with the oracle's."""
 import ctypes as C
