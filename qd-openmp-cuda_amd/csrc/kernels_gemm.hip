@@ -337,6 +337,195 @@ __device__ __forceinline__ void gemm_wave(const GemmWave &gw, long strip0, int n
   }
 }
 
+typedef double v2d __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ v2d buffer_load_v2d(__amdgpu_buffer_rsrc_t r, uint32_t voff) {
+  return __builtin_bit_cast(v2d, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, 0, 0));
+}
+
+// Forward Legendre product with paired rows (FP64): lane (lr, lk) loads the
+// complex point p0 + lr of k-step row 4 c + lk as one 16-byte value (its re
+// and im rows), and its phase the same way, so the phase multiply stays in
+// the lane.  The re rows of a strip of 16 points feed one MFMA and the im
+// rows a second, both with the same B fragment from LDS; the two
+// accumulators hold re and im of the same outputs, stored as 16 bytes.
+// Loads and masking as in gemm_wave (buffer range checks zero rows k >= K).
+template <int NT, int kCh, int DEPTH, typename Stage>
+__device__ __forceinline__ void gemm_wave_fwd_paired(const GemmWave &gw, long strip0, int nstrips, int sstride,
+                                                     Stage &&stage) {
+  const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+  const long npts = gw.rows / 2;
+  const int nch = (gw.ksteps + kCh - 1) / kCh;
+  int ns = 0;
+  for (int s = 0; s < nstrips; s++)
+    if ((strip0 + (long)s * sstride) * 16 < npts) ns = s + 1;
+  const int ntiles = ns * nch;
+  if (ntiles <= 0) {
+    stage();
+    return;
+  }
+  v2d abuf[DEPTH][kCh], tbuf[DEPTH][kCh];
+  auto pt_of = [&](int tile) { return (strip0 + (long)(tile / nch) * sstride) * 16 + lr; };
+  const uint32_t abytes = (uint32_t)((unsigned long)gw.K * (unsigned long)gw.lda * 8ul);
+  const __amdgpu_buffer_rsrc_t rA = make_rsrc(gw.A, abytes);
+  const __amdgpu_buffer_rsrc_t rT = make_rsrc((const void *)gw.vph, abytes);
+  const uint32_t kstride = (uint32_t)gw.lda * 32u;   // bytes between k and k + 4
+  auto load = [&](int tile, v2d(&a)[kCh], v2d(&tp)[kCh]) {
+    tile = min(tile, ntiles - 1);
+    const int c = tile % nch;
+    const long p = pt_of(tile);
+    const long pc = p < npts ? p : npts - 1;   // points past the end reread the last one
+    const uint32_t v0 = ((uint32_t)(4 * c * kCh + lk) * (uint32_t)gw.lda + 2u * (uint32_t)pc) * 8u;
+#pragma unroll
+    for (int q = 0; q < kCh; q++) {
+      a[q] = buffer_load_v2d(rA, v0 + q * kstride);
+      tp[q] = buffer_load_v2d(rT, v0 + q * kstride);
+    }
+  };
+  v4d accr[NT], acci[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) accr[t] = acci[t] = v4d{0.0, 0.0, 0.0, 0.0};
+  auto compute = [&](int tile, const v2d(&an)[kCh], const v2d(&tn)[kCh]) {
+    const int c = tile % nch;
+    const long p = pt_of(tile);
+    if (c == 0) {
+#pragma unroll
+      for (int t = 0; t < NT; t++) accr[t] = acci[t] = v4d{0.0, 0.0, 0.0, 0.0};
+    }
+#pragma unroll
+    for (int q = 0; q < kCh; q++) {
+      if (c * kCh + q < gw.ksteps) {
+        const v2d av = an[q], tv = tn[q];
+        const double ar = fma(-av.y, tv.y, av.x * tv.x);
+        const double ai = fma(av.x, tv.y, av.y * tv.x);
+        const double *brow = gw.Bs + (4 * (c * kCh + q) + lk) * gw.Ws + lr;
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          const double bv = brow[t * 16];
+          accr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, ar, accr[t], 0, 0, 0);
+          acci[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, ai, acci[t], 0, 0, 0);
+        }
+      }
+    }
+    if (c == nch - 1) {   // epilogue of the strip: streaming 16-byte stores
+      const bool pok = p < npts;
+#pragma unroll
+      for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int it = 0; it < 4; it++) {
+          const int col = acc_col<false>(t, lk, it);
+          if (pok && col < gw.N)
+            __builtin_nontemporal_store(v2d{accr[t][it], acci[t][it]}, (v2d *)(gw.C + (long)col * gw.ldc + 2 * p));
+        }
+    }
+  };
+#pragma unroll
+  for (int d = 0; d < DEPTH - 1; d++) load(d, abuf[d], tbuf[d]);
+  stage();
+  for (int t0 = 0; t0 < ntiles; t0 += DEPTH) {
+#pragma unroll
+    for (int d = 0; d < DEPTH; d++) {
+      const int tile = t0 + d;
+      load(tile + DEPTH - 1, abuf[(d + DEPTH - 1) % DEPTH], tbuf[(d + DEPTH - 1) % DEPTH]);
+      if (tile < ntiles) compute(tile, abuf[d], tbuf[d]);
+    }
+  }
+}
+
+// Backward Legendre product with paired rows (FP64): as gemm_wave_fwd_paired
+// without the phase, plus the potential-energy epilogue: each lane owns whole
+// points, so it loads the w_k V values of its point (4 NT per strip, issued
+// before the prefetch) and adds w V (re^2 + im^2) of every output into ob.
+template <int NT, int kCh, int DEPTH, typename Stage>
+__device__ __forceinline__ void gemm_wave_bwd_paired(const GemmWave &gw, long strip0, int nstrips, int sstride,
+                                                     double &ob, Stage &&stage) {
+  const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+  const long npts = gw.rows / 2;
+  const int nch = (gw.ksteps + kCh - 1) / kCh;
+  int ns = 0;
+  for (int s = 0; s < nstrips; s++)
+    if ((strip0 + (long)s * sstride) * 16 < npts) ns = s + 1;
+  const int ntiles = ns * nch;
+  if (ntiles <= 0) {
+    stage();
+    return;
+  }
+  v2d abuf[DEPTH][kCh];
+  auto pt_of = [&](int tile) { return (strip0 + (long)(tile / nch) * sstride) * 16 + lr; };
+  // operand rows k < K only: rows K.. of the padded product read as zero
+  const uint32_t abytes = (uint32_t)((unsigned long)gw.K * (unsigned long)gw.lda * 8ul);
+  const __amdgpu_buffer_rsrc_t rA = make_rsrc(gw.A, abytes);
+  const uint32_t kstride = (uint32_t)gw.lda * 32u;
+  auto load = [&](int tile, v2d(&a)[kCh]) {
+    tile = min(tile, ntiles - 1);
+    const int c = tile % nch;
+    const long p = pt_of(tile);
+    const long pc = p < npts ? p : npts - 1;
+    const uint32_t v0 = ((uint32_t)(4 * c * kCh + lk) * (uint32_t)gw.lda + 2u * (uint32_t)pc) * 8u;
+#pragma unroll
+    for (int q = 0; q < kCh; q++) a[q] = buffer_load_v2d(rA, v0 + q * kstride);
+  };
+  // w_k V of this lane's point for its 4 NT output columns (the table holds
+  // NT * 16 theta rows, zero past N: no clamp on the column)
+  auto load_pot = [&](int tile, double(&vh)[NT][4]) {
+    const long p = pt_of(tile);
+    const long pc = p < npts ? p : npts - 1;
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+      for (int it = 0; it < 4; it++) vh[t][it] = gw.wpot[(long)acc_col<false>(t, lk, it) * gw.M + pc];
+  };
+  v4d accr[NT], acci[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) accr[t] = acci[t] = v4d{0.0, 0.0, 0.0, 0.0};
+  auto compute = [&](int tile, const v2d(&an)[kCh], const double(&vh)[NT][4]) {
+    const int c = tile % nch;
+    const long p = pt_of(tile);
+    if (c == 0) {
+#pragma unroll
+      for (int t = 0; t < NT; t++) accr[t] = acci[t] = v4d{0.0, 0.0, 0.0, 0.0};
+    }
+#pragma unroll
+    for (int q = 0; q < kCh; q++) {
+      if (c * kCh + q < gw.ksteps) {
+        const v2d av = an[q];
+        const double *brow = gw.Bs + (4 * (c * kCh + q) + lk) * gw.Ws + lr;
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          const double bv = brow[t * 16];
+          accr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, av.x, accr[t], 0, 0, 0);
+          acci[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, av.y, acci[t], 0, 0, 0);
+        }
+      }
+    }
+    if (c == nch - 1) {
+      const bool pok = p < npts;
+#pragma unroll
+      for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int it = 0; it < 4; it++) {
+          const int col = acc_col<false>(t, lk, it);
+          const double re = accr[t][it], im = acci[t][it];
+          if (pok && col < gw.N)
+            __builtin_nontemporal_store(v2d{re, im}, (v2d *)(gw.C + (long)col * gw.ldc + 2 * p));
+          ob += (pok ? re * re + im * im : 0.0) * vh[t][it];   // w V = 0 for col >= N
+        }
+    }
+  };
+#pragma unroll
+  for (int d = 0; d < DEPTH - 1; d++) load(d, abuf[d]);
+  stage();
+  for (int t0 = 0; t0 < ntiles; t0 += DEPTH) {
+#pragma unroll
+    for (int d = 0; d < DEPTH; d++) {
+      const int tile = t0 + d;
+      double vh[NT][4];
+      if (tile < ntiles && tile % nch == nch - 1) load_pot(tile, vh);
+      load(tile + DEPTH - 1, abuf[(d + DEPTH - 1) % DEPTH]);
+      if (tile < ntiles) compute(tile, abuf[d], vh);
+    }
+  }
+}
+
 // WAVES waves per workgroup share one LDS copy of B: 8 where B is too large
 // for two 4-wave workgroups per CU.
 template <int NT, int KC, bool FWD, bool MIXED, int WAVES>
@@ -404,6 +593,121 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg(StepArgs a) {
   }
 }
 
+// Forward Legendre pass on paired rows (strips of 16 complex points).
+template <int NT, int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void k_leg_fwd_paired(StepArgs a) {
+  constexpr int TB = WAVES * 64;
+  extern __shared__ double lds_d[];
+  const int lu = xcd_logical_unit(blockIdx.x, gridDim.x);
+  const int job = locate_job(a.unit_base, a.n_jobs, lu);
+  const JobDev &jb = a.jobs[job];
+  const int u = lu - a.unit_base[job];
+  const int b = u % jb.nO, g = u / jb.nO;
+  const int om = jb.omin + b, nl = jb.lmax - om + 1;
+  const long M = jb.M;
+  const long G = M * jb.nt;
+  const long slab0 = ((long)b * jb.L + om) * M;
+  const int kp = jb.fwd_kp, np = jb.fwd_np;
+  constexpr int Ws = lds_stride(NT);
+  constexpr int W = NT * 16;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  double *Bs = lds_d;
+  auto stage = [&]() {
+    stage_b<TB, W>(Bs, jb.wP + (long)b * kp * np, kp, np, Ws);
+    __syncthreads();
+  };
+  GemmWave gw;
+  gw.A = (const double *)(jb.psi + b * G);
+  gw.C = (double *)(jb.fbr + slab0);
+  gw.rows = 2 * M;
+  gw.lda = gw.ldc = 2 * M;
+  gw.K = jb.nt;
+  gw.N = nl;
+  gw.ksteps = kp / 4;
+  gw.Bs = Bs; gw.Bh = nullptr; gw.Bl = nullptr; gw.Ws = Ws;
+  gw.vph = (const QD_G double *)jb.vphase;
+  gw.wpot = jb.wpot; gw.M = M;
+  const int S = jb.gemm_strips;
+  gemm_wave_fwd_paired<NT, 4, 3>(gw, (long)g * S * WAVES + wave, S, WAVES, stage);
+  // first kernel of the step: apply the previous step's counter advance
+  if (u == 0 && threadIdx.x == 0) advance_counters(jb);
+}
+
+// Backward Legendre pass on paired rows, with the potential-energy partials.
+template <int NT, int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void k_leg_bwd_paired(StepArgs a) {
+  constexpr int TB = WAVES * 64;
+  extern __shared__ double lds_d[];
+  __shared__ double red[WAVES];
+  const int lu = xcd_logical_unit(blockIdx.x, gridDim.x);
+  const int job = locate_job(a.unit_base, a.n_jobs, lu);
+  const JobDev &jb = a.jobs[job];
+  const int u = lu - a.unit_base[job];
+  const int groups = jb.gemm_bwd_groups;
+  const int b = u % jb.nO, g = u / jb.nO;
+  const int om = jb.omin + b, nl = jb.lmax - om + 1;
+  const long M = jb.M;
+  const long G = M * jb.nt;
+  const long slab0 = ((long)b * jb.L + om) * M;
+  const int kp = jb.bwd_kp, np = jb.bwd_np;
+  constexpr int Ws = lds_stride(NT);
+  constexpr int W = NT * 16;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  double *Bs = lds_d;
+  auto stage = [&]() {
+    stage_b<TB, W>(Bs, jb.Pt + (long)b * kp * np, kp, np, Ws);
+    __syncthreads();
+  };
+  GemmWave gw;
+  gw.A = (const double *)(jb.fbr + slab0);
+  gw.C = (double *)(jb.psi + b * G);
+  gw.rows = 2 * M;
+  gw.lda = gw.ldc = 2 * M;
+  gw.K = nl;
+  gw.N = jb.nt;
+  gw.ksteps = kp / 4;
+  gw.Bs = Bs; gw.Bh = nullptr; gw.Bl = nullptr; gw.Ws = Ws;
+  gw.vph = (const QD_G double *)jb.vphase;
+  gw.wpot = jb.wpot; gw.M = M;
+  double ob = 0.0;
+  const int S = jb.gemm_bwd_strips;
+  gemm_wave_bwd_paired<NT, 4, 3>(gw, (long)g * S * WAVES + wave, S, WAVES, ob, stage);
+  ob = wave_sum64(ob);
+  if (lane == 0) red[wave] = ob;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sum = (red[0] + red[1]) + (red[2] + red[3]);
+    if constexpr (WAVES == 8) sum += (red[4] + red[5]) + (red[6] + red[7]);
+    jb.part_obs[(long)b * groups + g] = sum;
+  }
+}
+
+template <int NT>
+static void launch_leg_bwd_paired_nt(const StepArgs &a, int total_units, int kp_max, int waves, hipStream_t s) {
+  const size_t lds = (size_t)kp_max * lds_stride(NT) * sizeof(double);
+  if (waves == 8) hipLaunchKernelGGL((k_leg_bwd_paired<NT, 8>), dim3(total_units), dim3(512), lds, s, a);
+  else hipLaunchKernelGGL((k_leg_bwd_paired<NT, 4>), dim3(total_units), dim3(256), lds, s, a);
+}
+
+// FP64 Legendre passes on paired rows (forward up to 6 column tiles, backward
+// up to 4); QD_LEG_PAIRED=0 selects the one-row-per-lane kernels.  Read once
+// per process, so every job agrees.  On the J=20 workload the paired kernels
+// take 64 / 57 us against 68 / 66 us.
+bool leg_paired() {
+  static const bool v = [] {
+    const char *e = std::getenv("QD_LEG_PAIRED");
+    return !(e && std::atoi(e) == 0);
+  }();
+  return v;
+}
+
+template <int NT>
+static void launch_leg_fwd_paired_nt(const StepArgs &a, int total_units, int kp_max, int waves, hipStream_t s) {
+  const size_t lds = (size_t)kp_max * lds_stride(NT) * sizeof(double);
+  if (waves == 8) hipLaunchKernelGGL((k_leg_fwd_paired<NT, 8>), dim3(total_units), dim3(512), lds, s, a);
+  else hipLaunchKernelGGL((k_leg_fwd_paired<NT, 4>), dim3(total_units), dim3(256), lds, s, a);
+}
+
 template <int NT, int KC, bool FWD, int WAVES>
 static void launch_leg_k(const StepArgs &a, int total_units, int kp_max, int mixed, hipStream_t s) {
   const size_t lds = (size_t)kp_max * lds_stride(NT) * (mixed ? 2 * sizeof(float) : sizeof(double));
@@ -463,10 +767,30 @@ bool leg_tiles_supported(int nt_tiles) {
 
 void launch_leg_fwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, int waves,
                     hipStream_t s) {
+  if (!mixed && leg_paired() && nt_tiles <= 6) {
+    switch (nt_tiles) {
+      case 1: launch_leg_fwd_paired_nt<1>(a, total_units, kp_max, waves, s); return;
+      case 2: launch_leg_fwd_paired_nt<2>(a, total_units, kp_max, waves, s); return;
+      case 3: launch_leg_fwd_paired_nt<3>(a, total_units, kp_max, waves, s); return;
+      case 4: launch_leg_fwd_paired_nt<4>(a, total_units, kp_max, waves, s); return;
+      case 5: launch_leg_fwd_paired_nt<5>(a, total_units, kp_max, waves, s); return;
+      case 6: launch_leg_fwd_paired_nt<6>(a, total_units, kp_max, waves, s); return;
+      default: break;
+    }
+  }
   launch_leg<true>(a, total_units, nt_tiles, kp_max, mixed, waves, s);
 }
 void launch_leg_bwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, int waves,
                     hipStream_t s) {
+  if (!mixed && leg_paired() && nt_tiles <= 4) {   // 5 and 6 tiles spill in 8-wave workgroups
+    switch (nt_tiles) {
+      case 1: launch_leg_bwd_paired_nt<1>(a, total_units, kp_max, waves, s); return;
+      case 2: launch_leg_bwd_paired_nt<2>(a, total_units, kp_max, waves, s); return;
+      case 3: launch_leg_bwd_paired_nt<3>(a, total_units, kp_max, waves, s); return;
+      case 4: launch_leg_bwd_paired_nt<4>(a, total_units, kp_max, waves, s); return;
+      default: break;
+    }
+  }
   launch_leg<false>(a, total_units, nt_tiles, kp_max, mixed, waves, s);
 }
 
@@ -529,12 +853,24 @@ static void set_leg_attrs(int lim) {
   }
 }
 
+template <int NT>
+static void set_leg_paired_attrs(int lim) {
+  (void)hipFuncSetAttribute((const void *)k_leg_fwd_paired<NT, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+  (void)hipFuncSetAttribute((const void *)k_leg_fwd_paired<NT, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+  if constexpr (NT <= 4) {
+    (void)hipFuncSetAttribute((const void *)k_leg_bwd_paired<NT, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void *)k_leg_bwd_paired<NT, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+  }
+}
+
 void init_attrs_gemm() {
   const int lim = 160 * 1024;
   set_leg_attrs<1>(lim); set_leg_attrs<2>(lim); set_leg_attrs<3>(lim); set_leg_attrs<4>(lim);
   set_leg_attrs<5>(lim); set_leg_attrs<6>(lim); set_leg_attrs<7>(lim); set_leg_attrs<8>(lim);
   set_leg_attrs<10>(lim); set_leg_attrs<12>(lim); set_leg_attrs<15>(lim);
   (void)hipFuncSetAttribute((const void *)k_gemm_probe<8>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+  set_leg_paired_attrs<1>(lim); set_leg_paired_attrs<2>(lim); set_leg_paired_attrs<3>(lim);
+  set_leg_paired_attrs<4>(lim); set_leg_paired_attrs<5>(lim); set_leg_paired_attrs<6>(lim);
 }
 
 }  // namespace qd

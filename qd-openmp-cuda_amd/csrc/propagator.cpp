@@ -256,28 +256,34 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   // strips per wave make the workgroups of all blocks fit one round of the
   // CUs (2 four-wave or 1 eight-wave workgroup per CU): each wave then runs
   // many pipelined tiles after paying the B staging and pipeline fill once.
-  auto pick_waves = [&](int kp, int np) {
+  // the paired-row passes take strips of 16 complex points (32 rows)
+  const bool fwd_paired = !sh.mixed && leg_paired() && d.fwd_np / 16 <= 6;
+  const bool bwd_paired = !sh.mixed && leg_paired() && d.bwd_np / 16 <= 4;
+  const long fwd_strips = fwd_paired ? ((long)M + 15) / 16 : strips;
+  const long bwd_strips = bwd_paired ? ((long)M + 15) / 16 : strips;
+  auto pick_waves = [&](int kp, int np, long nstrips) {
     if (np / 16 > (sh.mixed ? 5 : 8)) return 4;
     if (gemm_lds_bytes(kp, np, sh.mixed) > kLegLdsTwoGroups) return 8;
-    return strips * nO >= 8L * kCUs ? 8 : 4;
+    return nstrips * nO >= 8L * kCUs ? 8 : 4;
   };
-  d.gemm_fwd_waves = pick_waves(d.fwd_kp, d.fwd_np);
-  d.gemm_bwd_waves = pick_waves(d.bwd_kp, d.bwd_np);
+  d.gemm_fwd_waves = pick_waves(d.fwd_kp, d.fwd_np, fwd_strips);
+  d.gemm_bwd_waves = pick_waves(d.bwd_kp, d.bwd_np, bwd_strips);
   if (const char *e = std::getenv("QD_LEG_WAVES")) {   // tuning sweeps: 4 or 8 for both GEMMs
     const int w = std::atoi(e);
     if (w == 4 || w == 8) d.gemm_fwd_waves = d.gemm_bwd_waves = w;
   }
-  auto strips_per_wave = [&](int waves) {
+  auto strips_per_wave = [&](int waves, long nstrips) {
     const long slots = waves == 8 ? kCUs : 2L * kCUs;
-    long S = std::max(1L, (strips * nO + waves * slots - 1) / (waves * slots));
-    while (nO * ((strips + waves * S - 1) / (waves * S)) > slots) S++;
+    long S = std::max(1L, (nstrips * nO + waves * slots - 1) / (waves * slots));
+    while (nO * ((nstrips + waves * S - 1) / (waves * S)) > slots) S++;
     return (int)S;
   };
-  d.gemm_strips = strips_per_wave(d.gemm_fwd_waves);
-  d.gemm_bwd_strips = strips_per_wave(d.gemm_bwd_waves);
-  d.gemm_fwd_groups = (int)((strips + (long)d.gemm_fwd_waves * d.gemm_strips - 1) / ((long)d.gemm_fwd_waves * d.gemm_strips));
-  d.gemm_bwd_groups =
-      (int)((strips + (long)d.gemm_bwd_waves * d.gemm_bwd_strips - 1) / ((long)d.gemm_bwd_waves * d.gemm_bwd_strips));
+  d.gemm_strips = strips_per_wave(d.gemm_fwd_waves, fwd_strips);
+  d.gemm_bwd_strips = strips_per_wave(d.gemm_bwd_waves, bwd_strips);
+  d.gemm_fwd_groups = (int)((fwd_strips + (long)d.gemm_fwd_waves * d.gemm_strips - 1) /
+                            ((long)d.gemm_fwd_waves * d.gemm_strips));
+  d.gemm_bwd_groups = (int)((bwd_strips + (long)d.gemm_bwd_waves * d.gemm_bwd_strips - 1) /
+                            ((long)d.gemm_bwd_waves * d.gemm_bwd_strips));
   static const int rows_c[] = {8, 4, 2, 1};
   static const int cols_c[] = {16, 8, 4, 2, 1};
   // compile-time FFT kernels where the lengths allow; otherwise the runtime-size

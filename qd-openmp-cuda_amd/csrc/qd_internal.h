@@ -50,7 +50,8 @@ struct JobDev {
   int J;
   int mixed;                 // 1: mixed-precision Legendre GEMMs
   // work decomposition
-  int gemm_strips, gemm_bwd_strips;   // 16-row strips per wave in the forward / backward Legendre GEMM
+  int gemm_strips, gemm_bwd_strips;   // strips per wave in the forward / backward Legendre GEMM
+                                      // (16 rows, or 16 complex points in the paired-row passes)
   int gemm_fwd_waves, gemm_bwd_waves;   // waves per GEMM workgroup (4, or 8 to share a large B in LDS)
   int gemm_fwd_groups;       // workgroups per omega block (forward GEMM)
   int gemm_bwd_groups;       // workgroups per omega block (backward GEMM)
@@ -190,6 +191,8 @@ void launch_vphase(double2 *vph, const double *pot, const double *d1, const doub
                    double dt, hipStream_t stream);
 // column tile counts (N / 16) with a Legendre GEMM instantiation
 bool leg_tiles_supported(int nt_tiles);
+// FP64 Legendre passes on paired rows (default; QD_LEG_PAIRED=0 turns them off)
+bool leg_paired();
 void launch_leg_fwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, int waves,
                     hipStream_t stream);
 void launch_leg_bwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, int waves,

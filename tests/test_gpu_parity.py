@@ -365,3 +365,30 @@ def test_command_line_driver_matches_library(tmp_path):
     assert np.array_equal(got["psi"], ref.psi)
     assert np.array_equal(got["obs"], ref.obs)
     assert np.array_equal(got["crp"], ref.crp)
+
+
+def test_unpaired_legendre_kernels_match_oracle(tmp_path):
+    """QD_LEG_PAIRED=0 (read once per process, so in a child: the C++ driver)
+    selects the one-row-per-lane Legendre kernels for FP64 jobs; on the c3
+    block structure they match the oracle to the same bars as the default
+    paired-row kernels."""
+    import os
+    import subprocess
+    spec = dataclasses.replace(inputs.config_spec("c3"), n1=96, n2=64, ntheta=32)
+    job = inputs.build_job(spec)
+    steps = 3
+    fx, out = tmp_path / "job.bin", tmp_path / "out.bin"
+    inputs.write_fixture(job, str(fx), total_steps=steps)
+    drv = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "qd-openmp-cuda_amd",
+                       "qd_fixture_run")
+    _say("case unpaired Legendre kernels 96x64x32 J=20 K_max=4, 3 steps (child process)")
+    env = dict(os.environ, QD_LEG_PAIRED="0")
+    r = subprocess.run([drv, str(fx), str(out)], capture_output=True, text=True, timeout=120, env=env)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = inputs.read_fixture_output(str(out), job, steps)
+    ref = abi.Inputs(job, total_steps=steps, want_acf=False)
+    oracle.evolve(ref, n_threads=8)
+    e_psi = rel_l2(got["psi"], ref.psi)
+    e_obs = float(np.abs(got["obs"] - ref.obs).max()) / max(1.0, float(np.abs(ref.obs).max()))
+    _say(f"  psi rel {e_psi:.3e}  obs {e_obs:.3e}")
+    assert e_psi <= PSI_TOL and e_obs <= OBS_TOL
