@@ -689,10 +689,11 @@ static void launch_leg_bwd_paired_nt(const StepArgs &a, int total_units, int kp_
   else hipLaunchKernelGGL((k_leg_bwd_paired<NT, 4>), dim3(total_units), dim3(256), lds, s, a);
 }
 
-// FP64 Legendre passes on paired rows (forward up to 6 column tiles, backward
-// up to 4); QD_LEG_PAIRED=0 selects the one-row-per-lane kernels.  Read once
-// per process, so every job agrees.  On the J=20 workload the paired kernels
-// take 64 / 57 us against 68 / 66 us.
+// FP64 Legendre passes on paired rows (up to 4 column tiles: wider ones spill,
+// or with strips of 16 points fill fewer CUs than the one-row kernels);
+// QD_LEG_PAIRED=0 selects the one-row-per-lane kernels.  Read once per
+// process, so every job agrees.  On the J=20 workload the paired kernels take
+// 63 / 57 us against 68 / 66 us.
 bool leg_paired() {
   static const bool v = [] {
     const char *e = std::getenv("QD_LEG_PAIRED");
@@ -767,14 +768,12 @@ bool leg_tiles_supported(int nt_tiles) {
 
 void launch_leg_fwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, int waves,
                     hipStream_t s) {
-  if (!mixed && leg_paired() && nt_tiles <= 6) {
+  if (!mixed && leg_paired() && nt_tiles <= 4) {
     switch (nt_tiles) {
       case 1: launch_leg_fwd_paired_nt<1>(a, total_units, kp_max, waves, s); return;
       case 2: launch_leg_fwd_paired_nt<2>(a, total_units, kp_max, waves, s); return;
       case 3: launch_leg_fwd_paired_nt<3>(a, total_units, kp_max, waves, s); return;
       case 4: launch_leg_fwd_paired_nt<4>(a, total_units, kp_max, waves, s); return;
-      case 5: launch_leg_fwd_paired_nt<5>(a, total_units, kp_max, waves, s); return;
-      case 6: launch_leg_fwd_paired_nt<6>(a, total_units, kp_max, waves, s); return;
       default: break;
     }
   }
@@ -857,10 +856,8 @@ template <int NT>
 static void set_leg_paired_attrs(int lim) {
   (void)hipFuncSetAttribute((const void *)k_leg_fwd_paired<NT, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
   (void)hipFuncSetAttribute((const void *)k_leg_fwd_paired<NT, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
-  if constexpr (NT <= 4) {
-    (void)hipFuncSetAttribute((const void *)k_leg_bwd_paired<NT, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
-    (void)hipFuncSetAttribute((const void *)k_leg_bwd_paired<NT, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
-  }
+  (void)hipFuncSetAttribute((const void *)k_leg_bwd_paired<NT, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+  (void)hipFuncSetAttribute((const void *)k_leg_bwd_paired<NT, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
 }
 
 void init_attrs_gemm() {
@@ -870,7 +867,7 @@ void init_attrs_gemm() {
   set_leg_attrs<10>(lim); set_leg_attrs<12>(lim); set_leg_attrs<15>(lim);
   (void)hipFuncSetAttribute((const void *)k_gemm_probe<8>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
   set_leg_paired_attrs<1>(lim); set_leg_paired_attrs<2>(lim); set_leg_paired_attrs<3>(lim);
-  set_leg_paired_attrs<4>(lim); set_leg_paired_attrs<5>(lim); set_leg_paired_attrs<6>(lim);
+  set_leg_paired_attrs<4>(lim);
 }
 
 }  // namespace qd

@@ -257,7 +257,7 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   // CUs (2 four-wave or 1 eight-wave workgroup per CU): each wave then runs
   // many pipelined tiles after paying the B staging and pipeline fill once.
   // the paired-row passes take strips of 16 complex points (32 rows)
-  const bool fwd_paired = !sh.mixed && leg_paired() && d.fwd_np / 16 <= 6;
+  const bool fwd_paired = !sh.mixed && leg_paired() && d.fwd_np / 16 <= 4;
   const bool bwd_paired = !sh.mixed && leg_paired() && d.bwd_np / 16 <= 4;
   const long fwd_strips = fwd_paired ? ((long)M + 15) / 16 : strips;
   const long bwd_strips = bwd_paired ? ((long)M + 15) / 16 : strips;
