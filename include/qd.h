@@ -167,9 +167,14 @@ typedef struct {
   int32_t print_energies; /* print per-step lines in the reference's format */
   int32_t final_copy;     /* qd_evolve: copy the final state into wave_packets */
   double *observables;    /* qd_evolve: optional out [total_steps][n_omegas][QD_OBS_FIELDS] */
-  double *autocorrelation;/* optional out, complex <psi(0)|psi(t_n)> per step; when NULL at
-                             create the autocorrelation is not computed (saves one read
-                             of the initial state per step) */
+  double *autocorrelation;/* optional out, complex <psi(t_0)|psi(t_n)> per step n, summed over
+                             the omega blocks with the quadrature weights and dr1 dr2.
+                             psi(t_n) = exp(-i dt/2 V) psi_stored(n), the stored state with
+                             the leapfrogged V(dt/2) restored, at t_n = (steps_n + 1) dt;
+                             psi(t_0) is the caller's wave_packets on a fresh start
+                             (steps == 0) and exp(-i dt/2 V) wave_packets on a resume.
+                             When NULL at create the autocorrelation is not computed
+                             (saves one read of the initial coefficients per step) */
 } qd_inputs;
 
 typedef struct qd_handle qd_handle;
@@ -188,7 +193,8 @@ qd_status qd_copy_wavepackets(qd_handle *h, double *dst);
 /* Per-step observables recorded so far: dst[rec][omega][QD_OBS_FIELDS],
  * records first..first+count-1 (record 0 = first step run by this handle). */
 qd_status qd_get_observables(qd_handle *h, int32_t first, int32_t count, double *dst);
-/* Complex autocorrelation <psi(0)|psi(t_n)> for records first..first+count-1. */
+/* Complex autocorrelation <psi(t_0)|psi(t_n)> (defined at qd_inputs.autocorrelation)
+ * for records first..first+count-1. */
 qd_status qd_get_autocorrelation(qd_handle *h, int32_t first, int32_t count, double *dst);
 /* Reaction probabilities at the CRP energies from the flux accumulated so far. */
 qd_status qd_get_crp(qd_handle *h, double *crp);

@@ -33,7 +33,9 @@
  * quadrature orthonormality; the whole step against free-particle and
  * rigid-rotor closed forms, time reversibility, norm and energy conservation,
  * the n = 2 Coriolis closed form and the J = 0 identity; flux against the
- * Eckart barrier transmission.
+ * Eckart barrier transmission; the autocorrelation against the free-particle
+ * closed form <psi_0|IFFT2(exp(-i t T) FFT2 psi_0)>, against the phase
+ * exp(-i V0 t) that a constant potential V0 adds, and across a resume.
  */
 #include <algorithm>
 #include <cmath>
@@ -188,7 +190,7 @@ struct Job {
   std::vector<double> pot, w, kin1, kin2, I1, I2, d1, d2;
   bool dump = false;
   std::vector<CorMat> cor;                 // [l]
-  std::vector<cplx> psi0;                  // initial state, all blocks (autocorrelation)
+  std::vector<cplx> psi0;                  // state at the start time t_0, all blocks (autocorrelation)
 };
 
 static double kinetic_entry(int i, int n, double xl, double mass) {
@@ -275,9 +277,15 @@ static int setup_job(const qd_inputs *in, Job &jb, char *err) {
         jb.wP[b][(long)c * jb.nt + k] = jb.w[k] * pb[(long)c * jb.nt + k];
       }
   }
-  jb.psi0.resize(G * jb.nO);
-  for (int b = 0; b < jb.nO; b++) std::copy(jb.psi[b].begin(), jb.psi[b].end(), jb.psi0.begin() + b * G);
   jb.pot.assign(in->pot, in->pot + G);
+  // psi(t_0): the caller's array on a fresh start (steps == 0); on a resume
+  // the array is a stored state, which lacks the trailing V(dt/2) of the
+  // leapfrogged potential (cudaOpenMP2.cu:338-387), so it is restored here
+  jb.psi0.resize(G * jb.nO);
+  const bool resumed = *in->time.steps != 0;
+  for (int b = 0; b < jb.nO; b++)
+    for (long idx = 0; idx < G; idx++)
+      jb.psi0[b * G + idx] = resumed ? jb.psi[b][idx] * expi(-0.5 * jb.dt * jb.pot[idx]) : jb.psi[b][idx];
   jb.kin1.resize(jb.n1); jb.kin2.resize(jb.n2); jb.I1.resize(jb.n1); jb.I2.resize(jb.n2);
   for (int i = 0; i < jb.n1; i++) {
     jb.kin1[i] = kinetic_entry(i, jb.n1, jb.n1 * jb.dr1, jb.m1);
@@ -534,6 +542,8 @@ static void dvr_observables(Job &jb, double *norm, double *pe) {
   }
 }
 
+// C(t_n) = <psi(t_0)|psi(t_n)> = sum_b sum_k w_k sum_ij conj(psi0) psi dr1 dr2, with
+// psi(t_n) the stored state with the missing V(dt/2) restored (as for the flux).
 static cplx autocorrelation(Job &jb) {
   const int M = jb.M;
   const long G = (long)M * jb.nt;
@@ -544,7 +554,8 @@ static cplx autocorrelation(Job &jb) {
       cplx acc = 0.0;
       const cplx *s = jb.psi[b].data() + (long)k * M;
       const cplx *s0 = jb.psi0.data() + b * G + (long)k * M;
-      for (int m = 0; m < M; m++) acc += std::conj(s0[m]) * s[m];
+      const double *v = jb.pot.data() + (long)k * M;
+      for (int m = 0; m < M; m++) acc += std::conj(s0[m]) * (s[m] * expi(-0.5 * jb.dt * v[m]));
       sr += jb.w[k] * acc.real();
       si += jb.w[k] * acc.imag();
     }

@@ -7,6 +7,7 @@
 
 #include <limits>
 #include <cstdlib>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -85,7 +86,8 @@ class DeviceJob {
  private:
   std::vector<void *> allocs_;
   void *alloc(size_t bytes);
-  template <class T> T *upload(const std::vector<T> &v, hipStream_t s);
+  template <class T> T *upload(std::vector<T> v, hipStream_t s);
+  std::vector<std::shared_ptr<void>> staging_;   // host sources of the set-up copies
   int flux_window_start_ = 0;    // step counter at the first step of the open window
   double2 *flux_A_ = nullptr, *flux_B_ = nullptr, *flux_ph_ = nullptr;
   double *flux_E_ = nullptr, *flux_eta_ = nullptr, *flux_out_ = nullptr;

@@ -19,6 +19,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <numeric>
 
 #include "host_internal.h"
@@ -191,10 +192,16 @@ void *DeviceJob::alloc(size_t bytes) {
   return p;
 }
 
+// The host copy is kept in staging_ until the constructor's final stream
+// synchronisation: an asynchronous copy from pageable memory may still read
+// its source after the call returns.
 template <class T>
-T *DeviceJob::upload(const std::vector<T> &v, hipStream_t s) {
+T *DeviceJob::upload(std::vector<T> v, hipStream_t s) {
   T *p = (T *)alloc(v.size() * sizeof(T));
-  if (!v.empty()) QD_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  if (v.empty()) return p;
+  auto held = std::make_shared<std::vector<T>>(std::move(v));
+  staging_.push_back(held);
+  QD_HIP(hipMemcpyAsync(p, held->data(), held->size() * sizeof(T), hipMemcpyHostToDevice, s));
   return p;
 }
 
@@ -219,7 +226,7 @@ size_t DeviceJob::estimate_footprint(const JobShape &s, int n_energies) {
   size_t b = 0;
   b += 16 * (G + 4 * (size_t)s.M) * s.nO; // psi, with the GEMM padding
   b += 16 * (size_t)s.M * s.L * s.nO * (s.acf ? 1 : 0);   // FBR of the initial state
-  b += 16 * (size_t)s.M * (s.L + 4) * s.nO;   // fbr, with the GEMM padding
+  b += 16 * (size_t)s.M * std::max((s.L + 4) * s.nO, s.nt + 4);   // fbr, with the GEMM padding
   b += 24 * G;                            // vphase + pot
   b += 8 * (G + 16 * (size_t)s.M);        // weighted potential (padded to 16 theta rows)
   b += 16 * (size_t)s.L * s.n1 * s.nO * s.nO;
@@ -384,8 +391,10 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   d.psi = (double2 *)alloc(16 * (G * nO + psi_pad * M));
   QD_HIP(hipMemsetAsync(d.psi, 0, 16 * (G * nO + psi_pad * M), st));
   QD_HIP(hipMemcpyAsync(d.psi, in->omega_states.wave_packets, 16 * G * nO, hipMemcpyHostToDevice, st));
-  d.fbr = (double2 *)alloc(16 * (size_t)M * (L * nO + fbr_pad));
-  QD_HIP(hipMemsetAsync(d.fbr, 0, 16 * (size_t)M * (L * nO + fbr_pad), st));
+  // fbr also serves once as the staging block of the initial state's FBR
+  // coefficients below, so it holds at least fwd_kp slabs
+  const size_t fbr_slabs = std::max<size_t>((size_t)L * nO + fbr_pad, (size_t)d.fwd_kp);
+  d.fbr = (double2 *)alloc(16 * (size_t)M * fbr_slabs);
   d.c0 = nullptr;
   {
     double *pp = (double *)alloc(sizeof(double) * (size_t)G);
@@ -438,16 +447,29 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   d.part_ke = (double *)alloc(sizeof(double) * (size_t)d.units[KK_FFT_r]);
   d.part_obs = (double *)alloc(sizeof(double) * (size_t)nO * d.gemm_bwd_groups);
   if (sh.acf) {
-    // FBR coefficients of the initial state (psi still holds it): the
-    // autocorrelation <psi(0)|psi(t)> is formed on them in the FFT pass
+    // Autocorrelation <psi(t_0)|psi(t_n)>.  The state stored after a step
+    // lacks the trailing V(dt/2) of the leapfrogged potential
+    // (cudaOpenMP2.cu:338-387), so psi(t_n) = exp(-i dt/2 V) psi_stored(n),
+    // and on a fresh start psi(t_0) is the caller's psi_0; on a resume
+    // (steps != 0) the caller's array is itself a stored state, psi(t_0) =
+    // exp(-i dt/2 V) psi_in.  As V is real and diagonal,
+    //   <psi(t_0)|psi(t_n)> = <a|psi_stored(n)>,  a = psi_0 exp(+i dt/2 V) (fresh) or psi_in (resume),
+    // and since psi_stored(n) is a Legendre expansion (the output of the
+    // backward transform) the DVR inner product equals the FBR one on the
+    // forward-transformed a.  c0 holds those coefficients; the FFT pass
+    // forms the products.  Each block of a is staged in fbr (zeroed after).
     d.c0 = (double2 *)alloc(16 * (size_t)M * L * nO);
     for (int b = 0; b < nO; b++) {
       const int om = sh.omin + b, nl = sh.lmax - om + 1;
-      launch_gemm_probe((const double *)(d.psi + (size_t)b * G), 2 * M, d.wP + (size_t)b * d.fwd_kp * d.fwd_np,
-                        d.fwd_kp, d.fwd_np, (double *)(d.c0 + ((size_t)b * L + om) * M), 2 * M, 2 * M, nt, nl, st);
+      QD_HIP(hipMemcpyAsync(d.fbr, d.psi + (size_t)b * G, 16 * (size_t)G, hipMemcpyDeviceToDevice, st));
+      if (sh.steps0 == 0) launch_prephase(d.fbr, d.pot, G, 1, dt, st);
+      // rows nt..fwd_kp-1 of the staged operand are read and masked off
+      launch_gemm_probe((const double *)d.fbr, 2 * M, d.wP + (size_t)b * d.fwd_kp * d.fwd_np, d.fwd_kp, d.fwd_np,
+                        (double *)(d.c0 + ((size_t)b * L + om) * M), 2 * M, 2 * M, nt, nl, st);
       QD_HIP(hipGetLastError());
     }
   }
+  QD_HIP(hipMemsetAsync(d.fbr, 0, 16 * (size_t)M * fbr_slabs, st));
   d.ring = kRing;
   d.obs_ring = (double *)alloc(sizeof(double) * (size_t)kRing * nO * 5);
   d.acf_ring = (double2 *)alloc(sizeof(double2) * (size_t)kRing * nO);

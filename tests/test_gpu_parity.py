@@ -26,13 +26,26 @@ def _say(msg):
     sys.stdout.flush()
 
 
-def run_both(spec, steps, precision=abi.QD_PREC_FP64, start_steps=0):
+def _renormalised(job, target=0.99):
+    """The job with its wavepacket scaled to norm `target` on the grid (coarse
+    test grids under-resolve the analytic normalisation of the initial state)."""
+    psi = job["omega_states"]["wave_packets"]
+    w = job["theta"]["w"]
+    n = float(np.sum(np.abs(psi) ** 2 * w[None, None, :, None])) * job["r1"]["dr"] * job["r2"]["dr"]
+    out = dict(job)
+    out["omega_states"] = dict(job["omega_states"], wave_packets=np.asfortranarray(psi * np.sqrt(target / n)))
+    return out
+
+
+def run_both(spec, steps, precision=abi.QD_PREC_FP64, start_steps=0, threads=8, renormalise=False):
     job = inputs.build_job(spec)
+    if renormalise:
+        job = _renormalised(job)
     a = abi.Inputs(job, total_steps=steps, steps=start_steps)
     b = abi.Inputs(job, total_steps=steps, steps=start_steps, precision=precision)
-    _say(f"case {spec.n1}x{spec.n2}x{spec.ntheta} J={spec.J} p={spec.parity} k_max={spec.k_max} "
-         f"steps={steps} prec={precision}")
-    oracle.evolve(a, n_threads=8)
+    _say(f"case {spec.n1}x{spec.n2}x{spec.ntheta} l_max={spec.l_max} J={spec.J} p={spec.parity} "
+         f"k_max={spec.k_max} steps={steps} prec={precision}")
+    oracle.evolve(a, n_threads=threads)
     api.evolve(b)
     return a, b
 
@@ -392,3 +405,100 @@ def test_unpaired_legendre_kernels_match_oracle(tmp_path):
     e_obs = float(np.abs(got["obs"] - ref.obs).max()) / max(1.0, float(np.abs(ref.obs).max()))
     _say(f"  psi rel {e_psi:.3e}  obs {e_obs:.3e}")
     assert e_psi <= PSI_TOL and e_obs <= OBS_TOL
+
+
+def test_parity_c2_production_shape():
+    """The 256 x 192 x 96, J = 0 production shape: the Legendre GEMMs with 6
+    column tiles run the one-row-per-lane kernels in 8-wave workgroups (the
+    transforms that replace the reference's ZGEMMs, omegawavepacket.cu:200-240,
+    at this shape), the R passes take 256-point pencils and the r pass
+    192-point pencils.  Wavepacket, observables and autocorrelation after 3
+    steps against the oracle."""
+    spec = inputs.config_spec("c2")
+    a, b = run_both(spec, 3, threads=16)
+    check_parity(a, b)
+
+
+def test_parity_long_coriolis_run():
+    """500 steps with Coriolis coupling: the c3 block structure (J = 20,
+    K_max = 4, 5 blocks) on a 96 x 64 x 32 grid, 8-step graphs throughout.
+    Rounding differences must stay under the per-step bars over the run,
+    with the omega blocks mixing at every half step."""
+    spec = dataclasses.replace(inputs.config_spec("c3"), n1=96, n2=64, ntheta=32)
+    a, b = run_both(spec, 500, threads=16)
+    assert len(a.omegas) == 5
+    check_parity(a, b)
+    # the packet keeps most of its norm and the blocks exchange population
+    pop = a.obs[-1, :, 0]
+    _say(f"  omega populations after 500 steps {np.array2string(pop, precision=4)}")
+    assert pop[1:].sum() > 1e-3 * pop.sum()
+
+
+@pytest.mark.parametrize("shape", [
+    (16, 8, 232, 79, 0),     # forward 5 column tiles, backward 15 (the reference's n_theta = 232)
+    (24, 10, 192, 79, 2),    # forward 5, backward 12, three omega blocks
+    (20, 12, 160, 94, 0),    # forward 6, backward 10
+])
+def test_parity_wide_theta_tiles(shape):
+    """n_theta above 64: the one-row-per-lane Legendre kernels at 5 to 15
+    column tiles (l_max chosen so that both tables fit the LDS)."""
+    n1, n2, nt, lmax, J = shape
+    spec = inputs.JobSpec(n1=n1, n2=n2, ntheta=nt, l_max=lmax, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
+                          J=J, parity=0, k_max=-1, omega0=0, j0=1, E_coll=0.02)
+    a, b = run_both(spec, 3, renormalise=True)
+    check_parity(a, b)
+
+
+@pytest.mark.parametrize("waves", ["4", "8"])
+@pytest.mark.parametrize("shape", [(30, 18, 24), (50, 18, 96)])
+def test_parity_partial_last_strip(monkeypatch, waves, shape):
+    """n1 n2 not a multiple of 16 (540 and 900 points): the last strip of 16
+    points is partial, so the GEMMs clamp its loads to the last point and mask
+    its stores (paired-row kernels at 24 theta points, one-row kernels at 96),
+    in 4- and 8-wave workgroups."""
+    monkeypatch.setenv("QD_LEG_WAVES", waves)
+    n1, n2, nt = shape
+    spec = inputs.JobSpec(n1=n1, n2=n2, ntheta=nt, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
+                          J=3, parity=1, k_max=-1, omega0=1, j0=1, E_coll=0.02)
+    a, b = run_both(spec, 3, renormalise=True)
+    assert (n1 * n2) % 16 != 0
+    check_parity(a, b)
+
+
+def test_autocorrelation_across_resume_matches_oracle():
+    """Resumed from a stored state (steps != 0), the autocorrelation refers to
+    the state at the resume time (the stored state with V(dt/2) restored):
+    GPU against the oracle on the resumed run."""
+    spec = inputs.JobSpec(n1=64, n2=32, ntheta=24, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
+                          J=3, parity=1, k_max=-1, omega0=1, j0=1, E_coll=0.02)
+    job = inputs.build_job(spec)
+    first = abi.Inputs(job, total_steps=3)
+    api.evolve(first)
+    job2 = dict(job)
+    job2["omega_states"] = dict(job["omega_states"], wave_packets=first.psi.copy())
+    _say("case resume acf 64x32x24 J=3 3+4 steps")
+    a = abi.Inputs(job2, total_steps=4, steps=3)
+    b = abi.Inputs(job2, total_steps=4, steps=3)
+    oracle.evolve(a, 8)
+    api.evolve(b)
+    check_parity(a, b)
+
+
+def test_mixed_long_run_norm_drift():
+    """10000 steps of the J = 20, K_max = 4 workload (192 x 128 x 64, 5 blocks)
+    in the mixed form beside FP64: the mixed run completes (its norm check
+    never trips) and its total norm stays within 1e-6 of the FP64 norm at
+    every step."""
+    spec = inputs.config_spec("c3", total_steps=10000)
+    job = inputs.build_job(spec)
+    _say("case mixed long run 192x128x64 J=20 K_max=4 10000 steps, fp64 and mixed")
+    a = abi.Inputs(job, want_acf=False, final_copy=False)
+    b = abi.Inputs(job, want_acf=False, final_copy=False, precision=abi.QD_PREC_MIXED)
+    api.evolve(a)
+    api.evolve(b)
+    na, nb = a.obs[..., 0].sum(1), b.obs[..., 0].sum(1)
+    d = np.abs(nb - na)
+    _say(f"  norm fp64 {na[0]:.9f} -> {na[-1]:.9f}; mixed - fp64: max {d.max():.3e} "
+         f"(step {int(d.argmax())}), at 1000 {d[999]:.3e}, at 10000 {d[-1]:.3e}")
+    assert b.steps[0] == 10000
+    assert d.max() < 1e-6

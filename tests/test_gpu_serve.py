@@ -151,3 +151,73 @@ def test_serve_rejects_oversize_and_copyback():
     with api.Server() as srv:
         with pytest.raises(api.QDError):
             srv.submit(abi.Inputs(job, copy_every=2, callback=lambda user, steps: None))
+
+
+def test_serve_c5_shapes_mix_equals_alone_and_oracle():
+    """The shapes and block counts of the served partial-wave workload, in the
+    mix: 256x192x96 with 9 blocks (FP64) and 5 blocks (mixed), 192x128x64 with
+    7 blocks (FP64, with flux) and 9 blocks (mixed), 128x64x48 with 1 block
+    (FP64, with flux) and 3 blocks (mixed).  This takes the one-row-per-lane
+    GEMMs at 6 column tiles, the paired-row GEMMs, the 9-block R passes on
+    256- and 192-point pencils and both precisions through one server with
+    staggered submissions.  Every job equals qd_evolve alone bit for bit; two
+    FP64 jobs and one mixed job are checked against the oracle."""
+    from oracle import oracle
+
+    def spec(shape, J, p, k_max, steps, **kw):
+        n1, n2, nt = shape
+        return inputs.JobSpec(n1=n1, n2=n2, ntheta=nt, J=J, parity=p, k_max=k_max,
+                              j0=1 if (J + p) % 2 and J else 0, E_coll=0.03, total_steps=steps, **kw)
+    big, mid, small = (256, 192, 96), (192, 128, 64), (128, 64, 48)
+    flux = dict(n_energies=12, energy_range=(0.02, 0.12))
+    specs = [
+        (spec(big, 12, 0, 8, 16), abi.QD_PREC_FP64),
+        (spec(small, 0, 0, -1, 24, **flux), abi.QD_PREC_FP64),
+        (spec(mid, 20, 0, 6, 20, **flux), abi.QD_PREC_FP64),
+        (spec(big, 4, 0, 4, 16), abi.QD_PREC_MIXED),
+        (spec(mid, 30, 0, 8, 20), abi.QD_PREC_MIXED),
+        (spec(small, 2, 0, -1, 24), abi.QD_PREC_MIXED),
+    ]
+    _say("case serve c5 shapes: 256x192x96 (9, 5 blocks), 192x128x64 (7, 9), 128x64x48 (1, 3), "
+         "fp64 + mixed, flux on two, staggered")
+    pairs = []
+    for s, prec in specs:
+        job = inputs.build_job(s)
+        alone = abi.Inputs(job, precision=prec, want_acf=False)
+        mixed = abi.Inputs(job, precision=prec, want_acf=False)
+        api.evolve(alone)
+        pairs.append((job, alone, mixed))
+    assert [len(p[1].omegas) for p in pairs] == [9, 1, 7, 5, 9, 3]
+    with api.Server() as srv:
+        ids = [srv.submit(pairs[0][2]), srv.submit(pairs[1][2])]
+        for _ in range(3):
+            srv.step(8)
+        ids += [srv.submit(pairs[2][2]), srv.submit(pairs[3][2])]
+        for _ in range(2):
+            srv.step(8)
+        ids += [srv.submit(pairs[4][2]), srv.submit(pairs[5][2])]
+        for _ in range(200):
+            nr, nq = srv.step(8)
+            if nr == 0 and nq == 0:
+                break
+        st = [srv.status(i) for i in ids]
+        _say(f"  states {[s['state'] for s in st]} steps {[s['steps'] for s in st]}")
+        assert all(s["state"] == api.Server.DONE for s in st), [srv.error(i) for i in ids]
+    for q, (_, alone, mixed) in enumerate(pairs):
+        _say(f"  job {q}: max |dpsi| mix vs alone {np.abs(alone.psi - mixed.psi).max():.3e}")
+        _same(alone, mixed)
+    for q in (1, 2, 5):
+        job, _, served = pairs[q]
+        ref = abi.Inputs(job, want_acf=False)
+        oracle.evolve(ref, 16)
+        e_psi = float(np.linalg.norm(served.psi - ref.psi) / np.linalg.norm(ref.psi))
+        e_obs = float(np.abs(served.obs - ref.obs).max()) / max(1.0, float(np.abs(ref.obs).max()))
+        tol_psi, tol_obs = (1e-10, 1e-9) if specs[q][1] == abi.QD_PREC_FP64 else (1e-5, 1e-6)
+        msg = f"  job {q} vs oracle: psi rel {e_psi:.3e}  obs {e_obs:.3e}"
+        if job["crp"]["calculate_CRP"]:
+            e_crp = float((np.abs(served.crp - ref.crp) /
+                           (np.abs(ref.crp) + 1e-6 * np.abs(ref.crp).max())).max())
+            msg += f"  crp rel {e_crp:.3e}"
+            assert e_crp < 1e-9
+        _say(msg)
+        assert e_psi < tol_psi and e_obs < tol_obs

@@ -153,6 +153,71 @@ def test_free_particle_exact_propagator():
     assert np.abs(o.obs[:, 0, 2] - ke0).max() < 1e-12 * ke0
 
 
+def _free_acf_closed_form(job, times):
+    """<psi_0|psi(t)> for V = 0 and an l = 0 state: psi(t) = IFFT2(exp(-i t T) FFT2 psi_0)
+    on every theta slice, summed with the quadrature weights and dr1 dr2."""
+    p0 = job["omega_states"]["wave_packets"][:, :, :, 0]
+    r1, r2 = job["r1"], job["r2"]
+    k1 = 2 * np.pi * np.fft.fftfreq(r1["n"], d=r1["dr"])
+    k2 = 2 * np.pi * np.fft.fftfreq(r2["n"], d=r2["dr"])
+    T = k1[:, None] ** 2 / (2 * r1["mass"]) + k2[None, :] ** 2 / (2 * r2["mass"])
+    f0 = np.fft.fft2(p0, axes=(0, 1))
+    w = job["theta"]["w"]
+    out = []
+    for t in times:
+        pt = np.fft.ifft2(f0 * np.exp(-1j * t * T)[:, :, None], axes=(0, 1))
+        out.append(np.sum(np.conj(p0) * pt * w[None, None, :]) * r1["dr"] * r2["dr"])
+    return np.array(out)
+
+
+def test_autocorrelation_free_particle_closed_form():
+    """The oracle's autocorrelation, record n at t_n = (n + 1) dt, against the
+    closed form <psi_0|IFFT2(exp(-i t T) FFT2 psi_0)>.  Its imaginary part is
+    as large as its real part here, so a conjugate on the wrong side fails, and
+    dropping the weights w_k (sum 2) or dr1 dr2 changes the modulus."""
+    job = inputs.build_job(small_spec(E_coll=0.02))
+    job["pot"] = np.zeros_like(job["pot"])
+    n, dt = 8, job["time"]["time_step"]
+    o = run_oracle(job, n)
+    exact = _free_acf_closed_form(job, dt * np.arange(1, n + 1))
+    assert np.abs(exact.imag).max() > 0.3 * np.abs(exact).max()
+    assert np.abs(exact - 1).max() > 1e-2          # the packet has moved
+    assert np.abs(o.acf - exact).max() < 1e-12
+
+
+def test_autocorrelation_constant_potential_phase():
+    """A constant potential V0 commutes with every factor of the step, so the
+    autocorrelation is exp(-i V0 t_n) times the V = 0 one.  The stored state
+    carries exp(-i V0 (t_n - dt/2)) (the leapfrogged half step): an
+    autocorrelation taken on it without restoring V(dt/2) is off by the phase
+    V0 dt / 2 = 0.05 here."""
+    job = inputs.build_job(small_spec(E_coll=0.02))
+    V0, n, dt = 0.01, 8, job["time"]["time_step"]
+    job["pot"] = np.zeros_like(job["pot"])
+    free = run_oracle(job, n).acf
+    job["pot"] = np.full_like(job["pot"], V0)
+    shifted = run_oracle(job, n).acf
+    t = dt * np.arange(1, n + 1)
+    assert np.abs(shifted - np.exp(-1j * V0 * t) * free).max() < 1e-12
+    assert np.abs(shifted - np.exp(-1j * V0 * (t - dt / 2)) * free).max() > 1e-2
+
+
+def test_autocorrelation_across_resume():
+    """Resumed at steps = N1 from the stored state, the autocorrelation refers
+    to the state at t = N1 dt; with V = V0 constant and no absorber the
+    evolution is a group, so record m equals record m of a fresh run."""
+    job = inputs.build_job(small_spec(E_coll=0.02))
+    job["pot"] = np.full_like(job["pot"], 0.01)
+    n1, n2 = 3, 6
+    fresh = run_oracle(job, n2)
+    first = run_oracle(job, n1)
+    res = dict(job)
+    res["omega_states"] = dict(job["omega_states"], wave_packets=first.psi)
+    second = run_oracle(res, n2, start=n1)
+    assert second.steps[0] == n1 + n2
+    assert np.abs(second.acf - fresh.acf).max() < 1e-12
+
+
 def test_rigid_rotor_single_l():
     """V = 0 and a single-l angular state: the rotational energy is l(l+1)<I1+I2>
     of the radial density at every step and the norm is conserved."""
