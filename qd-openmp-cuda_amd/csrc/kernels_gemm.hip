@@ -36,6 +36,7 @@ namespace qd {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 
 __host__ __device__ constexpr int lds_stride(int nt_tiles) { return nt_tiles * 16 + ((nt_tiles & 1) ? 0 : 16); }
 
@@ -78,6 +79,7 @@ struct GemmWave {
   long rows;
   const double *Bs;              // FP64 B in LDS, [kp][Ws]
   const float *Bh, *Bl;          // FP32 hi/lo B in LDS (mixed form)
+  const v4u *Bx;                 // paired mixed form: bfloat16 (bl, bh) per chunk, lane group and column
   int Ws;
   const QD_G double *vph;        // forward: phase table viewed as real (2M x nt), same indexing as A
   const QD_G double *wpot;       // backward epilogue: w_k V(k, m) [NT * 16][M], zero rows k >= N
@@ -109,7 +111,7 @@ __device__ __forceinline__ void stage_b(T *Bs, Src Bg, int kp, int np, int Ws) {
     for (int q = 0; q < CH; q++) {
       const int e = e0 + q * TB + (int)threadIdx.x;
       const int k = e / W, c = e - k * W;
-      v[q] = (e < total && c < np) ? Bg[k * np + c] : T(0);
+      v[q] = (e < total && c < np) ? (T)Bg[k * np + c] : (T)0;
     }
 #pragma unroll
     for (int q = 0; q < CH; q++) {
@@ -342,6 +344,107 @@ __device__ __forceinline__ v2d buffer_load_v2d(__amdgpu_buffer_rsrc_t r, uint32_
   return __builtin_bit_cast(v2d, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, 0, 0));
 }
 
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+// bfloat16 bits of a finite float, rounded to nearest even.
+__device__ __forceinline__ unsigned bf16_bits(float f) {
+  const unsigned u = __float_as_uint(f);
+  return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+
+// Accumulators of one paired-row strip: re and im of 4 NT outputs per lane;
+// item it of tile t holds column acc_col<MIXED>(t, lk, it).
+//   FP64: v_mfma_f64_16x16x4f64 sums.
+//   MIXED: FP32 sums over the strip of hi*hi (h) and of the compensation
+//          terms hi*lo + lo*hi (x, 2^-24 of the total), added in FP64 at the
+//          end (value()).
+template <int NT, bool MIXED> struct PairAcc;
+template <int NT> struct PairAcc<NT, false> {
+  v4d r[NT], i[NT];
+  __device__ __forceinline__ void reset() {
+#pragma unroll
+    for (int t = 0; t < NT; t++) r[t] = i[t] = v4d{0.0, 0.0, 0.0, 0.0};
+  }
+  __device__ __forceinline__ double re(int t, int it) const { return r[t][it]; }
+  __device__ __forceinline__ double im(int t, int it) const { return i[t][it]; }
+};
+template <int NT> struct PairAcc<NT, true> {
+  v4f hr[NT], hi[NT], xr[NT], xi[NT];
+  __device__ __forceinline__ void reset() {
+#pragma unroll
+    for (int t = 0; t < NT; t++) hr[t] = hi[t] = xr[t] = xi[t] = v4f{0.f, 0.f, 0.f, 0.f};
+  }
+  __device__ __forceinline__ double re(int t, int it) const { return (double)hr[t][it] + (double)xr[t][it]; }
+  __device__ __forceinline__ double im(int t, int it) const { return (double)hi[t][it] + (double)xi[t][it]; }
+};
+
+// Multiply-accumulate of one k-chunk (kCh = 4 k-steps, 16 k) of a paired-row
+// strip: the re rows (ar) and im rows (ai) of the lane's point for k-steps
+// c kCh + q, against the B fragment of the same k-steps.
+//   FP64: one v_mfma_f64_16x16x4f64 each for re and im per column tile and
+//         k-step.
+//   MIXED: a = ah + al and B = bh + bl split into FP32 hi and lo parts.
+//     hi*hi: one v_mfma_f32_16x16x4f32 each for re and im per tile and
+//            k-step, B hi from gw.Bh.
+//     hi*lo + lo*hi: one v_mfma_f32_16x16x32_bf16 each for re and im per
+//            tile and chunk.  Lane (lr, lk) holds the 4 k of its k-rows
+//            (k = 16 c + lk + 4 q) and the MFMA pairs its 8 operand slots
+//            with the same slots of B: slots 0..3 ah(k_q) against bl(k_q),
+//            slots 4..7 al(k_q) against bh(k_q) (the staged table gw.Bx holds
+//            those B values as bfloat16, 16 bytes per lane and column).  The
+//            terms are 2^-24 of the product, so bfloat16's 8 bits leave an
+//            error of 2^-32 of it, below the FP32 rounding of hi*hi.
+template <int NT, int kCh, bool MIXED>
+__device__ __forceinline__ void paired_mac(const GemmWave &gw, int c, int lk, int lr, const double (&ar)[kCh],
+                                           const double (&ai)[kCh], PairAcc<NT, MIXED> &acc) {
+  if constexpr (!MIXED) {
+#pragma unroll
+    for (int q = 0; q < kCh; q++) {
+      if (c * kCh + q < gw.ksteps) {
+        const double *brow = gw.Bs + (4 * (c * kCh + q) + lk) * gw.Ws + lr;
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          const double bv = brow[t * 16];
+          acc.r[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, ar[q], acc.r[t], 0, 0, 0);
+          acc.i[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, ai[q], acc.i[t], 0, 0, 0);
+        }
+      }
+    }
+  } else {
+    static_assert(kCh == 4, "the compensation MFMA takes 4 k-steps per lane");
+    unsigned hrb[4], lrb[4], hib[4], lib[4];
+#pragma unroll
+    for (int q = 0; q < kCh; q++) {
+      const float arh = (float)ar[q], aih = (float)ai[q];
+      hrb[q] = bf16_bits(arh);
+      lrb[q] = bf16_bits((float)(ar[q] - (double)arh));
+      hib[q] = bf16_bits(aih);
+      lib[q] = bf16_bits((float)(ai[q] - (double)aih));
+      if (c * kCh + q < gw.ksteps) {
+        const float *brow = gw.Bh + (4 * (c * kCh + q) + lk) * gw.Ws + lr;
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          const float bh = brow[t * 16];
+          acc.hr[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(bh, arh, acc.hr[t], 0, 0, 0);
+          acc.hi[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(bh, aih, acc.hi[t], 0, 0, 0);
+        }
+      }
+    }
+    // operand rows past K load as zero, and B rows past kp are staged as zero
+    const v8bf xa_r = __builtin_bit_cast(v8bf, v4u{hrb[0] | (hrb[1] << 16), hrb[2] | (hrb[3] << 16),
+                                                  lrb[0] | (lrb[1] << 16), lrb[2] | (lrb[3] << 16)});
+    const v8bf xa_i = __builtin_bit_cast(v8bf, v4u{hib[0] | (hib[1] << 16), hib[2] | (hib[3] << 16),
+                                                  lib[0] | (lib[1] << 16), lib[2] | (lib[3] << 16)});
+    const v4u *xrow = gw.Bx + (4 * c + lk) * (NT * 16) + lr;
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      const v8bf xb = __builtin_bit_cast(v8bf, xrow[t * 16]);
+      acc.xr[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xb, xa_r, acc.xr[t], 0, 0, 0);
+      acc.xi[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xb, xa_i, acc.xi[t], 0, 0, 0);
+    }
+  }
+}
+
 // Forward Legendre product with paired rows (FP64): lane (lr, lk) loads the
 // complex point p0 + lr of k-step row 4 c + lk as one 16-byte value (its re
 // and im rows), and its phase the same way, so the phase multiply stays in
@@ -349,7 +452,8 @@ __device__ __forceinline__ v2d buffer_load_v2d(__amdgpu_buffer_rsrc_t r, uint32_
 // rows a second, both with the same B fragment from LDS; the two
 // accumulators hold re and im of the same outputs, stored as 16 bytes.
 // Loads and masking as in gemm_wave (buffer range checks zero rows k >= K).
-template <int NT, int kCh, int DEPTH, typename Stage>
+// MIXED: the products in FP32 hi/lo (paired_mac), everything else as FP64.
+template <int NT, int kCh, int DEPTH, bool MIXED, typename Stage>
 __device__ __forceinline__ void gemm_wave_fwd_paired(const GemmWave &gw, long strip0, int nstrips, int sstride,
                                                      Stage &&stage) {
   const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
@@ -381,40 +485,29 @@ __device__ __forceinline__ void gemm_wave_fwd_paired(const GemmWave &gw, long st
       tp[q] = buffer_load_v2d(rT, v0 + q * kstride);
     }
   };
-  v4d accr[NT], acci[NT];
-#pragma unroll
-  for (int t = 0; t < NT; t++) accr[t] = acci[t] = v4d{0.0, 0.0, 0.0, 0.0};
+  PairAcc<NT, MIXED> acc;
+  acc.reset();
   auto compute = [&](int tile, const v2d(&an)[kCh], const v2d(&tn)[kCh]) {
     const int c = tile % nch;
     const long p = pt_of(tile);
-    if (c == 0) {
-#pragma unroll
-      for (int t = 0; t < NT; t++) accr[t] = acci[t] = v4d{0.0, 0.0, 0.0, 0.0};
-    }
+    if (c == 0) acc.reset();
+    double ar[kCh], ai[kCh];
 #pragma unroll
     for (int q = 0; q < kCh; q++) {
-      if (c * kCh + q < gw.ksteps) {
-        const v2d av = an[q], tv = tn[q];
-        const double ar = fma(-av.y, tv.y, av.x * tv.x);
-        const double ai = fma(av.x, tv.y, av.y * tv.x);
-        const double *brow = gw.Bs + (4 * (c * kCh + q) + lk) * gw.Ws + lr;
-#pragma unroll
-        for (int t = 0; t < NT; t++) {
-          const double bv = brow[t * 16];
-          accr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, ar, accr[t], 0, 0, 0);
-          acci[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, ai, acci[t], 0, 0, 0);
-        }
-      }
+      const v2d av = an[q], tv = tn[q];
+      ar[q] = fma(-av.y, tv.y, av.x * tv.x);
+      ai[q] = fma(av.x, tv.y, av.y * tv.x);
     }
+    paired_mac<NT, kCh, MIXED>(gw, c, lk, lr, ar, ai, acc);
     if (c == nch - 1) {   // epilogue of the strip: streaming 16-byte stores
       const bool pok = p < npts;
 #pragma unroll
       for (int t = 0; t < NT; t++)
 #pragma unroll
         for (int it = 0; it < 4; it++) {
-          const int col = acc_col<false>(t, lk, it);
+          const int col = acc_col<MIXED>(t, lk, it);
           if (pok && col < gw.N)
-            __builtin_nontemporal_store(v2d{accr[t][it], acci[t][it]}, (v2d *)(gw.C + (long)col * gw.ldc + 2 * p));
+            __builtin_nontemporal_store(v2d{acc.re(t, it), acc.im(t, it)}, (v2d *)(gw.C + (long)col * gw.ldc + 2 * p));
         }
     }
   };
@@ -435,7 +528,8 @@ __device__ __forceinline__ void gemm_wave_fwd_paired(const GemmWave &gw, long st
 // without the phase, plus the potential-energy epilogue: each lane owns whole
 // points, so it loads the w_k V values of its point (4 NT per strip, issued
 // before the prefetch) and adds w V (re^2 + im^2) of every output into ob.
-template <int NT, int kCh, int DEPTH, typename Stage>
+// MIXED: the products in FP32 hi/lo (paired_mac).
+template <int NT, int kCh, int DEPTH, bool MIXED, typename Stage>
 __device__ __forceinline__ void gemm_wave_bwd_paired(const GemmWave &gw, long strip0, int nstrips, int sstride,
                                                      double &ob, Stage &&stage) {
   const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
@@ -472,39 +566,29 @@ __device__ __forceinline__ void gemm_wave_bwd_paired(const GemmWave &gw, long st
 #pragma unroll
     for (int t = 0; t < NT; t++)
 #pragma unroll
-      for (int it = 0; it < 4; it++) vh[t][it] = gw.wpot[(long)acc_col<false>(t, lk, it) * gw.M + pc];
+      for (int it = 0; it < 4; it++) vh[t][it] = gw.wpot[(long)acc_col<MIXED>(t, lk, it) * gw.M + pc];
   };
-  v4d accr[NT], acci[NT];
-#pragma unroll
-  for (int t = 0; t < NT; t++) accr[t] = acci[t] = v4d{0.0, 0.0, 0.0, 0.0};
+  PairAcc<NT, MIXED> acc;
+  acc.reset();
   auto compute = [&](int tile, const v2d(&an)[kCh], const double(&vh)[NT][4]) {
     const int c = tile % nch;
     const long p = pt_of(tile);
-    if (c == 0) {
-#pragma unroll
-      for (int t = 0; t < NT; t++) accr[t] = acci[t] = v4d{0.0, 0.0, 0.0, 0.0};
-    }
+    if (c == 0) acc.reset();
+    double ar[kCh], ai[kCh];
 #pragma unroll
     for (int q = 0; q < kCh; q++) {
-      if (c * kCh + q < gw.ksteps) {
-        const v2d av = an[q];
-        const double *brow = gw.Bs + (4 * (c * kCh + q) + lk) * gw.Ws + lr;
-#pragma unroll
-        for (int t = 0; t < NT; t++) {
-          const double bv = brow[t * 16];
-          accr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, av.x, accr[t], 0, 0, 0);
-          acci[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, av.y, acci[t], 0, 0, 0);
-        }
-      }
+      ar[q] = an[q].x;
+      ai[q] = an[q].y;
     }
+    paired_mac<NT, kCh, MIXED>(gw, c, lk, lr, ar, ai, acc);
     if (c == nch - 1) {
       const bool pok = p < npts;
 #pragma unroll
       for (int t = 0; t < NT; t++)
 #pragma unroll
         for (int it = 0; it < 4; it++) {
-          const int col = acc_col<false>(t, lk, it);
-          const double re = accr[t][it], im = acci[t][it];
+          const int col = acc_col<MIXED>(t, lk, it);
+          const double re = acc.re(t, it), im = acc.im(t, it);
           if (pok && col < gw.N)
             __builtin_nontemporal_store(v2d{re, im}, (v2d *)(gw.C + (long)col * gw.ldc + 2 * p));
           ob += (pok ? re * re + im * im : 0.0) * vh[t][it];   // w V = 0 for col >= N
@@ -572,6 +656,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg(StepArgs a) {
   gw.N = FWD ? nl : jb.nt;
   gw.ksteps = kp / 4;
   gw.Bs = Bs; gw.Bh = Bh; gw.Bl = Bl; gw.Ws = Ws;
+  gw.Bx = nullptr;
   gw.vph = (const QD_G double *)jb.vphase;
   gw.wpot = jb.wpot; gw.M = M;
   double ob = 0.0;
@@ -594,7 +679,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg(StepArgs a) {
 }
 
 // Forward Legendre pass on paired rows (strips of 16 complex points).
-template <int NT, int WAVES>
+template <int NT, int WAVES, bool MIXED>
 __global__ __launch_bounds__(WAVES * 64) void k_leg_fwd_paired(StepArgs a) {
   constexpr int TB = WAVES * 64;
   extern __shared__ double lds_d[];
@@ -612,8 +697,16 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg_fwd_paired(StepArgs a) {
   constexpr int W = NT * 16;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   double *Bs = lds_d;
+  float *Bh = (float *)lds_d;
+  v4u *Bx = (v4u *)(Bh + kp * Ws);
+  const int xrows = 4 * ((kp + 15) / 16);
   auto stage = [&]() {
-    stage_b<TB, W>(Bs, jb.wP + (long)b * kp * np, kp, np, Ws);
+    if (!MIXED) {
+      stage_b<TB, W>(Bs, jb.wP + (long)b * kp * np, kp, np, Ws);
+    } else {
+      stage_b<TB, W>(Bh, jb.wP_hi + (long)b * kp * np, kp, np, Ws);
+      stage_b<TB, W>(Bx, (const QD_G v4u *)jb.wP_x + (long)b * xrows * np, xrows, np, W);
+    }
     __syncthreads();
   };
   GemmWave gw;
@@ -624,17 +717,17 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg_fwd_paired(StepArgs a) {
   gw.K = jb.nt;
   gw.N = nl;
   gw.ksteps = kp / 4;
-  gw.Bs = Bs; gw.Bh = nullptr; gw.Bl = nullptr; gw.Ws = Ws;
+  gw.Bs = Bs; gw.Bh = Bh; gw.Bl = nullptr; gw.Bx = Bx; gw.Ws = Ws;
   gw.vph = (const QD_G double *)jb.vphase;
   gw.wpot = jb.wpot; gw.M = M;
   const int S = jb.gemm_strips;
-  gemm_wave_fwd_paired<NT, 4, 3>(gw, (long)g * S * WAVES + wave, S, WAVES, stage);
+  gemm_wave_fwd_paired<NT, 4, 3, MIXED>(gw, (long)g * S * WAVES + wave, S, WAVES, stage);
   // first kernel of the step: apply the previous step's counter advance
   if (u == 0 && threadIdx.x == 0) advance_counters(jb);
 }
 
 // Backward Legendre pass on paired rows, with the potential-energy partials.
-template <int NT, int WAVES>
+template <int NT, int WAVES, bool MIXED>
 __global__ __launch_bounds__(WAVES * 64) void k_leg_bwd_paired(StepArgs a) {
   constexpr int TB = WAVES * 64;
   extern __shared__ double lds_d[];
@@ -654,8 +747,16 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg_bwd_paired(StepArgs a) {
   constexpr int W = NT * 16;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   double *Bs = lds_d;
+  float *Bh = (float *)lds_d;
+  v4u *Bx = (v4u *)(Bh + kp * Ws);
+  const int xrows = 4 * ((kp + 15) / 16);
   auto stage = [&]() {
-    stage_b<TB, W>(Bs, jb.Pt + (long)b * kp * np, kp, np, Ws);
+    if (!MIXED) {
+      stage_b<TB, W>(Bs, jb.Pt + (long)b * kp * np, kp, np, Ws);
+    } else {
+      stage_b<TB, W>(Bh, jb.Pt_hi + (long)b * kp * np, kp, np, Ws);
+      stage_b<TB, W>(Bx, (const QD_G v4u *)jb.Pt_x + (long)b * xrows * np, xrows, np, W);
+    }
     __syncthreads();
   };
   GemmWave gw;
@@ -666,12 +767,13 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg_bwd_paired(StepArgs a) {
   gw.K = nl;
   gw.N = jb.nt;
   gw.ksteps = kp / 4;
-  gw.Bs = Bs; gw.Bh = nullptr; gw.Bl = nullptr; gw.Ws = Ws;
+  gw.Bs = Bs; gw.Bh = Bh; gw.Bl = nullptr; gw.Bx = Bx; gw.Ws = Ws;
   gw.vph = (const QD_G double *)jb.vphase;
   gw.wpot = jb.wpot; gw.M = M;
   double ob = 0.0;
   const int S = jb.gemm_bwd_strips;
-  gemm_wave_bwd_paired<NT, 4, 3>(gw, (long)g * S * WAVES + wave, S, WAVES, ob, stage);
+  // mixed: two tiles in flight (three spill beside the epilogue's potentials)
+  gemm_wave_bwd_paired<NT, 4, MIXED ? 2 : 3, MIXED>(gw, (long)g * S * WAVES + wave, S, WAVES, ob, stage);
   ob = wave_sum64(ob);
   if (lane == 0) red[wave] = ob;
   __syncthreads();
@@ -682,11 +784,25 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg_bwd_paired(StepArgs a) {
   }
 }
 
+// LDS of the paired-row kernels: B as FP64, or (mixed) B hi as FP32 plus the
+// bfloat16 compensation table of 16 bytes per chunk of 16 k, lane group and column
+size_t leg_paired_lds_bytes(int kp, int np, int mixed) {
+  const int nt = np / 16;
+  if (!mixed) return (size_t)kp * lds_stride(nt) * sizeof(double);
+  return (size_t)kp * lds_stride(nt) * sizeof(float) + (size_t)4 * ((kp + 15) / 16) * np * 16;
+}
+
 template <int NT>
-static void launch_leg_bwd_paired_nt(const StepArgs &a, int total_units, int kp_max, int waves, hipStream_t s) {
-  const size_t lds = (size_t)kp_max * lds_stride(NT) * sizeof(double);
-  if (waves == 8) hipLaunchKernelGGL((k_leg_bwd_paired<NT, 8>), dim3(total_units), dim3(512), lds, s, a);
-  else hipLaunchKernelGGL((k_leg_bwd_paired<NT, 4>), dim3(total_units), dim3(256), lds, s, a);
+static void launch_leg_bwd_paired_nt(const StepArgs &a, int total_units, int kp_max, int mixed, int waves,
+                                     hipStream_t s) {
+  const size_t lds = leg_paired_lds_bytes(kp_max, NT * 16, mixed);
+  if (mixed) {
+    if (waves == 8) hipLaunchKernelGGL((k_leg_bwd_paired<NT, 8, true>), dim3(total_units), dim3(512), lds, s, a);
+    else hipLaunchKernelGGL((k_leg_bwd_paired<NT, 4, true>), dim3(total_units), dim3(256), lds, s, a);
+  } else {
+    if (waves == 8) hipLaunchKernelGGL((k_leg_bwd_paired<NT, 8, false>), dim3(total_units), dim3(512), lds, s, a);
+    else hipLaunchKernelGGL((k_leg_bwd_paired<NT, 4, false>), dim3(total_units), dim3(256), lds, s, a);
+  }
 }
 
 // FP64 Legendre passes on paired rows (up to 4 column tiles: wider ones spill,
@@ -703,10 +819,16 @@ bool leg_paired() {
 }
 
 template <int NT>
-static void launch_leg_fwd_paired_nt(const StepArgs &a, int total_units, int kp_max, int waves, hipStream_t s) {
-  const size_t lds = (size_t)kp_max * lds_stride(NT) * sizeof(double);
-  if (waves == 8) hipLaunchKernelGGL((k_leg_fwd_paired<NT, 8>), dim3(total_units), dim3(512), lds, s, a);
-  else hipLaunchKernelGGL((k_leg_fwd_paired<NT, 4>), dim3(total_units), dim3(256), lds, s, a);
+static void launch_leg_fwd_paired_nt(const StepArgs &a, int total_units, int kp_max, int mixed, int waves,
+                                     hipStream_t s) {
+  const size_t lds = leg_paired_lds_bytes(kp_max, NT * 16, mixed);
+  if (mixed) {
+    if (waves == 8) hipLaunchKernelGGL((k_leg_fwd_paired<NT, 8, true>), dim3(total_units), dim3(512), lds, s, a);
+    else hipLaunchKernelGGL((k_leg_fwd_paired<NT, 4, true>), dim3(total_units), dim3(256), lds, s, a);
+  } else {
+    if (waves == 8) hipLaunchKernelGGL((k_leg_fwd_paired<NT, 8, false>), dim3(total_units), dim3(512), lds, s, a);
+    else hipLaunchKernelGGL((k_leg_fwd_paired<NT, 4, false>), dim3(total_units), dim3(256), lds, s, a);
+  }
 }
 
 template <int NT, int KC, bool FWD, int WAVES>
@@ -766,14 +888,16 @@ bool leg_tiles_supported(int nt_tiles) {
   }
 }
 
+bool leg_paired_tiles(int nt_tiles) { return leg_paired() && nt_tiles <= 4; }
+
 void launch_leg_fwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, int waves,
                     hipStream_t s) {
-  if (!mixed && leg_paired() && nt_tiles <= 4) {
+  if (leg_paired_tiles(nt_tiles)) {
     switch (nt_tiles) {
-      case 1: launch_leg_fwd_paired_nt<1>(a, total_units, kp_max, waves, s); return;
-      case 2: launch_leg_fwd_paired_nt<2>(a, total_units, kp_max, waves, s); return;
-      case 3: launch_leg_fwd_paired_nt<3>(a, total_units, kp_max, waves, s); return;
-      case 4: launch_leg_fwd_paired_nt<4>(a, total_units, kp_max, waves, s); return;
+      case 1: launch_leg_fwd_paired_nt<1>(a, total_units, kp_max, mixed, waves, s); return;
+      case 2: launch_leg_fwd_paired_nt<2>(a, total_units, kp_max, mixed, waves, s); return;
+      case 3: launch_leg_fwd_paired_nt<3>(a, total_units, kp_max, mixed, waves, s); return;
+      case 4: launch_leg_fwd_paired_nt<4>(a, total_units, kp_max, mixed, waves, s); return;
       default: break;
     }
   }
@@ -781,12 +905,12 @@ void launch_leg_fwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max
 }
 void launch_leg_bwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, int waves,
                     hipStream_t s) {
-  if (!mixed && leg_paired() && nt_tiles <= 4) {   // 5 and 6 tiles spill in 8-wave workgroups
+  if (leg_paired_tiles(nt_tiles)) {   // 5 and 6 tiles spill in 8-wave workgroups
     switch (nt_tiles) {
-      case 1: launch_leg_bwd_paired_nt<1>(a, total_units, kp_max, waves, s); return;
-      case 2: launch_leg_bwd_paired_nt<2>(a, total_units, kp_max, waves, s); return;
-      case 3: launch_leg_bwd_paired_nt<3>(a, total_units, kp_max, waves, s); return;
-      case 4: launch_leg_bwd_paired_nt<4>(a, total_units, kp_max, waves, s); return;
+      case 1: launch_leg_bwd_paired_nt<1>(a, total_units, kp_max, mixed, waves, s); return;
+      case 2: launch_leg_bwd_paired_nt<2>(a, total_units, kp_max, mixed, waves, s); return;
+      case 3: launch_leg_bwd_paired_nt<3>(a, total_units, kp_max, mixed, waves, s); return;
+      case 4: launch_leg_bwd_paired_nt<4>(a, total_units, kp_max, mixed, waves, s); return;
       default: break;
     }
   }
@@ -852,12 +976,17 @@ static void set_leg_attrs(int lim) {
   }
 }
 
+template <int NT, bool MIXED>
+static void set_leg_paired_attrs_p(int lim) {
+  (void)hipFuncSetAttribute((const void *)k_leg_fwd_paired<NT, 4, MIXED>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+  (void)hipFuncSetAttribute((const void *)k_leg_fwd_paired<NT, 8, MIXED>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+  (void)hipFuncSetAttribute((const void *)k_leg_bwd_paired<NT, 4, MIXED>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+  (void)hipFuncSetAttribute((const void *)k_leg_bwd_paired<NT, 8, MIXED>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+}
 template <int NT>
 static void set_leg_paired_attrs(int lim) {
-  (void)hipFuncSetAttribute((const void *)k_leg_fwd_paired<NT, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
-  (void)hipFuncSetAttribute((const void *)k_leg_fwd_paired<NT, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
-  (void)hipFuncSetAttribute((const void *)k_leg_bwd_paired<NT, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
-  (void)hipFuncSetAttribute((const void *)k_leg_bwd_paired<NT, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+  set_leg_paired_attrs_p<NT, false>(lim);
+  set_leg_paired_attrs_p<NT, true>(lim);
 }
 
 void init_attrs_gemm() {

@@ -45,7 +45,43 @@ int gemm_lds_stride(int nt_tiles) { return nt_tiles * 16 + ((nt_tiles & 1) ? 0 :
 
 size_t gemm_lds_bytes(int kp, int np, bool mixed) {
   const int nt = np / 16;
+  if (leg_paired_tiles(nt)) return leg_paired_lds_bytes(kp, np, mixed ? 1 : 0);
   return (size_t)kp * gemm_lds_stride(nt) * (mixed ? 2 * sizeof(float) : sizeof(double));
+}
+
+// bfloat16 bits of a finite float, rounded to nearest even
+uint16_t bf16_bits(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+// Compensation table of the paired-row mixed kernels for B ([nO][kp][np]
+// row-major FP64): per block, entry (4 c + g, n) holds 8 bfloat16,
+// B_lo(k_q, n) for q < 4 then B_hi(k_q, n) for q < 4, k_q = 16 c + g + 4 q,
+// B_hi = fp32(B), B_lo = fp32(B - B_hi); zero for k_q >= kp.
+std::vector<uint4> bf16_compensation_table(const std::vector<double> &B, int nO, int kp, int np) {
+  const int xrows = 4 * ((kp + 15) / 16);
+  std::vector<uint4> out((size_t)nO * xrows * np);
+  for (int b = 0; b < nO; b++)
+    for (int r = 0; r < xrows; r++)
+      for (int n = 0; n < np; n++) {
+        uint16_t h[8];
+        for (int q = 0; q < 4; q++) {
+          const int k = 16 * (r / 4) + (r % 4) + 4 * q;
+          const double v = k < kp ? B[((size_t)b * kp + k) * np + n] : 0.0;
+          const float hi = (float)v, lo = (float)(v - (double)hi);
+          h[q] = bf16_bits(lo);
+          h[4 + q] = bf16_bits(hi);
+        }
+        uint4 e;
+        e.x = h[0] | ((uint32_t)h[1] << 16);
+        e.y = h[2] | ((uint32_t)h[3] << 16);
+        e.z = h[4] | ((uint32_t)h[5] << 16);
+        e.w = h[6] | ((uint32_t)h[7] << 16);
+        out[((size_t)b * xrows + r) * np + n] = e;
+      }
+  return out;
 }
 
 constexpr size_t kLegLdsTwoGroups = 80 * 1024;
@@ -264,8 +300,8 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   // CUs (2 four-wave or 1 eight-wave workgroup per CU): each wave then runs
   // many pipelined tiles after paying the B staging and pipeline fill once.
   // the paired-row passes take strips of 16 complex points (32 rows)
-  const bool fwd_paired = !sh.mixed && leg_paired() && d.fwd_np / 16 <= 4;
-  const bool bwd_paired = !sh.mixed && leg_paired() && d.bwd_np / 16 <= 4;
+  const bool fwd_paired = leg_paired_tiles(d.fwd_np / 16);
+  const bool bwd_paired = leg_paired_tiles(d.bwd_np / 16);
   const long fwd_strips = fwd_paired ? ((long)M + 15) / 16 : strips;
   const long bwd_strips = bwd_paired ? ((long)M + 15) / 16 : strips;
   auto pick_waves = [&](int kp, int np, long nstrips) {
@@ -430,6 +466,8 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
     for (size_t q = 0; q < Pt.size(); q++) { h2[q] = (float)Pt[q]; lo2[q] = (float)(Pt[q] - (double)h2[q]); }
     d.wP_hi = upload(h, st); d.wP_lo = upload(lo, st);
     d.Pt_hi = upload(h2, st); d.Pt_lo = upload(lo2, st);
+    if (fwd_paired) d.wP_x = upload(bf16_compensation_table(wP, nO, d.fwd_kp, d.fwd_np), st);
+    if (bwd_paired) d.Pt_x = upload(bf16_compensation_table(Pt, nO, d.bwd_kp, d.bwd_np), st);
   }
   d.corV = upload(corV, st);
   d.corPhi = upload(corPhi, st);
@@ -663,6 +701,11 @@ void Engine::set_jobs(const std::vector<DeviceJob *> &jobs) {
     QD_REQUIRE(d.mixed == mixed_, "all jobs of one engine must share a precision");
     QD_REQUIRE(d.gemm_fwd_waves == fwd_waves_ && d.gemm_bwd_waves == bwd_waves_,
                "all jobs of one engine must share the GEMM workgroup size");
+    // the strips of a job (16 points when paired, 16 rows otherwise) follow
+    // its tile counts, and one launch runs one kernel for every job
+    QD_REQUIRE(leg_paired_tiles(d.fwd_np / 16) == leg_paired_tiles(jobs[0]->dev.fwd_np / 16) &&
+                   leg_paired_tiles(d.bwd_np / 16) == leg_paired_tiles(jobs[0]->dev.bwd_np / 16),
+               "all jobs of one engine must share the Legendre kernel form (paired or one-row strips)");
     descs[q] = d;
     for (int k = 0; k < KK_COUNT; k++) bases[(size_t)k * (n + 1) + q + 1] = bases[(size_t)k * (n + 1) + q] + d.units[k];
     lds_fftR_ = std::max(lds_fftR_, (size_t)((d.nO + 1) * d.fft_rows * d.n1 + d.n1) * 16);

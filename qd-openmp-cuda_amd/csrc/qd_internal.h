@@ -75,6 +75,10 @@ struct JobDev {
   const QD_G double *Pt;          // [nO][bwd_kp][bwd_np] row-major: P_l(x_k), k = l - omega
   const QD_G float *wP_hi, *wP_lo;// split FP32 copies for the mixed form
   const QD_G float *Pt_hi, *Pt_lo;
+  // paired-row mixed form: per block [ceil(kp / 16) * 4][np] 16-byte entries,
+  // entry (4 c + g, n) = bfloat16 (B_lo(k_q, n), q < 4; B_hi(k_q, n), q < 4)
+  // with k_q = 16 c + g + 4 q and B_hi = fp32(B), B_lo = fp32(B - B_hi)
+  const QD_G uint4 *wP_x, *Pt_x;
   const QD_G double *corV;        // [L][nO][nO] eigenvectors of C_l: corV[(l nO + a) nO + b] = v_a(omega_b)
   const QD_G double2 *corPhi;     // [L][n1][nO] exp(-i dt/2 l(l+1) I1_i) exp(-i dt/2 e_a I1_i)
   const QD_G double2 *rot2;       // [L][n2] exp(-i dt/2 l(l+1) / (2 m2 r_j^2))
@@ -191,8 +195,12 @@ void launch_vphase(double2 *vph, const double *pot, const double *d1, const doub
                    double dt, hipStream_t stream);
 // column tile counts (N / 16) with a Legendre GEMM instantiation
 bool leg_tiles_supported(int nt_tiles);
-// FP64 Legendre passes on paired rows (default; QD_LEG_PAIRED=0 turns them off)
+// Legendre passes on paired rows (default; QD_LEG_PAIRED=0 turns them off)
 bool leg_paired();
+// whether a Legendre pass with nt_tiles column tiles runs the paired-row kernels
+bool leg_paired_tiles(int nt_tiles);
+// LDS bytes of a paired-row Legendre kernel for B of kp x np (np a multiple of 16)
+size_t leg_paired_lds_bytes(int kp, int np, int mixed);
 void launch_leg_fwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, int waves,
                     hipStream_t stream);
 void launch_leg_bwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, int waves,

@@ -148,7 +148,11 @@ def test_flux_c2_shape_parity_200_energies():
     oracle.evolve(a, 16)
     api.evolve(b)
     ok = _resolved(job, 1e-8)
-    e = crp_rel_err(b.crp[ok], a.crp[ok])
+    # after 20 steps only the tail of the packet reaches the surface, and P
+    # spans eight decades with sign changes: the rounding floor is 1e-12 of
+    # the largest |P| (the converged P(E) of this shape is checked against
+    # the oracle's 2000-step fixture in test_mixed_crp_error_c4_shape)
+    e = crp_rel_err(b.crp[ok], a.crp[ok], floor=1e-3)
     e_psi = float(np.linalg.norm(b.psi - a.psi) / np.linalg.norm(a.psi))
     e_obs = float(np.abs(b.obs - a.obs).max()) / max(1.0, float(np.abs(a.obs).max()))
     _say(f"  crp vs oracle {e:.3e} relative over {int(ok.sum())} energies (max |P| "
@@ -159,24 +163,38 @@ def test_flux_c2_shape_parity_200_energies():
 
 
 def test_mixed_crp_error_c4_shape():
-    """Mixed-precision Legendre GEMMs against FP64 on the 256x192x96 J=0
-    problem, 2000 steps (the flux has passed the surface by then), 200
-    energies: |dP| <= 1e-6 wherever eta_sq >= 1e-2 of its maximum.  Beyond
-    that the error grows like 1/eta_sq (printed per energy)."""
-    job = inputs.build_job(inputs.config_spec("c4", total_steps=2000))
-    _say("case mixed crp 256x192x96 J=0 steps=2000 energies=200")
+    """The 256x192x96 J=0 problem, 2000 steps (the flux has passed the surface
+    by then), 200 energies, against the FP64 oracle's P(E) (the committed
+    fixture tests/golden/c4_oracle_crp.npz, written by make_c4_oracle.py):
+      FP64 form: within 1e-9 relative wherever eta_sq >= 1e-4 of its maximum
+        (below that P = flux / eta_sq divides rounding noise by eta_sq);
+      mixed form (FP32 hi/lo Legendre products): |dP| <= 1e-6 wherever
+        eta_sq >= 1e-2 of its maximum; the error at every energy is printed,
+        it grows like 1/eta_sq outside the packet's energy range."""
+    import os
+    gold = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c4_oracle_crp.npz"))
+    steps = int(gold["steps"])
+    job = inputs.build_job(inputs.config_spec("c4", total_steps=steps))
+    assert np.array_equal(job["crp"]["energies"], gold["energies"])
+    _say(f"case mixed crp 256x192x96 J=0 steps={steps} energies=200 against the oracle fixture")
     a = abi.Inputs(job, want_acf=False, want_observables=False, final_copy=False)
     b = abi.Inputs(job, want_acf=False, want_observables=False, final_copy=False, precision=abi.QD_PREC_MIXED)
     api.evolve(a)
     api.evolve(b)
-    err = np.abs(b.crp - a.crp)
+    ref = gold["crp"]
     eta = job["crp"]["eta_sq"] / job["crp"]["eta_sq"].max()
-    for q in range(0, 200, 10):
+    e64 = np.abs(a.crp - ref)
+    emx = np.abs(b.crp - ref)
+    for q in range(0, 200, 5):
         _say(f"  E_coll {job['crp']['energies'][q] - job['meta']['e_int']:.4f}  eta/max {eta[q]:.1e}  "
-             f"P {a.crp[q]:+.4e}  |dP| {err[q]:.2e}")
-    ok = eta >= 1e-2
-    _say(f"  max |dP| {err[ok].max():.3e} over {int(ok.sum())} energies with eta >= 1e-2 max")
-    assert err[ok].max() <= 1e-6
+             f"P {ref[q]:+.6e}  |dP| fp64 {e64[q]:.2e} mixed {emx[q]:.2e}")
+    for frac in (1e-2, 1e-3, 1e-4, 1e-6):
+        ok = eta >= frac
+        _say(f"  eta >= {frac:.0e} max ({int(ok.sum())} energies): max |dP| fp64 {e64[ok].max():.3e}  "
+             f"mixed {emx[ok].max():.3e}")
+    ok64 = eta >= 1e-4
+    assert crp_rel_err(a.crp[ok64], ref[ok64]) < CRP_TOL
+    assert emx[eta >= 1e-2].max() <= 1e-6
 
 
 def test_reaction_probability_within_bounds_at_dt_5():
