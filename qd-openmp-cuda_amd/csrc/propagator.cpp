@@ -20,6 +20,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <numeric>
 
 #include "host_internal.h"
@@ -99,12 +100,13 @@ void init_attrs_fft();
 void init_attrs_gemm();
 
 void init_kernel_attributes() {
-  static bool done = false;
-  if (done) return;
-  init_attrs_fft();
-  init_attrs_gemm();
-  (void)hipGetLastError();   // an unsupported attribute must not leak into later error checks
-  done = true;
+  // once per process, also when several host threads create handles at once
+  static std::once_flag once;
+  std::call_once(once, [] {
+    init_attrs_fft();
+    init_attrs_gemm();
+    (void)hipGetLastError();   // an unsupported attribute must not leak into later error checks
+  });
 }
 
 JobShape validate_inputs(const qd_inputs *in) {
