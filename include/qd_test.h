@@ -28,6 +28,17 @@ qd_status qd_probe_fft(double *data, int32_t n, int32_t count, int32_t inverse);
  * CoriolisMatrix.C:63-99).  evecs[alpha*n + i] = component i of vector alpha. */
 qd_status qd_probe_coriolis(int32_t J, int32_t l, int32_t omega_min, int32_t n, double *evals, double *evecs);
 
+/* Allocation-free variant of qd_probe_fft, with no counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  `data` is DEVICE memory holding
+ * `count` contiguous complex pencils of length n (interleaved re, im), which
+ * are transformed in place on `stream` (a hipStream_t; NULL = the default
+ * stream); the call returns without waiting for them.  The twiddle table
+ * lives in a static device array of 512 entries per GPU, so n must be at most
+ * 512 (QD_ERR_CAPACITY otherwise) and factor into 2, 3, 5.  Every call on a
+ * GPU rewrites that GPU's table: calls that overlap on one GPU race. */
+qd_status qd_fft_pencils_noalloc(double *data, int32_t n, int32_t count, int32_t inverse, void *stream);
+
 /* Plain device-to-device copy of `bytes` bytes, `repeats` times, timed with
  * HIP events; calibrates the HBM byte counters (FETCH_SIZE / WRITE_SIZE)
  * against a kernel of known traffic.  No counterpart in the reference. */

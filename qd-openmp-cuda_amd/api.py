@@ -33,6 +33,7 @@ _DECLS = {
     "qd_version": (C.c_char_p, []),
     "qd_probe_gemm": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]),
     "qd_probe_fft": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
+    "qd_fft_pencils_noalloc": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]),
     "qd_probe_coriolis": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
     "qd_probe_copy": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
     "qd_server_create": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_void_p)]),
@@ -234,6 +235,12 @@ def probe_fft(x: np.ndarray, inverse: bool = False) -> np.ndarray:
     count, n = a.shape
     _check(lib().qd_probe_fft(a.ctypes.data, n, count, int(inverse)))
     return a
+
+
+def fft_pencils_noalloc(data_ptr: int, n: int, count: int, inverse: bool = False, stream: int = 0) -> None:
+    """qd_fft_pencils_noalloc: in-place DFT of `count` pencils of length n at
+    the DEVICE address data_ptr, enqueued on `stream` (0: the default stream)."""
+    _check(lib().qd_fft_pencils_noalloc(C.c_void_p(data_ptr), n, count, int(inverse), C.c_void_p(stream or None)))
 
 
 def probe_coriolis(J: int, l: int, omega_min: int, n: int):

@@ -136,4 +136,9 @@ class Engine {
 
 void init_kernel_attributes();
 
+// In-place unnormalised DFT of `count` contiguous pencils of length n in
+// device memory, on `stream`, with the twiddles in a static device table
+// (no allocation; n <= kFftMaxN, else Error(QD_ERR_CAPACITY)).
+void fft_pencils_noalloc(double2 *data, int n, int count, int inverse, hipStream_t stream);
+
 }  // namespace qd

@@ -17,7 +17,10 @@
 //                    and rotational energies (cudaOpenMP2.cu:364-370)
 // Forward uses exp(-2 pi i jk/n), inverse exp(+2 pi i jk/n), both unnormalised,
 // like CUFFT_FORWARD / CUFFT_INVERSE.
+#include <mutex>
+
 #include "fft_common.h"
+#include "host_internal.h"
 
 namespace qd {
 
@@ -222,6 +225,61 @@ void launch_fft_probe(double2 *data, const double2 *tw, int n, int count, int in
   const size_t lds = 3 * (size_t)n * sizeof(double2);
   if (inverse) hipLaunchKernelGGL(k_fft_probe<true>, dim3(count), dim3(kBlock), lds, stream, data, tw, n);
   else hipLaunchKernelGGL(k_fft_probe<false>, dim3(count), dim3(kBlock), lds, stream, data, tw, n);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free batched DFT on caller device memory (qd_fft_pencils_noalloc).
+// Its one scratch buffer, the twiddle table, is `scratch`: a file-scope
+// device array of kFftMaxN entries (8 KB), which every GPU the library runs
+// on holds until the process ends.  Host code never hands `scratch` itself to
+// a kernel: the array's device address on the current GPU comes from
+// hipGetSymbolAddress, looked up once per device (with that device current)
+// and kept.  Every call on one GPU rewrites that GPU's table, so calls that
+// overlap on one GPU (different streams or threads) race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double2 scratch[kFftMaxN];
+
+constexpr int kMaxDevices = 64;
+
+// tw[e] = exp(-2 pi i e / n), e < n
+__global__ __launch_bounds__(kBlock) void k_twiddles(double2 *tw, int n) {
+  for (int e = threadIdx.x; e < n; e += kBlock) {
+    double sn, cs;
+    sincospi(-2.0 * (double)e / (double)n, &sn, &cs);
+    tw[e] = make_double2(cs, sn);
+  }
+}
+
+// device address of `scratch` on the current device
+static double2 *scratch_address() {
+  static std::mutex mu;
+  static double2 *addr[kMaxDevices] = {};
+  int dev = 0;
+  QD_HIP(hipGetDevice(&dev));
+  QD_REQUIRE(dev >= 0 && dev < kMaxDevices, "device index out of range");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void *p = nullptr;
+    QD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    QD_REQUIRE(p != nullptr, "hipGetSymbolAddress returned no address for the FFT scratch table");
+    addr[dev] = (double2 *)p;
+  }
+  return addr[dev];
+}
+
+void fft_pencils_noalloc(double2 *data, int n, int count, int inverse, hipStream_t stream) {
+  QD_REQUIRE(data != nullptr && count > 0 && n >= 2, "bad FFT arguments (data, n >= 2, count > 0)");
+  if (n > kFftMaxN)
+    throw Error(QD_ERR_CAPACITY, "n = " + std::to_string(n) + " exceeds the 512 twiddles of the FFT scratch table");
+  int m = n;
+  for (int p : {2, 3, 5})
+    while (m % p == 0) m /= p;
+  QD_REQUIRE(m == 1, "n must factor into 2, 3, 5");
+  init_kernel_attributes();
+  double2 *tw = scratch_address();
+  hipLaunchKernelGGL(k_twiddles, dim3(1), dim3(kBlock), 0, stream, tw, n);
+  launch_fft_probe(data, tw, n, count, inverse, stream);
+  QD_HIP(hipGetLastError());
 }
 
 void init_attrs_fft() {

@@ -339,6 +339,13 @@ qd_status qd_probe_fft(double *data, int32_t n, int32_t count, int32_t inverse) 
   })
 }
 
+qd_status qd_fft_pencils_noalloc(double *data, int32_t n, int32_t count, int32_t inverse, void *stream) {
+  QD_GUARD(nullptr, {
+    fft_pencils_noalloc(reinterpret_cast<double2 *>(data), n, count, inverse, static_cast<hipStream_t>(stream));
+    return QD_OK;
+  })
+}
+
 qd_status qd_probe_copy(int64_t bytes, int32_t repeats, double *us_per_copy) {
   QD_GUARD(nullptr, {
     QD_REQUIRE(bytes >= 16 && repeats > 0 && us_per_copy, "bad probe arguments");
