@@ -13,7 +13,8 @@ partial-wave jobs and reported under "serving" (grid-point-steps/s, jobs per
 hour, p50/p99 latency from submit to result), FP64 and mixed.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-mixed]
-                    [--no-serve] [--serve-jobs N] [--serve-steps S]
+                    [--no-serve] [--serve-jobs N] [--serve-steps S] [--serve-energies E]
+                    [--no-configs] [--bandwidth MIB] [--c4]
 """
 import argparse
 import dataclasses
@@ -162,7 +163,72 @@ def mixed_crp_error(api, abi, inputs, steps):
     return out
 
 
-def measure(api, abi, job, precision, steps, warmup):
+def step_model_seconds(job):
+    """Model time of one step: sum over the kernels of max(bytes / HBM peak,
+    flops / FP64 MFMA peak), with the per-kernel figures of kernel_model."""
+    model, _ = kernel_model(job)
+    return sum(max(m["bytes"] / (HBM_PEAK_GBS * 1e9), m["flops"] / (FP64_MFMA_PEAK_TFLOPS * 1e12))
+               for m in model.values())
+
+
+def other_configs(api, abi, inputs, cpu_threads, with_cpu):
+    """The other two single-job configurations: c1 (128x64x48, J=0), the CPU-
+    runnable case, over its full 2000 steps on the GPU and on the host (the
+    oracle), and c2 (256x192x96, J=0), 200 steps on the GPU and 50 steps on
+    the host after 5 warm-up steps.  Grid-point-steps/s of each, the GPU
+    step against the per-kernel HBM/MFMA model."""
+    if ROOT not in sys.path:
+        sys.path.insert(0, ROOT)
+    out = {}
+    for name, gpu_steps, cpu_steps, cpu_warm in (("c1", 2000, 2000, 0), ("c2", 200, 50, 5)):
+        spec = inputs.config_spec(name)
+        job = inputs.build_job(spec)
+        _, pts = kernel_model(job)
+        rec = {"workload": f"{spec.n1}x{spec.n2}x{spec.ntheta}, J={spec.J}, {len(job['omega_states']['omegas'])} "
+                           f"block(s), dt={spec.dt} au"}
+        for prec_name, prec in (("fp64", abi.QD_PREC_FP64), ("mixed", abi.QD_PREC_MIXED)):
+            el, _ = measure(api, abi, job, prec, gpu_steps, 20, profile_steps=0)
+            rec[prec_name] = {"steps": gpu_steps, "ms_per_step": el / gpu_steps * 1e3,
+                              "grid_point_steps_per_s": pts * gpu_steps / el}
+        rec["fp64"]["model_frac"] = step_model_seconds(job) / (rec["fp64"]["ms_per_step"] * 1e-3)
+        if with_cpu:
+            from oracle import oracle
+            if cpu_warm:
+                oracle.evolve(abi.Inputs(job, total_steps=cpu_warm, want_acf=False), cpu_threads)
+            inp = abi.Inputs(job, total_steps=cpu_steps, want_acf=False, want_observables=False, final_copy=False)
+            t0 = time.perf_counter()
+            oracle.evolve(inp, cpu_threads)
+            dt = time.perf_counter() - t0
+            rec["cpu_oracle"] = {"steps": cpu_steps, "seconds": dt, "threads": cpu_threads,
+                                 "grid_point_steps_per_s": pts * cpu_steps / dt,
+                                 "sample": ("the full run" if cpu_steps == 2000 else
+                                            f"{cpu_steps} steps after {cpu_warm} warm-up steps")}
+        print(f"# {name}: fp64 {rec['fp64']['ms_per_step']:.4f} ms/step, mixed "
+              f"{rec['mixed']['ms_per_step']:.4f} ms/step" +
+              (f", cpu {rec['cpu_oracle']['grid_point_steps_per_s']:.3g} gps/s" if with_cpu else ""),
+              file=sys.stderr, flush=True)
+        out[name] = rec
+    return out
+
+
+def served_crp_error(crp64, crpmx, etas):
+    """Reaction probabilities of the served mixed form against the served FP64
+    form, job by job (FP64 served results equal qd_evolve alone bitwise and
+    match the oracle in the tests): max |dP| over the energies where the job's
+    eta_sq is at least 1e-2 / 1e-3 of its maximum."""
+    out = {}
+    for frac in (1e-2, 1e-3):
+        worst, n = 0.0, 0
+        for a, b, e in zip(crp64, crpmx, etas):
+            ok = e >= frac * e.max()
+            if ok.any():
+                worst = max(worst, float(np.abs(a[ok] - b[ok]).max()))
+                n += int(ok.sum())
+        out[f"eta_ge_{frac:.0e}_max"] = {"energies": n, "max_abs_err": worst}
+    return out
+
+
+def measure(api, abi, job, precision, steps, warmup, profile_steps=20):
     inp = abi.Inputs(job, total_steps=steps + warmup, precision=precision, want_acf=False,
                      want_observables=False, final_copy=False)
     with api.Propagator(inp) as p:
@@ -172,7 +238,7 @@ def measure(api, abi, job, precision, steps, warmup):
         p.run(steps)
         p.synchronize()
         elapsed = time.perf_counter() - t0
-        prof = p.profile(20)
+        prof = p.profile(profile_steps) if profile_steps else None
     return elapsed, prof
 
 
@@ -183,7 +249,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=50)
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the c1 / c2 GPU and CPU figures")
+    ap.add_argument("--serve-energies", type=int, default=50,
+                    help="reaction probabilities per served job (0: none)")
+    ap.add_argument("--workload", default="c3", choices=("c1", "c2", "c3"),
+                    help="single-job workload of the timed steps (profiling; the headline is c3)")
+    ap.add_argument("--bandwidth", type=int, default=0,
+                    help="only run the HBM ceiling probes over this many MiB and exit")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--c4", action="store_true",
                     help="only run the mixed-precision CRP comparison (256x192x96, J=0, 200 energies) and exit")
@@ -205,15 +279,26 @@ def main():
         us = api.probe_copy(nbytes, 10)
         print(json.dumps({"copy_bytes": nbytes, "us_per_copy": us, "GB/s": 2 * nbytes / us / 1e3}))
         return
+    if args.bandwidth:
+        nbytes = args.bandwidth << 20
+        res = {"bytes": nbytes}
+        us = api.probe_copy(nbytes, 10)
+        res["plain_copy"] = {"us": us, "GB/s": 2 * nbytes / us / 1e3}
+        us = api.probe_bandwidth(nbytes, 1, 10)
+        res["wide_copy_nt"] = {"us": us, "GB/s": 2 * nbytes / us / 1e3}
+        us = api.probe_bandwidth(nbytes, 2, 10)
+        res["read_only"] = {"us": us, "GB/s": nbytes / us / 1e3}
+        print(json.dumps(res))
+        return
 
     if args.c4:
         print(json.dumps(mixed_crp_error(api, abi, inputs, args.c4_steps)))
         return
 
-    spec = inputs.config_spec("c3")
+    spec = inputs.config_spec(args.workload)
     job = inputs.build_job(spec)
     model, pts = kernel_model(job)
-    print(f"# workload c3: {spec.n1}x{spec.n2}x{spec.ntheta}, J={spec.J}, K_max={spec.k_max}, "
+    print(f"# workload {args.workload}: {spec.n1}x{spec.n2}x{spec.ntheta}, J={spec.J}, K_max={spec.k_max}, "
           f"{len(job['omega_states']['omegas'])} blocks; steps={args.steps} warmup={args.warmup}",
           file=sys.stderr, flush=True)
 
@@ -264,26 +349,40 @@ def main():
     if not args.no_serve:
         serving = {}
         from qd_openmp_cuda_amd import serve
-        stream = inputs.serve_jobs(n_jobs=args.serve_jobs, total_steps=args.serve_steps)
+        stream = inputs.serve_jobs(n_jobs=args.serve_jobs, total_steps=args.serve_steps,
+                                   n_energies=args.serve_energies)
+        want_crp = args.serve_energies > 0
+        crp = {}
         # "poisson": the seeded arrival times (throughput bounded by the arrival
         # rate); "saturated": every request at t = 0 (the server's capacity)
         for name, prec in (("fp64", abi.QD_PREC_FP64), ("mixed", abi.QD_PREC_MIXED)):
             for mode, scale in (("poisson", 1.0), ("saturated", 0.0)):
-                prepared = serve.prepare(stream, precision=prec)
+                prepared = serve.prepare(stream, precision=prec, want_crp=want_crp)
                 r = serve.run_stream(prepared, time_scale=scale)
+                if mode == "saturated" and want_crp:
+                    crp[name] = [inp.crp.copy() for _, inp in prepared]
+                    crp["eta"] = [inp.eta.copy() for _, inp in prepared]
                 del prepared
                 r.pop("latency_s")
                 serving[f"{name}_{mode}"] = r
                 print(f"# serving {name} {mode}: {r['grid_point_steps_per_s']:.4g} gps/s, "
                       f"p50 {r['latency_p50_s']:.2f} s, p99 {r['latency_p99_s']:.2f} s, {r['failed']} failed",
                       file=sys.stderr, flush=True)
+        if want_crp:
+            serving["mixed_vs_fp64_crp"] = served_crp_error(crp["fp64"], crp["mixed"], crp["eta"])
         serving["workload"] = (f"{args.serve_jobs} Poisson arrivals (4/s, seed 5), J uniform 0..40, "
                                f"K_max=min(J,8), shapes 128x64x48 / 192x128x64 / 256x192x96, "
-                               f"{args.serve_steps} steps each")
+                               f"{args.serve_steps} steps each, reaction probabilities at "
+                               f"{args.serve_energies} energies per job")
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(job, abi, args.cpu_steps)
+
+    configs = None
+    if not args.no_configs:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        configs = other_configs(api, abi, inputs, threads, not args.no_cpu)
 
     out = {
         "metric": "grid-point-steps/s, FP64 vs mixed, J=20 K=4; p50/p99 partial-wave latency",
@@ -298,14 +397,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "J=20, K_max=4, 5 omega blocks of 192x128x64 (R x r x theta), complex128, "
-                               "LEPS surface, dt=10 au",
+        "config": {"workload": ("J=20, K_max=4, 5 omega blocks of 192x128x64 (R x r x theta), complex128, "
+                                "LEPS surface, dt=10 au") if args.workload == "c3" else
+                               f"{args.workload}: {spec.n1}x{spec.n2}x{spec.ntheta}, J={spec.J}, complex128",
                    "grid_points_per_step": pts, "parallelism": "single GPU, all omega blocks resident"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "mixed": mixed,
         "serving": serving,
         "kernels": per_kernel,
+        "configs": configs,
     }
     print(json.dumps(out))
 

@@ -44,6 +44,13 @@ qd_status qd_fft_pencils_noalloc(double *data, int32_t n, int32_t count, int32_t
  * against a kernel of known traffic.  No counterpart in the reference. */
 qd_status qd_probe_copy(int64_t bytes, int32_t repeats, double *us_per_copy);
 
+/* HBM ceiling probes over `bytes` bytes, `repeats` passes timed with HIP
+ * events: kind 1 copies (reads and writes `bytes`) with eight 16-byte loads
+ * in flight per thread and nontemporal stores; kind 2 only reads.  Used to
+ * quote kernels against a measured ceiling and to calibrate the byte
+ * counters.  No counterpart in the reference. */
+qd_status qd_probe_bandwidth(int64_t bytes, int32_t repeats, int32_t kind, double *us_per_pass);
+
 #ifdef __cplusplus
 }
 #endif

@@ -36,6 +36,7 @@ _DECLS = {
     "qd_fft_pencils_noalloc": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]),
     "qd_probe_coriolis": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
     "qd_probe_copy": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
+    "qd_probe_bandwidth": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
     "qd_server_create": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_void_p)]),
     "qd_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32)]),
     "qd_server_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
@@ -248,6 +249,14 @@ def probe_coriolis(J: int, l: int, omega_min: int, n: int):
     v = np.zeros((n, n), order="F")
     _check(lib().qd_probe_coriolis(J, l, omega_min, n, e.ctypes.data, v.ctypes.data))
     return e, v
+
+
+def probe_bandwidth(nbytes: int, kind: int, repeats: int = 10) -> float:
+    """Microseconds per pass of the HBM ceiling probes: kind 1 wide copy of
+    nbytes (nontemporal stores), kind 2 read-only pass over nbytes."""
+    us = C.c_double()
+    _check(lib().qd_probe_bandwidth(nbytes, repeats, kind, C.byref(us)))
+    return us.value
 
 
 def probe_copy(nbytes: int, repeats: int = 10) -> float:

@@ -37,6 +37,19 @@ namespace qd {
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+// bfloat16 bits of a finite float, rounded to nearest even.
+__device__ __forceinline__ unsigned bf16_bits(float f) {
+  const unsigned u = __float_as_uint(f);
+  return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+
+// bfloat16 operand of the compensation MFMA: slots 0..3 the hi parts, 4..7 the lo parts
+__device__ __forceinline__ v8bf bf16_operand(const unsigned (&hb)[4], const unsigned (&lb)[4]) {
+  return __builtin_bit_cast(v8bf, v4u{hb[0] | (hb[1] << 16), hb[2] | (hb[3] << 16), lb[0] | (lb[1] << 16),
+                                      lb[2] | (lb[3] << 16)});
+}
 
 __host__ __device__ constexpr int lds_stride(int nt_tiles) { return nt_tiles * 16 + ((nt_tiles & 1) ? 0 : 16); }
 
@@ -213,17 +226,25 @@ __device__ __forceinline__ void gemm_wave(const GemmWave &gw, long strip0, int n
       }
     }
   };
+  // FP64 sums (acc); mixed: the hi*hi sums of every 4 k-steps (16 k) in
+  // FP32 are added into acc at once, the compensation terms (xx, 2^-24 of
+  // the total) accumulate in FP32 over the strip and join in the epilogue
   v4d acc[NT];
+  v4f xx[MIXED ? NT : 1];
+  auto reset = [&]() {
 #pragma unroll
-  for (int t = 0; t < NT; t++) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int t = 0; t < NT; t++) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
+    if constexpr (MIXED) {
+#pragma unroll
+      for (int t = 0; t < NT; t++) xx[t] = v4f{0.f, 0.f, 0.f, 0.f};
+    }
+  };
+  reset();
   auto compute = [&](int tile, const double(&an)[kCh], const double(&tn)[kCh], const double(&vh)[NT][2]) {
     const int c = tile % nch;
     const long r = row_of(tile);
     const bool rok = r < gw.rows;
-    if (c == 0) {
-#pragma unroll
-      for (int t = 0; t < NT; t++) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
-    }
+    if (c == 0) reset();
     double ac[kCh];
 #pragma unroll
     for (int q = 0; q < kCh; q++) {
@@ -239,7 +260,7 @@ __device__ __forceinline__ void gemm_wave(const GemmWave &gw, long strip0, int n
         ac[q] = av;
       }
     }
-    if (!MIXED) {
+    if constexpr (!MIXED) {
 #pragma unroll
       for (int q = 0; q < kCh; q++) {
         if (c * kCh + q < gw.ksteps) {
@@ -250,31 +271,41 @@ __device__ __forceinline__ void gemm_wave(const GemmWave &gw, long strip0, int n
         }
       }
     } else {
-      // FP32 hi/lo products, FP32 accumulation over 4 k-steps, flushed to FP64
+      // a = ah + al, B = bh + bl in FP32 hi and lo parts: hi*hi on the FP32
+      // MFMA per k-step; hi*lo + lo*hi (2^-24 of the product) on one bf16
+      // MFMA per 4 k-steps (16 k), whose 8 operand slots pair ah(k_q) with
+      // bl(k_q) and al(k_q) with bh(k_q) for the lane's k-rows
+      // k_q = 16 c16 + lk + 4 q (the table gw.Bx; see paired_mac)
+      static_assert(kCh % 4 == 0, "k-chunks of whole compensation groups");
 #pragma unroll
       for (int q0 = 0; q0 < kCh; q0 += 4) {
-        v4f hh[NT], hl[NT];
+        if (c * kCh + q0 < gw.ksteps) {
+          unsigned hb[4], lb[4];
+          v4f hh[NT];
 #pragma unroll
-        for (int t = 0; t < NT; t++) { hh[t] = v4f{0.f, 0.f, 0.f, 0.f}; hl[t] = v4f{0.f, 0.f, 0.f, 0.f}; }
+          for (int t = 0; t < NT; t++) hh[t] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int q = q0; q < q0 + 4; q++) {
-          if (c * kCh + q < gw.ksteps) {
+          for (int q = q0; q < q0 + 4; q++) {
             const float ah = (float)ac[q];
-            const float al = (float)(ac[q] - (double)ah);
-            const int off = (4 * (c * kCh + q) + lk) * gw.Ws + lr;
+            hb[q - q0] = bf16_bits(ah);
+            lb[q - q0] = bf16_bits((float)(ac[q] - (double)ah));
+            if (c * kCh + q < gw.ksteps) {
+              const float *brow = gw.Bh + (4 * (c * kCh + q) + lk) * gw.Ws + lr;
 #pragma unroll
-            for (int t = 0; t < NT; t++) {
-              const float bh = gw.Bh[off + t * 16], bl = gw.Bl[off + t * 16];
-              hh[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(bh, ah, hh[t], 0, 0, 0);
-              hl[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(bl, ah, hl[t], 0, 0, 0);
-              hl[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(bh, al, hl[t], 0, 0, 0);
+              for (int t = 0; t < NT; t++)
+                hh[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(brow[t * 16], ah, hh[t], 0, 0, 0);
             }
           }
+#pragma unroll
+          for (int t = 0; t < NT; t++)
+#pragma unroll
+            for (int it = 0; it < 4; it++) acc[t][it] += (double)hh[t][it];
+          const v8bf xa = bf16_operand(hb, lb);
+          const v4u *xrow = gw.Bx + ((c * kCh + q0) + lk) * (NT * 16) + lr;
+#pragma unroll
+          for (int t = 0; t < NT; t++)
+            xx[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, xrow[t * 16]), xa, xx[t], 0, 0, 0);
         }
-#pragma unroll
-        for (int t = 0; t < NT; t++)
-#pragma unroll
-          for (int it = 0; it < 4; it++) acc[t][it] += (double)hh[t][it] + (double)hl[t][it];
       }
     }
     if (c == nch - 1) {   // epilogue of the strip
@@ -293,7 +324,8 @@ __device__ __forceinline__ void gemm_wave(const GemmWave &gw, long strip0, int n
         for (int it = 0; it < 4; it++) {
           const int col = acc_col<MIXED>(t, lk, it);
           const bool ok = rok && col < gw.N;
-          const double v = acc[t][it];
+          double v = acc[t][it];
+          if constexpr (MIXED) v += (double)xx[t][it];
           // streaming store: the output is read once, by the next pass; kept
           // out of L2 it leaves room for the reused tables (backward GEMM on
           // the J=20 workload: 74 -> 66 us)
@@ -344,22 +376,10 @@ __device__ __forceinline__ v2d buffer_load_v2d(__amdgpu_buffer_rsrc_t r, uint32_
   return __builtin_bit_cast(v2d, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, 0, 0));
 }
 
-typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
-// bfloat16 bits of a finite float, rounded to nearest even.
-__device__ __forceinline__ unsigned bf16_bits(float f) {
-  const unsigned u = __float_as_uint(f);
-  return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
-}
-
-// Accumulators of one paired-row strip: re and im of 4 NT outputs per lane;
-// item it of tile t holds column acc_col<MIXED>(t, lk, it).
-//   FP64: v_mfma_f64_16x16x4f64 sums.
-//   MIXED: FP32 sums over the strip of hi*hi (h) and of the compensation
-//          terms hi*lo + lo*hi (x, 2^-24 of the total), added in FP64 at the
-//          end (value()).
-template <int NT, bool MIXED> struct PairAcc;
-template <int NT> struct PairAcc<NT, false> {
+// FP64 accumulators of one paired-row strip: re and im of 4 NT outputs per
+// lane; item it of tile t holds column acc_col<MIXED>(t, lk, it).
+template <int NT> struct PairAcc {
   v4d r[NT], i[NT];
   __device__ __forceinline__ void reset() {
 #pragma unroll
@@ -367,15 +387,6 @@ template <int NT> struct PairAcc<NT, false> {
   }
   __device__ __forceinline__ double re(int t, int it) const { return r[t][it]; }
   __device__ __forceinline__ double im(int t, int it) const { return i[t][it]; }
-};
-template <int NT> struct PairAcc<NT, true> {
-  v4f hr[NT], hi[NT], xr[NT], xi[NT];
-  __device__ __forceinline__ void reset() {
-#pragma unroll
-    for (int t = 0; t < NT; t++) hr[t] = hi[t] = xr[t] = xi[t] = v4f{0.f, 0.f, 0.f, 0.f};
-  }
-  __device__ __forceinline__ double re(int t, int it) const { return (double)hr[t][it] + (double)xr[t][it]; }
-  __device__ __forceinline__ double im(int t, int it) const { return (double)hi[t][it] + (double)xi[t][it]; }
 };
 
 // Multiply-accumulate of one k-chunk (kCh = 4 k-steps, 16 k) of a paired-row
@@ -385,7 +396,7 @@ template <int NT> struct PairAcc<NT, true> {
 //         k-step.
 //   MIXED: a = ah + al and B = bh + bl split into FP32 hi and lo parts.
 //     hi*hi: one v_mfma_f32_16x16x4f32 each for re and im per tile and
-//            k-step, B hi from gw.Bh.
+//            k-step, B hi from gw.Bh, summed in FP32 over the chunk.
 //     hi*lo + lo*hi: one v_mfma_f32_16x16x32_bf16 each for re and im per
 //            tile and chunk.  Lane (lr, lk) holds the 4 k of its k-rows
 //            (k = 16 c + lk + 4 q) and the MFMA pairs its 8 operand slots
@@ -394,9 +405,10 @@ template <int NT> struct PairAcc<NT, true> {
 //            those B values as bfloat16, 16 bytes per lane and column).  The
 //            terms are 2^-24 of the product, so bfloat16's 8 bits leave an
 //            error of 2^-32 of it, below the FP32 rounding of hi*hi.
+//     Both FP32 sums of the chunk are added into the FP64 accumulators.
 template <int NT, int kCh, bool MIXED>
 __device__ __forceinline__ void paired_mac(const GemmWave &gw, int c, int lk, int lr, const double (&ar)[kCh],
-                                           const double (&ai)[kCh], PairAcc<NT, MIXED> &acc) {
+                                           const double (&ai)[kCh], PairAcc<NT> &acc) {
   if constexpr (!MIXED) {
 #pragma unroll
     for (int q = 0; q < kCh; q++) {
@@ -413,6 +425,9 @@ __device__ __forceinline__ void paired_mac(const GemmWave &gw, int c, int lk, in
   } else {
     static_assert(kCh == 4, "the compensation MFMA takes 4 k-steps per lane");
     unsigned hrb[4], lrb[4], hib[4], lib[4];
+    v4f hr[NT], hi[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) hr[t] = hi[t] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int q = 0; q < kCh; q++) {
       const float arh = (float)ar[q], aih = (float)ai[q];
@@ -425,22 +440,26 @@ __device__ __forceinline__ void paired_mac(const GemmWave &gw, int c, int lk, in
 #pragma unroll
         for (int t = 0; t < NT; t++) {
           const float bh = brow[t * 16];
-          acc.hr[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(bh, arh, acc.hr[t], 0, 0, 0);
-          acc.hi[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(bh, aih, acc.hi[t], 0, 0, 0);
+          hr[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(bh, arh, hr[t], 0, 0, 0);
+          hi[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(bh, aih, hi[t], 0, 0, 0);
         }
       }
     }
     // operand rows past K load as zero, and B rows past kp are staged as zero
-    const v8bf xa_r = __builtin_bit_cast(v8bf, v4u{hrb[0] | (hrb[1] << 16), hrb[2] | (hrb[3] << 16),
-                                                  lrb[0] | (lrb[1] << 16), lrb[2] | (lrb[3] << 16)});
-    const v8bf xa_i = __builtin_bit_cast(v8bf, v4u{hib[0] | (hib[1] << 16), hib[2] | (hib[3] << 16),
-                                                  lib[0] | (lib[1] << 16), lib[2] | (lib[3] << 16)});
+    const v8bf xa_r = bf16_operand(hrb, lrb);
+    const v8bf xa_i = bf16_operand(hib, lib);
     const v4u *xrow = gw.Bx + (4 * c + lk) * (NT * 16) + lr;
+    const v4f zero = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int t = 0; t < NT; t++) {
       const v8bf xb = __builtin_bit_cast(v8bf, xrow[t * 16]);
-      acc.xr[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xb, xa_r, acc.xr[t], 0, 0, 0);
-      acc.xi[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xb, xa_i, acc.xi[t], 0, 0, 0);
+      const v4f xr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xb, xa_r, zero, 0, 0, 0);
+      const v4f xi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xb, xa_i, zero, 0, 0, 0);
+#pragma unroll
+      for (int it = 0; it < 4; it++) {
+        acc.r[t][it] += (double)hr[t][it] + (double)xr[it];
+        acc.i[t][it] += (double)hi[t][it] + (double)xi[it];
+      }
     }
   }
 }
@@ -485,7 +504,7 @@ __device__ __forceinline__ void gemm_wave_fwd_paired(const GemmWave &gw, long st
       tp[q] = buffer_load_v2d(rT, v0 + q * kstride);
     }
   };
-  PairAcc<NT, MIXED> acc;
+  PairAcc<NT> acc;
   acc.reset();
   auto compute = [&](int tile, const v2d(&an)[kCh], const v2d(&tn)[kCh]) {
     const int c = tile % nch;
@@ -568,7 +587,7 @@ __device__ __forceinline__ void gemm_wave_bwd_paired(const GemmWave &gw, long st
 #pragma unroll
       for (int it = 0; it < 4; it++) vh[t][it] = gw.wpot[(long)acc_col<MIXED>(t, lk, it) * gw.M + pc];
   };
-  PairAcc<NT, MIXED> acc;
+  PairAcc<NT> acc;
   acc.reset();
   auto compute = [&](int tile, const v2d(&an)[kCh], const double(&vh)[NT][4]) {
     const int c = tile % nch;
@@ -583,6 +602,10 @@ __device__ __forceinline__ void gemm_wave_bwd_paired(const GemmWave &gw, long st
     paired_mac<NT, kCh, MIXED>(gw, c, lk, lr, ar, ai, acc);
     if (c == nch - 1) {
       const bool pok = p < npts;
+      // the mixed form loads the potentials here, after its FP32 partial
+      // sums are gone (issued ahead of the chunk they would not fit beside them)
+      double vm[NT][4];
+      if constexpr (MIXED) load_pot(tile, vm);
 #pragma unroll
       for (int t = 0; t < NT; t++)
 #pragma unroll
@@ -591,7 +614,8 @@ __device__ __forceinline__ void gemm_wave_bwd_paired(const GemmWave &gw, long st
           const double re = acc.re(t, it), im = acc.im(t, it);
           if (pok && col < gw.N)
             __builtin_nontemporal_store(v2d{re, im}, (v2d *)(gw.C + (long)col * gw.ldc + 2 * p));
-          ob += (pok ? re * re + im * im : 0.0) * vh[t][it];   // w V = 0 for col >= N
+          const double wv = MIXED ? vm[t][it] : vh[t][it];
+          ob += (pok ? re * re + im * im : 0.0) * wv;   // w V = 0 for col >= N
         }
     }
   };
@@ -603,7 +627,7 @@ __device__ __forceinline__ void gemm_wave_bwd_paired(const GemmWave &gw, long st
     for (int d = 0; d < DEPTH; d++) {
       const int tile = t0 + d;
       double vh[NT][4];
-      if (tile < ntiles && tile % nch == nch - 1) load_pot(tile, vh);
+      if (!MIXED && tile < ntiles && tile % nch == nch - 1) load_pot(tile, vh);
       load(tile + DEPTH - 1, abuf[(d + DEPTH - 1) % DEPTH]);
       if (tile < ntiles) compute(tile, abuf[d], vh);
     }
@@ -637,13 +661,15 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg(StepArgs a) {
   // compiler, so strip and tile indices (and the operand row bases) are scalar
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   double *Bs = lds_d;
-  float *Bh = (float *)lds_d, *Bl = Bh + kp * Ws;
+  float *Bh = (float *)lds_d;
+  v4u *Bx = (v4u *)(Bh + kp * Ws);
+  const int xrows = 4 * ((kp + 15) / 16);
   auto stage = [&]() {
     if (!MIXED) {
       stage_b<TB, W>(Bs, (FWD ? jb.wP : jb.Pt) + (long)b * kp * np, kp, np, Ws);
     } else {
       stage_b<TB, W>(Bh, (FWD ? jb.wP_hi : jb.Pt_hi) + (long)b * kp * np, kp, np, Ws);
-      stage_b<TB, W>(Bl, (FWD ? jb.wP_lo : jb.Pt_lo) + (long)b * kp * np, kp, np, Ws);
+      stage_b<TB, W>(Bx, (const QD_G v4u *)(FWD ? jb.wP_x : jb.Pt_x) + (long)b * xrows * np, xrows, np, W);
     }
     __syncthreads();
   };
@@ -655,8 +681,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg(StepArgs a) {
   gw.K = FWD ? jb.nt : nl;
   gw.N = FWD ? nl : jb.nt;
   gw.ksteps = kp / 4;
-  gw.Bs = Bs; gw.Bh = Bh; gw.Bl = Bl; gw.Ws = Ws;
-  gw.Bx = nullptr;
+  gw.Bs = Bs; gw.Bh = Bh; gw.Bl = nullptr; gw.Bx = Bx; gw.Ws = Ws;
   gw.vph = (const QD_G double *)jb.vphase;
   gw.wpot = jb.wpot; gw.M = M;
   double ob = 0.0;
@@ -721,7 +746,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg_fwd_paired(StepArgs a) {
   gw.vph = (const QD_G double *)jb.vphase;
   gw.wpot = jb.wpot; gw.M = M;
   const int S = jb.gemm_strips;
-  gemm_wave_fwd_paired<NT, 4, 3, MIXED>(gw, (long)g * S * WAVES + wave, S, WAVES, stage);
+  // mixed: two tiles in flight (three spill beside the FP32 partial sums)
+  gemm_wave_fwd_paired<NT, 4, MIXED ? 2 : 3, MIXED>(gw, (long)g * S * WAVES + wave, S, WAVES, stage);
   // first kernel of the step: apply the previous step's counter advance
   if (u == 0 && threadIdx.x == 0) advance_counters(jb);
 }
@@ -784,9 +810,9 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg_bwd_paired(StepArgs a) {
   }
 }
 
-// LDS of the paired-row kernels: B as FP64, or (mixed) B hi as FP32 plus the
+// LDS of the Legendre kernels: B as FP64, or (mixed) B hi as FP32 plus the
 // bfloat16 compensation table of 16 bytes per chunk of 16 k, lane group and column
-size_t leg_paired_lds_bytes(int kp, int np, int mixed) {
+size_t leg_lds_bytes(int kp, int np, int mixed) {
   const int nt = np / 16;
   if (!mixed) return (size_t)kp * lds_stride(nt) * sizeof(double);
   return (size_t)kp * lds_stride(nt) * sizeof(float) + (size_t)4 * ((kp + 15) / 16) * np * 16;
@@ -795,7 +821,7 @@ size_t leg_paired_lds_bytes(int kp, int np, int mixed) {
 template <int NT>
 static void launch_leg_bwd_paired_nt(const StepArgs &a, int total_units, int kp_max, int mixed, int waves,
                                      hipStream_t s) {
-  const size_t lds = leg_paired_lds_bytes(kp_max, NT * 16, mixed);
+  const size_t lds = leg_lds_bytes(kp_max, NT * 16, mixed);
   if (mixed) {
     if (waves == 8) hipLaunchKernelGGL((k_leg_bwd_paired<NT, 8, true>), dim3(total_units), dim3(512), lds, s, a);
     else hipLaunchKernelGGL((k_leg_bwd_paired<NT, 4, true>), dim3(total_units), dim3(256), lds, s, a);
@@ -821,7 +847,7 @@ bool leg_paired() {
 template <int NT>
 static void launch_leg_fwd_paired_nt(const StepArgs &a, int total_units, int kp_max, int mixed, int waves,
                                      hipStream_t s) {
-  const size_t lds = leg_paired_lds_bytes(kp_max, NT * 16, mixed);
+  const size_t lds = leg_lds_bytes(kp_max, NT * 16, mixed);
   if (mixed) {
     if (waves == 8) hipLaunchKernelGGL((k_leg_fwd_paired<NT, 8, true>), dim3(total_units), dim3(512), lds, s, a);
     else hipLaunchKernelGGL((k_leg_fwd_paired<NT, 4, true>), dim3(total_units), dim3(256), lds, s, a);
@@ -833,7 +859,7 @@ static void launch_leg_fwd_paired_nt(const StepArgs &a, int total_units, int kp_
 
 template <int NT, int KC, bool FWD, int WAVES>
 static void launch_leg_k(const StepArgs &a, int total_units, int kp_max, int mixed, hipStream_t s) {
-  const size_t lds = (size_t)kp_max * lds_stride(NT) * (mixed ? 2 * sizeof(float) : sizeof(double));
+  const size_t lds = leg_lds_bytes(kp_max, NT * 16, mixed);
   if (mixed) hipLaunchKernelGGL((k_leg<NT, KC, FWD, true, WAVES>), dim3(total_units), dim3(WAVES * 64), lds, s, a);
   else hipLaunchKernelGGL((k_leg<NT, KC, FWD, false, WAVES>), dim3(total_units), dim3(WAVES * 64), lds, s, a);
 }

@@ -184,4 +184,58 @@ void launch_copy(const double2 *src, double2 *dst, long n, hipStream_t stream) {
   hipLaunchKernelGGL(k_copy, dim3(4096), dim3(kBlock), 0, stream, src, dst, n);
 }
 
+// HBM ceiling probes.  Each thread moves 8 values of 16 bytes per trip, all
+// loads issued before the first store; every workgroup walks contiguous
+// chunks of 8 x 256 values.  k_copy_wide writes with nontemporal stores;
+// k_read_wide only reads (one partial sum per workgroup is written, so the
+// loads cannot be dropped).
+constexpr int kWide = 8;
+
+__global__ __launch_bounds__(kBlock) void k_copy_wide(const double2 *__restrict__ src, double2 *__restrict__ dst,
+                                                      long n) {
+  const long chunk = (long)kWide * kBlock;
+  for (long c0 = (long)blockIdx.x * chunk; c0 < n; c0 += (long)gridDim.x * chunk) {
+    double2 v[kWide];
+#pragma unroll
+    for (int k = 0; k < kWide; k++) {
+      const long q = c0 + (long)k * kBlock + threadIdx.x;
+      v[k] = q < n ? src[q] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int k = 0; k < kWide; k++) {
+      const long q = c0 + (long)k * kBlock + threadIdx.x;
+      typedef double v2d_t __attribute__((ext_vector_type(2)));
+      if (q < n) __builtin_nontemporal_store(v2d_t{v[k].x, v[k].y}, (v2d_t *)&dst[q]);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_read_wide(const double2 *__restrict__ src, double *__restrict__ out,
+                                                      long n) {
+  __shared__ double red[4];
+  const long chunk = (long)kWide * kBlock;
+  double acc = 0.0;
+  for (long c0 = (long)blockIdx.x * chunk; c0 < n; c0 += (long)gridDim.x * chunk) {
+    double2 v[kWide];
+#pragma unroll
+    for (int k = 0; k < kWide; k++) {
+      const long q = c0 + (long)k * kBlock + threadIdx.x;
+      v[k] = q < n ? src[q] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int k = 0; k < kWide; k++) acc += v[k].x + v[k].y;
+  }
+  acc = wave_sum64(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) out[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// kind 1: k_copy_wide (src -> dst), 2: k_read_wide (src, partial sums into dst);
+// 2048 workgroups: 8 per CU
+void launch_bandwidth(int kind, const double2 *src, double2 *dst, long n, hipStream_t stream) {
+  if (kind == 2) hipLaunchKernelGGL(k_read_wide, dim3(2048), dim3(kBlock), 0, stream, src, (double *)dst, n);
+  else hipLaunchKernelGGL(k_copy_wide, dim3(2048), dim3(kBlock), 0, stream, src, dst, n);
+}
+
 }  // namespace qd

@@ -42,13 +42,8 @@ bool factors_235(int n) {
 
 int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
-int gemm_lds_stride(int nt_tiles) { return nt_tiles * 16 + ((nt_tiles & 1) ? 0 : 16); }
 
-size_t gemm_lds_bytes(int kp, int np, bool mixed) {
-  const int nt = np / 16;
-  if (leg_paired_tiles(nt)) return leg_paired_lds_bytes(kp, np, mixed ? 1 : 0);
-  return (size_t)kp * gemm_lds_stride(nt) * (mixed ? 2 * sizeof(float) : sizeof(double));
-}
+size_t gemm_lds_bytes(int kp, int np, bool mixed) { return leg_lds_bytes(kp, np, mixed ? 1 : 0); }
 
 // bfloat16 bits of a finite float, rounded to nearest even
 uint16_t bf16_bits(float f) {
@@ -297,7 +292,7 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   // always when B is over 80 KB (one workgroup per CU fits), and otherwise
   // when there are strips enough to give every CU one 8-wave workgroup
   // (the J=20 workload: 72 / 74 us instead of 74 / 80 us).  Not where the
-  // 8-wave build spills (more than 8 column tiles; mixed: more than 5).  The
+  // 8-wave build spills (more than 8 column tiles; mixed: more than 6).  The
   // strips per wave make the workgroups of all blocks fit one round of the
   // CUs (2 four-wave or 1 eight-wave workgroup per CU): each wave then runs
   // many pipelined tiles after paying the B staging and pipeline fill once.
@@ -307,7 +302,7 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   const long fwd_strips = fwd_paired ? ((long)M + 15) / 16 : strips;
   const long bwd_strips = bwd_paired ? ((long)M + 15) / 16 : strips;
   auto pick_waves = [&](int kp, int np, long nstrips) {
-    if (np / 16 > (sh.mixed ? 5 : 8)) return 4;
+    if (np / 16 > (sh.mixed ? 6 : 8)) return 4;
     if (gemm_lds_bytes(kp, np, sh.mixed) > kLegLdsTwoGroups) return 8;
     return nstrips * nO >= 8L * kCUs ? 8 : 4;
   };
@@ -463,13 +458,14 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   d.wP = upload(wP, st);
   d.Pt = upload(Pt, st);
   if (sh.mixed) {
-    std::vector<float> h(wP.size()), lo(wP.size()), h2(Pt.size()), lo2(Pt.size());
-    for (size_t q = 0; q < wP.size(); q++) { h[q] = (float)wP[q]; lo[q] = (float)(wP[q] - (double)h[q]); }
-    for (size_t q = 0; q < Pt.size(); q++) { h2[q] = (float)Pt[q]; lo2[q] = (float)(Pt[q] - (double)h2[q]); }
-    d.wP_hi = upload(h, st); d.wP_lo = upload(lo, st);
-    d.Pt_hi = upload(h2, st); d.Pt_lo = upload(lo2, st);
-    if (fwd_paired) d.wP_x = upload(bf16_compensation_table(wP, nO, d.fwd_kp, d.fwd_np), st);
-    if (bwd_paired) d.Pt_x = upload(bf16_compensation_table(Pt, nO, d.bwd_kp, d.bwd_np), st);
+    // FP32 hi parts for the hi*hi products, bfloat16 tables for the compensation terms
+    std::vector<float> h(wP.size()), h2(Pt.size());
+    for (size_t q = 0; q < wP.size(); q++) h[q] = (float)wP[q];
+    for (size_t q = 0; q < Pt.size(); q++) h2[q] = (float)Pt[q];
+    d.wP_hi = upload(h, st);
+    d.Pt_hi = upload(h2, st);
+    d.wP_x = upload(bf16_compensation_table(wP, nO, d.fwd_kp, d.fwd_np), st);
+    d.Pt_x = upload(bf16_compensation_table(Pt, nO, d.bwd_kp, d.bwd_np), st);
   }
   d.corV = upload(corV, st);
   d.corPhi = upload(corPhi, st);

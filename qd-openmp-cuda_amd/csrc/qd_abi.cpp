@@ -373,6 +373,38 @@ qd_status qd_probe_copy(int64_t bytes, int32_t repeats, double *us_per_copy) {
   })
 }
 
+qd_status qd_probe_bandwidth(int64_t bytes, int32_t repeats, int32_t kind, double *us_per_pass) {
+  QD_GUARD(nullptr, {
+    QD_REQUIRE(bytes >= 16 * 2048 && repeats > 0 && us_per_pass && (kind == 1 || kind == 2),
+               "bad probe arguments");
+    const long n = bytes / 16;
+    double2 *a, *b;
+    QD_HIP(hipMalloc(&a, 16 * n));
+    QD_HIP(hipMalloc(&b, kind == 1 ? 16 * n : 16 * 2048));
+    QD_HIP(hipMemset(a, 0, 16 * n));
+    hipEvent_t e0, e1;
+    QD_HIP(hipEventCreate(&e0));
+    QD_HIP(hipEventCreate(&e1));
+    launch_bandwidth(kind, a, b, n, nullptr);
+    QD_HIP(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < repeats; r++) {
+      const bool swap = kind == 1 && (r % 2);
+      launch_bandwidth(kind, swap ? b : a, swap ? a : b, n, nullptr);
+    }
+    QD_HIP(hipEventRecord(e1, nullptr));
+    QD_HIP(hipEventSynchronize(e1));
+    QD_HIP(hipGetLastError());
+    float ms = 0.f;
+    QD_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *us_per_pass = ms * 1000.0 / repeats;
+    QD_HIP(hipEventDestroy(e0));
+    QD_HIP(hipEventDestroy(e1));
+    QD_HIP(hipFree(a));
+    QD_HIP(hipFree(b));
+    return QD_OK;
+  })
+}
+
 qd_status qd_probe_coriolis(int32_t J, int32_t l, int32_t omega_min, int32_t n, double *evals, double *evecs) {
   QD_GUARD(nullptr, {
     QD_REQUIRE(evals && evecs && n >= 1 && n <= 64, "bad probe arguments");

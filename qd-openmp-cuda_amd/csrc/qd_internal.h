@@ -73,9 +73,8 @@ struct JobDev {
   const QD_G double *wpot;        // [bwd_np][M] w_k V(theta_k, m), zero rows k >= nt
   const QD_G double *wP;          // [nO][fwd_kp][fwd_np] row-major (c fastest): w_k P_l(x_k)
   const QD_G double *Pt;          // [nO][bwd_kp][bwd_np] row-major: P_l(x_k), k = l - omega
-  const QD_G float *wP_hi, *wP_lo;// split FP32 copies for the mixed form
-  const QD_G float *Pt_hi, *Pt_lo;
-  // paired-row mixed form: per block [ceil(kp / 16) * 4][np] 16-byte entries,
+  const QD_G float *wP_hi, *Pt_hi;// mixed form: FP32 hi parts of wP and Pt
+  // mixed form: per block [ceil(kp / 16) * 4][np] 16-byte entries,
   // entry (4 c + g, n) = bfloat16 (B_lo(k_q, n), q < 4; B_hi(k_q, n), q < 4)
   // with k_q = 16 c + g + 4 q and B_hi = fp32(B), B_lo = fp32(B - B_hi)
   const QD_G uint4 *wP_x, *Pt_x;
@@ -199,8 +198,8 @@ bool leg_tiles_supported(int nt_tiles);
 bool leg_paired();
 // whether a Legendre pass with nt_tiles column tiles runs the paired-row kernels
 bool leg_paired_tiles(int nt_tiles);
-// LDS bytes of a paired-row Legendre kernel for B of kp x np (np a multiple of 16)
-size_t leg_paired_lds_bytes(int kp, int np, int mixed);
+// LDS bytes of a Legendre kernel for B of kp x np (np a multiple of 16)
+size_t leg_lds_bytes(int kp, int np, int mixed);
 void launch_leg_fwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, int waves,
                     hipStream_t stream);
 void launch_leg_bwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, int waves,
@@ -231,6 +230,9 @@ void launch_flux_finish(const double2 *A, const double2 *B, const double *w, con
 
 // plain copy kernel (traffic calibration)
 void launch_copy(const double2 *src, double2 *dst, long n, hipStream_t stream);
+// HBM ceiling probes: kind 1 wide copy with nontemporal stores, kind 2 read only
+// (dst receives 2048 partial sums)
+void launch_bandwidth(int kind, const double2 *src, double2 *dst, long n, hipStream_t stream);
 
 // GEMM probe for tests: C (rows x n) = A (rows x k, lda) * B ([kp][np] row-major)
 void launch_gemm_probe(const double *A, int lda, const double *B, int kp, int np, double *C, int ldc,

@@ -378,10 +378,11 @@ def config_spec(name: str, **over) -> JobSpec:
 
 def serve_jobs(n_jobs: int = 64, seed: int = 5, rate_per_s: float = 4.0,
                shapes: Sequence[tuple] = ((128, 64, 48), (192, 128, 64), (256, 192, 96)),
-               j_max: int = 40, k_cap: int = 8, total_steps: int = 200):
+               j_max: int = 40, k_cap: int = 8, total_steps: int = 200, n_energies: int = 0):
     """Seeded Poisson stream of partial-wave jobs (J uniform in 0..j_max,
-    K_max = min(J, k_cap), v0 in {0,1}, j0 in {0,1,2}, shape from `shapes`).
-    Returns a list of (arrival_seconds, JobSpec)."""
+    K_max = min(J, k_cap), v0 in {0,1}, j0 in {0,1,2}, shape from `shapes`;
+    reaction probabilities at n_energies collision energies in [0.01, 0.3]
+    hartree when n_energies > 0).  Returns a list of (arrival_seconds, JobSpec)."""
     rng = np.random.default_rng(seed)
     t = 0.0
     jobs = []
@@ -396,7 +397,7 @@ def serve_jobs(n_jobs: int = 64, seed: int = 5, rate_per_s: float = 4.0,
         lo = 0 if (J == 0 or (J + p) % 2 == 0) else 1
         j0 = max(j0, lo)
         spec = JobSpec(n1=n1, n2=n2, ntheta=nt, J=J, parity=p, k_max=min(J, k_cap),
-                       v0=v0, j0=j0, E_coll=ec, total_steps=total_steps)
+                       v0=v0, j0=j0, E_coll=ec, total_steps=total_steps, n_energies=n_energies)
         jobs.append((t, spec))
     return jobs
 
