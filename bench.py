@@ -254,6 +254,9 @@ def main():
                     help="skip the c1 / c2 GPU and CPU figures")
     ap.add_argument("--serve-energies", type=int, default=50,
                     help="reaction probabilities per served job (0: none)")
+    ap.add_argument("--serve-only", default="", choices=("", "fp64_saturated", "mixed_saturated", "fp64_poisson",
+                                                         "mixed_poisson"),
+                    help="only run this serving case and print its figures (profiling)")
     ap.add_argument("--workload", default="c3", choices=("c1", "c2", "c3"),
                     help="single-job workload of the timed steps (profiling; the headline is c3)")
     ap.add_argument("--bandwidth", type=int, default=0,
@@ -293,6 +296,18 @@ def main():
 
     if args.c4:
         print(json.dumps(mixed_crp_error(api, abi, inputs, args.c4_steps)))
+        return
+
+    if args.serve_only:
+        from qd_openmp_cuda_amd import serve
+        name, mode = args.serve_only.split("_")
+        stream = inputs.serve_jobs(n_jobs=args.serve_jobs, total_steps=args.serve_steps,
+                                   n_energies=args.serve_energies)
+        prepared = serve.prepare(stream, precision=abi.QD_PREC_MIXED if name == "mixed" else abi.QD_PREC_FP64,
+                                 want_crp=args.serve_energies > 0)
+        r = serve.run_stream(prepared, time_scale=0.0 if mode == "saturated" else 1.0)
+        r.pop("latency_s")
+        print(json.dumps(r))
         return
 
     spec = inputs.config_spec(args.workload)
@@ -370,6 +385,16 @@ def main():
                       file=sys.stderr, flush=True)
         if want_crp:
             serving["mixed_vs_fp64_crp"] = served_crp_error(crp["fp64"], crp["mixed"], crp["eta"])
+        # the same 64 jobs one at a time (qd_run, device-resident, FP64): the
+        # time the saturated server would need at the single-job rates
+        alone = 0.0
+        for _, spec in stream:
+            alone += measure(api, abi, inputs.build_job(spec), abi.QD_PREC_FP64, spec.total_steps, 0,
+                             profile_steps=0)[0]
+        serving["fp64_alone_s"] = alone
+        serving["fp64_saturated_vs_alone"] = alone / serving["fp64_saturated"]["wall_s"]
+        print(f"# serving fp64: the jobs one at a time take {alone:.2f} s of steps; saturated server at "
+              f"{serving['fp64_saturated_vs_alone']:.2f} of that rate", file=sys.stderr, flush=True)
         serving["workload"] = (f"{args.serve_jobs} Poisson arrivals (4/s, seed 5), J uniform 0..40, "
                                f"K_max=min(J,8), shapes 128x64x48 / 192x128x64 / 256x192x96, "
                                f"{args.serve_steps} steps each, reaction probabilities at "

@@ -50,10 +50,25 @@ struct JobShape {
 // Throws Error(QD_ERR_INVALID) on any inconsistency; reads only host memory.
 JobShape validate_inputs(const qd_inputs *in);
 
+// Where a job's device memory and its large host<->device copies come from.
+// The plain form allocates with hipMalloc, frees with hipFree and copies
+// from and to the caller's pageable arrays; the server's form recycles
+// allocations and copies through pinned staging buffers (server.cpp).
+struct DeviceMemory {
+  virtual ~DeviceMemory() = default;
+  virtual void *allocate(size_t bytes);                 // Error(QD_ERR_NOMEM) on failure
+  virtual void release(void *p, size_t bytes);
+  // host -> device on `s`, ordered before later work on s; src may be reused on return
+  virtual void h2d(void *dst, const void *src, size_t bytes, hipStream_t s);
+  // device -> host after the work already on `s`; complete on return
+  virtual void d2h(void *dst, const void *src, size_t bytes, hipStream_t s);
+  static DeviceMemory &plain();
+};
+
 // Device memory owned by one job.
 class DeviceJob {
  public:
-  DeviceJob(const qd_inputs *in, const JobShape &shape, hipStream_t stream);
+  DeviceJob(const qd_inputs *in, const JobShape &shape, hipStream_t stream, DeviceMemory *mem = nullptr);
   ~DeviceJob();
   DeviceJob(const DeviceJob &) = delete;
   DeviceJob &operator=(const DeviceJob &) = delete;
@@ -84,7 +99,8 @@ class DeviceJob {
   static size_t estimate_footprint(const JobShape &s, int n_energies);
 
  private:
-  std::vector<void *> allocs_;
+  DeviceMemory *mem_;
+  std::vector<std::pair<void *, size_t>> allocs_;
   void *alloc(size_t bytes);
   template <class T> T *upload(std::vector<T> v, hipStream_t s);
   std::vector<std::shared_ptr<void>> staging_;   // host sources of the set-up copies
@@ -114,6 +130,9 @@ class Engine {
   // launch gap between graphs (about 9 us on the J=20 workload) k - 1 times.
   static constexpr int kGraphSteps = 8;
   void step_graph(int k = 1);
+  // whether the graph of k steps is captured and instantiated (launching it
+  // captures nothing)
+  bool graph_ready(int k) const { return graph_exec_[k == 1 ? 0 : 1] != nullptr; }
   void drop_graph();
 
  private:
@@ -122,6 +141,10 @@ class Engine {
   JobDev *d_jobs_ = nullptr;
   int *d_bases_ = nullptr;      // [KK_COUNT][n_jobs + 1]
   int n_alloc_jobs_ = 0;
+  // host sources of the descriptor uploads, kept until upload_done_ passes
+  std::vector<JobDev> descs_;
+  std::vector<int> bases_;
+  hipEvent_t upload_done_ = nullptr;
   int total_[KK_COUNT] = {};
   size_t lds_fftR_ = 0, lds_fftr_ = 0;
   int ct_n1_ = 0, ct_n2_ = 0;   // lengths of the compile-time FFT kernels, 0 = runtime path

@@ -215,12 +215,37 @@ JobShape validate_inputs(const qd_inputs *in) {
 // ---------------------------------------------------------------------------
 // DeviceJob
 // ---------------------------------------------------------------------------
-void *DeviceJob::alloc(size_t bytes) {
+void *DeviceMemory::allocate(size_t bytes) {
   void *p = nullptr;
-  if (bytes == 0) bytes = 16;
   hipError_t e = hipMalloc(&p, bytes);
-  if (e != hipSuccess) throw Error(QD_ERR_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
-  allocs_.push_back(p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw Error(QD_ERR_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  }
+  return p;
+}
+
+void DeviceMemory::release(void *p, size_t) { (void)hipFree(p); }
+
+void DeviceMemory::h2d(void *dst, const void *src, size_t bytes, hipStream_t s) {
+  QD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+}
+
+void DeviceMemory::d2h(void *dst, const void *src, size_t bytes, hipStream_t s) {
+  QD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+  QD_HIP(hipStreamSynchronize(s));
+}
+
+DeviceMemory &DeviceMemory::plain() {
+  static DeviceMemory m;
+  return m;
+}
+
+void *DeviceJob::alloc(size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  bytes = (bytes + 255) / 256 * 256;
+  void *p = mem_->allocate(bytes);
+  allocs_.emplace_back(p, bytes);
   footprint += bytes;
   return p;
 }
@@ -239,7 +264,7 @@ T *DeviceJob::upload(std::vector<T> v, hipStream_t s) {
 }
 
 DeviceJob::~DeviceJob() {
-  for (void *p : allocs_) (void)hipFree(p);
+  for (auto &a : allocs_) mem_->release(a.first, a.second);
 }
 
 // zeroed slabs past the last block that the GEMM operand reads may touch:
@@ -268,7 +293,8 @@ size_t DeviceJob::estimate_footprint(const JobShape &s, int n_energies) {
   return b;
 }
 
-DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : shape(sh) {
+DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st, DeviceMemory *mem)
+    : shape(sh), mem_(mem ? mem : &DeviceMemory::plain()) {
   const int n1 = sh.n1, n2 = sh.n2, nt = sh.nt, M = sh.M, L = sh.L, nO = sh.nO;
   const long G = (long)M * nt;
   const double dt = sh.dt;
@@ -423,7 +449,7 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   const size_t psi_pad = gemm_pad_slabs_psi(sh, d), fbr_pad = gemm_pad_slabs_fbr(sh, d);
   d.psi = (double2 *)alloc(16 * (G * nO + psi_pad * M));
   QD_HIP(hipMemsetAsync(d.psi, 0, 16 * (G * nO + psi_pad * M), st));
-  QD_HIP(hipMemcpyAsync(d.psi, in->omega_states.wave_packets, 16 * G * nO, hipMemcpyHostToDevice, st));
+  mem_->h2d(d.psi, in->omega_states.wave_packets, 16 * G * nO, st);
   // fbr also serves once as the staging block of the initial state's FBR
   // coefficients below, so it holds at least fwd_kp slabs
   const size_t fbr_slabs = std::max<size_t>((size_t)L * nO + fbr_pad, (size_t)d.fwd_kp);
@@ -431,7 +457,7 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st) : 
   d.c0 = nullptr;
   {
     double *pp = (double *)alloc(sizeof(double) * (size_t)G);
-    QD_HIP(hipMemcpyAsync(pp, in->pot, sizeof(double) * (size_t)G, hipMemcpyHostToDevice, st));
+    mem_->h2d(pp, in->pot, sizeof(double) * (size_t)G, st);
     d.pot = pp;
   }
   {
@@ -597,8 +623,7 @@ void DeviceJob::drain(int records_total) {
 
 void DeviceJob::copy_psi(double *dst, hipStream_t s) const {
   const size_t bytes = 16 * (size_t)shape.M * shape.nt * shape.nO;
-  QD_HIP(hipMemcpyAsync(dst, dev.psi, bytes, hipMemcpyDeviceToHost, s));
-  QD_HIP(hipStreamSynchronize(s));
+  mem_->d2h(dst, dev.psi, bytes, s);
 }
 
 void DeviceJob::prephase_if_first(hipStream_t s) {
@@ -654,6 +679,7 @@ Engine::Engine(hipStream_t s) : stream_(s) {}
 
 Engine::~Engine() {
   drop_graph();
+  if (upload_done_) (void)hipEventDestroy(upload_done_);
   if (d_jobs_) (void)hipFree(d_jobs_);
   if (d_bases_) (void)hipFree(d_bases_);
 }
@@ -667,11 +693,18 @@ void Engine::drop_graph() {
   }
 }
 
+// The descriptor tables go up with asynchronous copies on the engine's
+// stream, ordered after the steps already enqueued there; the host copies
+// stay untouched until the previous upload has passed (upload_done_), so the
+// host does not wait for the stream's queued steps.
 void Engine::set_jobs(const std::vector<DeviceJob *> &jobs) {
   drop_graph();
   jobs_ = jobs;
   const int n = (int)jobs.size();
+  if (!upload_done_) QD_HIP(hipEventCreateWithFlags(&upload_done_, hipEventDisableTiming));
+  else QD_HIP(hipEventSynchronize(upload_done_));
   if (n > n_alloc_jobs_) {
+    QD_HIP(hipStreamSynchronize(stream_));   // the old tables may still be read by queued steps
     if (d_jobs_) QD_HIP(hipFree(d_jobs_));
     if (d_bases_) QD_HIP(hipFree(d_bases_));
     d_jobs_ = nullptr;
@@ -682,8 +715,10 @@ void Engine::set_jobs(const std::vector<DeviceJob *> &jobs) {
     n_alloc_jobs_ = cap;
   }
   if (n == 0) return;
-  std::vector<JobDev> descs(n);
-  std::vector<int> bases((size_t)KK_COUNT * (n + 1), 0);
+  descs_.assign(n, JobDev{});
+  bases_.assign((size_t)KK_COUNT * (n + 1), 0);
+  std::vector<JobDev> &descs = descs_;
+  std::vector<int> &bases = bases_;
   lds_fftR_ = lds_fftr_ = 0;
   ct_n1_ = jobs[0]->dev.n1;
   ct_n2_ = jobs[0]->dev.n2;
@@ -728,7 +763,7 @@ void Engine::set_jobs(const std::vector<DeviceJob *> &jobs) {
   for (int k = 0; k < KK_COUNT; k++) total_[k] = bases[(size_t)k * (n + 1) + n];
   QD_HIP(hipMemcpyAsync(d_jobs_, descs.data(), sizeof(JobDev) * n, hipMemcpyHostToDevice, stream_));
   QD_HIP(hipMemcpyAsync(d_bases_, bases.data(), sizeof(int) * bases.size(), hipMemcpyHostToDevice, stream_));
-  QD_HIP(hipStreamSynchronize(stream_));
+  QD_HIP(hipEventRecord(upload_done_, stream_));
 }
 
 StepArgs Engine::args(int kind) const {

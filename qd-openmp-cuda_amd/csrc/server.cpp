@@ -10,11 +10,14 @@
 // their kernels overlap on the device.
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -41,7 +44,7 @@ struct ServerJob {
   JobShape shape{};
   size_t estimate = 0;       // footprint estimate used for admission
   size_t charged = 0;        // device bytes actually held while running
-  int state = QD_JOB_QUEUED;
+  int state = QD_JOB_QUEUED; // guarded by qd_server::mu
   int target = 0;            // steps to run (time.total_steps)
   int enqueued = 0;          // steps enqueued so far
   int checked = 0;           // observable records norm-checked
@@ -55,27 +58,155 @@ struct ShapeClass {
   hipStream_t stream = nullptr;
   std::unique_ptr<Engine> engine;
   std::vector<ServerJob *> members;
+  std::deque<hipEvent_t> in_flight;   // one event per enqueued step graph, oldest first
+  std::vector<hipEvent_t> spare;
   bool dirty = true;
 };
 
+// Work of the server's host worker thread: building an admitted job (device
+// allocation, tables, the wavepacket upload) and retiring a finished one
+// (waiting for its last step, norm check, outputs, freeing its memory), so
+// that the stepping thread only launches steps.
+struct Task {
+  enum Kind { BUILD, RETIRE } kind;
+  ServerJob *sj;
+  hipEvent_t after = nullptr;   // RETIRE: recorded on the class stream after the job's last step
+};
+
 constexpr int kDrainEvery = 512;   // records per job between ring drains (ring holds 1024)
-// Jobs admitted per server step.  Building a job (allocation, tables, the
-// wavepacket upload: up to 0.7 GB for 9 blocks of 256 x 192 x 96) is host
-// work; admitting a few per step lets the classes already running keep the
-// GPU busy meanwhile, instead of a burst of arrivals holding it idle.
+// Jobs handed to the builder per server step: the builder works through them
+// while the running classes keep the GPU busy.
 constexpr int kAdmitPerStep = 4;
+// Step graphs a class may have queued on its stream before the stepping
+// thread waits for the oldest: bounds how far the host runs ahead.
+constexpr int kMaxInFlight = 3;
+
+// Device memory of the served jobs.  Allocations of retired jobs are kept
+// and handed to later jobs of the same sizes (the served shapes repeat):
+// hipFree waits for the device and holds the runtime meanwhile, which
+// stalled the stepping thread's launches.  A failed hipMalloc returns the
+// cache and retries.  The large copies (wavepackets, potential) go through
+// two pinned staging buffers of 32 MB, so that the DMA engine moves them and
+// the runtime is not held by a copy from pageable memory.  Used by the
+// worker thread only (and by the server's destruction after it has joined).
+class ServerMemory : public DeviceMemory {
+ public:
+  ~ServerMemory() override {
+    trim();
+    for (int b = 0; b < 2; b++) {
+      if (ev_[b]) (void)hipEventSynchronize(ev_[b]);
+      if (ev_[b]) (void)hipEventDestroy(ev_[b]);
+      if (pin_[b]) (void)hipHostFree(pin_[b]);
+    }
+  }
+  void *allocate(size_t bytes) override {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      auto it = free_.find(bytes);
+      if (it != free_.end()) {
+        void *p = it->second;
+        free_.erase(it);
+        return p;
+      }
+    }
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      trim();
+      hipError_t e = hipMalloc(&p, bytes);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(QD_ERR_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+      }
+    }
+    return p;
+  }
+  void release(void *p, size_t bytes) override {
+    std::lock_guard<std::mutex> lk(mu_);
+    free_.emplace(bytes, p);
+  }
+  void trim() {
+    std::lock_guard<std::mutex> lk(mu_);
+    for (auto &f : free_) (void)hipFree(f.second);
+    free_.clear();
+  }
+  void h2d(void *dst, const void *src, size_t bytes, hipStream_t s) override {
+    pinned();
+    int b = 0;
+    for (size_t off = 0; off < bytes; off += kChunk, b ^= 1) {
+      const size_t n = std::min(kChunk, bytes - off);
+      if (used_[b]) QD_HIP(hipEventSynchronize(ev_[b]));
+      std::memcpy(pin_[b], (const char *)src + off, n);
+      QD_HIP(hipMemcpyAsync((char *)dst + off, pin_[b], n, hipMemcpyHostToDevice, s));
+      QD_HIP(hipEventRecord(ev_[b], s));
+      used_[b] = true;
+    }
+  }
+  void d2h(void *dst, const void *src, size_t bytes, hipStream_t s) override {
+    pinned();
+    int b = 0, pb = -1;
+    size_t poff = 0, pn = 0;
+    auto finish = [&]() {   // the previous chunk: wait, copy out of its staging buffer
+      if (pb < 0) return;
+      QD_HIP(hipEventSynchronize(ev_[pb]));
+      std::memcpy((char *)dst + poff, pin_[pb], pn);
+    };
+    for (size_t off = 0; off < bytes; off += kChunk, b ^= 1) {
+      const size_t n = std::min(kChunk, bytes - off);
+      if (used_[b]) QD_HIP(hipEventSynchronize(ev_[b]));
+      QD_HIP(hipMemcpyAsync(pin_[b], (const char *)src + off, n, hipMemcpyDeviceToHost, s));
+      QD_HIP(hipEventRecord(ev_[b], s));
+      used_[b] = true;
+      finish();
+      pb = b;
+      poff = off;
+      pn = n;
+    }
+    finish();
+  }
+
+ private:
+  static constexpr size_t kChunk = 32u << 20;
+  std::mutex mu_;
+  std::multimap<size_t, void *> free_;
+  void *pin_[2] = {nullptr, nullptr};
+  hipEvent_t ev_[2] = {nullptr, nullptr};
+  bool used_[2] = {false, false};
+  void pinned() {
+    for (int b = 0; b < 2; b++) {
+      if (!pin_[b]) QD_HIP(hipHostMalloc(&pin_[b], kChunk, hipHostMallocDefault));
+      if (!ev_[b]) QD_HIP(hipEventCreateWithFlags(&ev_[b], hipEventDisableTiming));
+    }
+  }
+};
 
 }  // namespace
 
 struct qd_server {
-  size_t budget = 0, used = 0;
+  size_t budget = 0, used = 0;   // guarded by mu
   int max_running = 0;
   int next_id = 1;
-  hipStream_t admit_stream = nullptr;
-  std::deque<ServerJob *> queue;
+  int active = 0;                // admitted and not yet DONE / FAILED (guarded by mu)
+  hipStream_t admit_stream = nullptr;   // the worker's stream
+  int device = 0;
+  std::deque<ServerJob *> queue;        // submitted, not admitted (stepping thread)
+  ServerMemory memory;                  // device memory and staging of the served jobs (outlives jobs)
   std::map<int, std::unique_ptr<ServerJob>> jobs;
   std::vector<std::unique_ptr<ShapeClass>> classes;
   std::string err;
+
+  std::mutex mu;
+  std::condition_variable cv;
+  // held by the worker for each task and by the stepping thread while it
+  // captures a step graph: HIP calls of the worker (allocation, copies,
+  // frees) during a capture on another thread have invalidated it
+  std::mutex hip_mu;
+  std::deque<Task> tasks;               // for the worker
+  std::vector<ServerJob *> built;       // built by the worker, not yet in a class
+  std::vector<ServerJob *> requeue;     // builds that ran out of device memory: back to the queue
+  bool stop = false;
+  int pending = 0;                      // tasks queued or running in the worker
+  std::thread worker;
 
   ShapeClass &class_for(const DeviceJob &j) {
     const JobDev &d = j.dev;
@@ -91,75 +222,100 @@ struct qd_server {
     return *classes.back();
   }
 
-  int running() const {
-    int n = 0;
-    for (auto &c : classes) n += (int)c->members.size();
-    return n;
+  int running() {
+    std::lock_guard<std::mutex> lk(mu);
+    return active;
   }
 
-  void fail_job(ServerJob *sj, const std::string &msg) {
+  // caller holds mu
+  void fail_job_locked(ServerJob *sj, const std::string &msg) {
     sj->state = QD_JOB_FAILED;
     sj->err = msg;
     sj->t_done = now_s();
   }
 
-  void admit() {
-    int admitted = 0;
-    for (auto it = queue.begin(); it != queue.end();) {
-      ServerJob *sj = *it;
-      if (max_running > 0 && running() >= max_running) break;
-      if (admitted >= kAdmitPerStep) break;
-      if (used + sj->estimate > budget) {   // backfill: a later, smaller job may fit
-        ++it;
-        continue;
+  void push_task(const Task &t) {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      tasks.push_back(t);
+      pending++;
+    }
+    cv.notify_all();
+  }
+
+  // ---- worker thread
+  void worker_main() {
+    (void)hipSetDevice(device);
+    for (;;) {
+      Task t;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return stop || !tasks.empty(); });
+        if (tasks.empty()) return;   // stop requested and nothing left
+        t = tasks.front();
+        tasks.pop_front();
       }
-      try {
-        sj->job.reset(new DeviceJob(sj->in, sj->shape, admit_stream));
-      } catch (const Error &e) {
-        sj->job.reset();
-        if (e.code == QD_ERR_NOMEM && running() > 0) {   // the estimate was short: retry later
-          ++it;
-          continue;
-        }
-        fail_job(sj, e.what());
-        it = queue.erase(it);
-        continue;
+      {
+        std::lock_guard<std::mutex> hl(hip_mu);
+        if (t.kind == Task::BUILD) build(t.sj);
+        else retire_job(t.sj, t.after);
       }
-      sj->charged = std::max(sj->estimate, sj->job->footprint);
-      used += sj->charged;
-      sj->state = QD_JOB_RUNNING;
-      sj->t_admit = now_s();
-      admitted++;
-      ShapeClass &c = class_for(*sj->job);
-      c.members.push_back(sj);
-      c.dirty = true;
-      it = queue.erase(it);
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        pending--;
+      }
+      cv.notify_all();
     }
   }
 
-  // drain the observable ring of every member, check the norm; class stream idle
-  void drain_class(ShapeClass &c) {
-    QD_HIP(hipStreamSynchronize(c.stream));
-    for (ServerJob *sj : c.members) {
-      if (sj->state != QD_JOB_RUNNING) continue;
-      DeviceJob &j = *sj->job;
-      j.drain(sj->enqueued);
-      const double tol = j.shape.mixed ? 1e-6 : 1e-10;
-      const int nO = j.shape.nO;
-      for (int r = sj->checked; r < j.records_done; r++) {
-        double tn = 0.0;
-        for (int b = 0; b < nO; b++) tn += j.obs[(size_t)r * nO * 5 + b * 5];
-        if (!(tn <= 1.0 + tol)) {
-          char buf[160];
-          std::snprintf(buf, sizeof buf, "total norm %.17g exceeds 1 at step %d (cudaOpenMP2.cu:437-440)", tn,
-                        j.shape.steps0 + r);
-          *sj->in->time.steps = j.shape.steps0 + r + 1;
-          fail_job(sj, buf);
-          break;
-        }
+  void build(ServerJob *sj) {
+    try {
+      std::unique_ptr<DeviceJob> j(new DeviceJob(sj->in, sj->shape, admit_stream, &memory));
+      std::lock_guard<std::mutex> lk(mu);
+      sj->job = std::move(j);
+      used += sj->job->footprint > sj->charged ? sj->job->footprint - sj->charged : 0;
+      sj->charged = std::max(sj->charged, sj->job->footprint);
+      sj->t_admit = now_s();
+      built.push_back(sj);
+    } catch (const Error &e) {
+      std::lock_guard<std::mutex> lk(mu);
+      used -= std::min(used, sj->charged);
+      sj->charged = 0;
+      active--;
+      if (e.code == QD_ERR_NOMEM && active > 0) {   // the estimate was short: retry later
+        sj->state = QD_JOB_QUEUED;
+        requeue.push_back(sj);
+      } else {
+        fail_job_locked(sj, e.what());
       }
-      if (sj->state == QD_JOB_RUNNING) sj->checked = j.records_done;
+    } catch (const std::exception &e) {
+      std::lock_guard<std::mutex> lk(mu);
+      used -= std::min(used, sj->charged);
+      sj->charged = 0;
+      active--;
+      fail_job_locked(sj, e.what());
     }
+  }
+
+  // records not yet norm-checked; returns false (job failed) on a norm past 1
+  bool check_norms(ServerJob *sj, std::string &msg) {
+    DeviceJob &j = *sj->job;
+    const double tol = j.shape.mixed ? 1e-6 : 1e-10;
+    const int nO = j.shape.nO;
+    for (int r = sj->checked; r < j.records_done; r++) {
+      double tn = 0.0;
+      for (int b = 0; b < nO; b++) tn += j.obs[(size_t)r * nO * 5 + b * 5];
+      if (!(tn <= 1.0 + tol)) {
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "total norm %.17g exceeds 1 at step %d (cudaOpenMP2.cu:437-440)", tn,
+                      j.shape.steps0 + r);
+        *sj->in->time.steps = j.shape.steps0 + r + 1;
+        msg = buf;
+        return false;
+      }
+    }
+    sj->checked = j.records_done;
+    return true;
   }
 
   void write_outputs(ServerJob *sj, hipStream_t s) {
@@ -175,29 +331,93 @@ struct qd_server {
     if (j.shape.crp) j.flux_to_crp(in->crp.CRP, s);
   }
 
-  // retire finished and failed members of a class (class stream idle)
-  void retire(ShapeClass &c) {
-    std::vector<ServerJob *> keep;
-    for (ServerJob *sj : c.members) {
-      const bool finished = sj->state == QD_JOB_RUNNING && sj->enqueued >= sj->target;
-      if (finished) {
-        try {
-          write_outputs(sj, c.stream);
-          sj->state = QD_JOB_DONE;
-          sj->t_done = now_s();
-        } catch (const Error &e) {
-          fail_job(sj, e.what());
-        }
-      }
-      if (sj->state == QD_JOB_RUNNING) {
-        keep.push_back(sj);
+  // after the job's last step (event `after` on its class stream): drain and
+  // check its records, write its outputs, free its device memory
+  void retire_job(ServerJob *sj, hipEvent_t after) {
+    std::string msg;
+    bool ok = true;
+    try {
+      QD_HIP(hipStreamWaitEvent(admit_stream, after, 0));
+      QD_HIP(hipEventSynchronize(after));
+      sj->job->drain(sj->enqueued);
+      ok = check_norms(sj, msg);
+      if (ok) write_outputs(sj, admit_stream);
+    } catch (const std::exception &e) {
+      ok = false;
+      msg = e.what();
+    }
+    (void)hipEventDestroy(after);
+    {
+      // the outputs are in place: the job is done before its memory is
+      // freed (hipFree may wait for the device)
+      std::lock_guard<std::mutex> lk(mu);
+      if (ok) {
+        sj->state = QD_JOB_DONE;
+        sj->t_done = now_s();
       } else {
-        sj->job.reset();
-        used -= std::min(used, sj->charged);
-        c.dirty = true;
+        fail_job_locked(sj, msg);
       }
     }
-    c.members.swap(keep);
+    sj->job.reset();
+    std::lock_guard<std::mutex> lk(mu);
+    used -= std::min(used, sj->charged);
+    sj->charged = 0;
+    active--;
+  }
+
+  // ---- stepping thread
+  void admit() {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      for (ServerJob *sj : requeue) queue.push_front(sj);
+      requeue.clear();
+    }
+    int admitted = 0;
+    for (auto it = queue.begin(); it != queue.end();) {
+      ServerJob *sj = *it;
+      if (admitted >= kAdmitPerStep) break;
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        if (max_running > 0 && active >= max_running) break;
+        if (used + sj->estimate > budget) {   // backfill: a later, smaller job may fit
+          ++it;
+          continue;
+        }
+        used += sj->estimate;
+        sj->charged = sj->estimate;
+        sj->state = QD_JOB_RUNNING;
+        active++;
+      }
+      admitted++;
+      push_task({Task::BUILD, sj, nullptr});
+      it = queue.erase(it);
+    }
+  }
+
+  // drain the observable ring of every member, check the norm (class stream
+  // idle); a job whose norm passed 1 fails and is handed to the worker
+  void drain_class(ShapeClass &c) {
+    QD_HIP(hipStreamSynchronize(c.stream));
+    for (ServerJob *sj : c.members) {
+      sj->job->drain(sj->enqueued);
+      std::string msg;
+      if (!check_norms(sj, msg)) {
+        std::lock_guard<std::mutex> lk(mu);
+        sj->err = msg;
+        sj->target = sj->enqueued;   // retire it now; the worker reports the failure
+      }
+    }
+  }
+
+  hipEvent_t class_event(ShapeClass &c) {
+    hipEvent_t e;
+    if (!c.spare.empty()) {
+      e = c.spare.back();
+      c.spare.pop_back();
+    } else {
+      QD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    return e;
   }
 
   // max_steps >= Engine::kGraphSteps lets a class advance by kGraphSteps
@@ -205,12 +425,29 @@ struct qd_server {
   // flux window within them (the ring holds kDrainEvery + kGraphSteps)
   void step(int max_steps) {
     admit();
-    // enqueue the steps of every class first, so the classes' streams run
-    // concurrently; then drain / retire where needed
-    std::vector<ShapeClass *> settle;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      // with no class to step, wait for the worker to deliver a job (or to
+      // finish what it holds), so that every call makes progress
+      bool idle = true;
+      for (auto &c : classes) idle &= c->members.empty();
+      if (idle) cv.wait(lk, [&] { return !built.empty() || pending == 0; });
+      for (ServerJob *sj : built) {
+        ShapeClass &c = class_for(*sj->job);
+        c.members.push_back(sj);
+        c.dirty = true;
+      }
+      built.clear();
+    }
     for (auto &cp : classes) {
       ShapeClass &c = *cp;
       if (c.members.empty()) continue;
+      // bound the steps queued ahead on this class's stream
+      while ((int)c.in_flight.size() >= kMaxInFlight) {
+        QD_HIP(hipEventSynchronize(c.in_flight.front()));
+        c.spare.push_back(c.in_flight.front());
+        c.in_flight.pop_front();
+      }
       if (c.dirty) {
         std::vector<DeviceJob *> js;
         for (ServerJob *sj : c.members) js.push_back(sj->job.get());
@@ -224,22 +461,53 @@ struct qd_server {
         for (ServerJob *sj : c.members)
           if (sj->target - sj->enqueued < k || sj->job->steps_to_flux_flush() < k) k = 1;
       }
-      c.engine->step_graph(k);
-      bool finishing = false, drain = false;
+      if (c.engine->graph_ready(k)) {
+        c.engine->step_graph(k);
+      } else {
+        // capture only while the worker is between tasks; otherwise launch
+        // the steps directly and capture on a later call
+        std::unique_lock<std::mutex> hl(hip_mu, std::try_to_lock);
+        if (hl.owns_lock()) c.engine->step_graph(k);
+        else
+          for (int i = 0; i < k; i++) c.engine->step();
+      }
+      bool drain = false;
       for (ServerJob *sj : c.members) {
         DeviceJob &j = *sj->job;
         j.steps += k;
         sj->enqueued += k;
         j.flux_after_step(c.stream);
-        finishing |= sj->enqueued >= sj->target;
-        drain |= sj->enqueued - j.records_done >= kDrainEvery;
+        drain |= sj->enqueued < sj->target && sj->enqueued - j.records_done >= kDrainEvery;
       }
-      if (finishing || drain) settle.push_back(&c);
+      hipEvent_t e = class_event(c);
+      QD_HIP(hipEventRecord(e, c.stream));
+      c.in_flight.push_back(e);
+      if (drain) drain_class(c);
+      // finished jobs leave the class; the worker retires them once the
+      // stream has passed their last step
+      std::vector<ServerJob *> keep;
+      for (ServerJob *sj : c.members) {
+        if (sj->enqueued >= sj->target) {
+          hipEvent_t done;
+          QD_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+          QD_HIP(hipEventRecord(done, c.stream));
+          push_task({Task::RETIRE, sj, done});
+          c.dirty = true;
+        } else {
+          keep.push_back(sj);
+        }
+      }
+      c.members.swap(keep);
     }
-    for (ShapeClass *c : settle) {
-      drain_class(*c);
-      retire(*c);
+  }
+
+  void shutdown() {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      stop = true;
     }
+    cv.notify_all();
+    if (worker.joinable()) worker.join();
   }
 };
 
@@ -279,6 +547,9 @@ qd_status qd_server_create(int64_t hbm_budget_bytes, int32_t max_running, qd_ser
     s->budget = hbm_budget_bytes > 0 ? (size_t)hbm_budget_bytes : (size_t)(0.9 * (double)free_b);
     s->max_running = max_running > 0 ? max_running : 0;
     QD_HIP(hipStreamCreateWithFlags(&s->admit_stream, hipStreamNonBlocking));
+    QD_HIP(hipGetDevice(&s->device));
+    qd_server *sp = s.get();
+    s->worker = std::thread([sp] { sp->worker_main(); });
     *out = s.release();
     return QD_OK;
   })
@@ -286,10 +557,15 @@ qd_status qd_server_create(int64_t hbm_budget_bytes, int32_t max_running, qd_ser
 
 void qd_server_destroy(qd_server *s) {
   if (!s) return;
+  s->shutdown();   // the worker finishes its queued builds and retirements first
   for (auto &c : s->classes)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (auto &c : s->classes) c->engine.reset();
-  s->jobs.clear();   // frees the device jobs
+  for (auto &c : s->classes) {
+    c->engine.reset();
+    for (hipEvent_t e : c->in_flight) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->spare) (void)hipEventDestroy(e);
+  }
+  s->jobs.clear();   // frees the device jobs still held
   for (auto &c : s->classes)
     if (c->stream) (void)hipStreamDestroy(c->stream);
   if (s->admit_stream) (void)hipStreamDestroy(s->admit_stream);
@@ -332,8 +608,13 @@ qd_status qd_server_advance(qd_server *s, int32_t max_steps, int32_t *n_running,
   if (max_steps < 1) return sfail(s, QD_ERR_INVALID, "max_steps must be >= 1");
   QD_SERVER_GUARD(s, {
     s->step(max_steps);
+    int nq;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      nq = (int)(s->queue.size() + s->requeue.size());
+    }
     if (n_running) *n_running = s->running();
-    if (n_queued) *n_queued = (int32_t)s->queue.size();
+    if (n_queued) *n_queued = nq;
     return QD_OK;
   })
 }
@@ -344,6 +625,7 @@ qd_status qd_server_job_status(qd_server *s, int32_t job_id, int32_t *state, int
   auto it = s->jobs.find(job_id);
   if (it == s->jobs.end()) return sfail(s, QD_ERR_INVALID, "unknown job id");
   const ServerJob &sj = *it->second;
+  std::lock_guard<std::mutex> lk(s->mu);
   if (state) *state = sj.state;
   if (steps_done) *steps_done = sj.enqueued;
   if (latency_s) *latency_s = (sj.t_done >= 0 ? sj.t_done : now_s()) - sj.t_submit;
@@ -354,11 +636,14 @@ qd_status qd_server_job_status(qd_server *s, int32_t job_id, int32_t *state, int
 const char *qd_server_job_error(qd_server *s, int32_t job_id) {
   if (!s) return "";
   auto it = s->jobs.find(job_id);
-  return it == s->jobs.end() ? "unknown job id" : it->second->err.c_str();
+  if (it == s->jobs.end()) return "unknown job id";
+  std::lock_guard<std::mutex> lk(s->mu);
+  return it->second->err.c_str();
 }
 
 qd_status qd_server_memory(qd_server *s, int64_t *used, int64_t *budget) {
   if (!s) return sfail(nullptr, QD_ERR_INVALID, "server is NULL");
+  std::lock_guard<std::mutex> lk(s->mu);
   if (used) *used = (int64_t)s->used;
   if (budget) *budget = (int64_t)s->budget;
   return QD_OK;
