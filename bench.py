@@ -191,6 +191,16 @@ def other_configs(api, abi, inputs, cpu_threads, with_cpu):
             rec[prec_name] = {"steps": gpu_steps, "ms_per_step": el / gpu_steps * 1e3,
                               "grid_point_steps_per_s": pts * gpu_steps / el}
         rec["fp64"]["model_frac"] = step_model_seconds(job) / (rec["fp64"]["ms_per_step"] * 1e-3)
+        if name == "c1":
+            # the latency-bound small job batched: 16 copies stepped together by
+            # the server (one launch per kernel for all 16)
+            from qd_openmp_cuda_amd import serve
+            copies, steps16 = 16, 400
+            prepared = [(0.0, abi.Inputs(job, total_steps=steps16, want_acf=False, final_copy=False,
+                                         calculate_crp=False)) for _ in range(copies)]
+            r = serve.run_stream(prepared, time_scale=0.0)
+            rec["fp64_served_x16"] = {"jobs": copies, "steps": steps16, "wall_s": r["wall_s"],
+                                      "grid_point_steps_per_s": r["grid_point_steps_per_s"]}
         if with_cpu:
             from oracle import oracle
             if cpu_warm:
@@ -259,6 +269,9 @@ def main():
                     help="only run this serving case and print its figures (profiling)")
     ap.add_argument("--workload", default="c3", choices=("c1", "c2", "c3"),
                     help="single-job workload of the timed steps (profiling; the headline is c3)")
+    ap.add_argument("--shape", default="", help="profiling: N1xN2xNTHETA grid in place of the workload's")
+    ap.add_argument("--blocks", type=int, default=0,
+                    help="profiling: omega blocks (J = 2 (blocks - 1), K_max = blocks - 1) with --shape")
     ap.add_argument("--bandwidth", type=int, default=0,
                     help="only run the HBM ceiling probes over this many MiB and exit")
     ap.add_argument("--no-serve", action="store_true")
@@ -311,6 +324,11 @@ def main():
         return
 
     spec = inputs.config_spec(args.workload)
+    if args.shape:
+        n1, n2, nt = (int(v) for v in args.shape.lower().split("x"))
+        nb = max(args.blocks, 1)
+        spec = dataclasses.replace(spec, n1=n1, n2=n2, ntheta=nt, J=2 * (nb - 1), parity=0, k_max=nb - 1,
+                                   R0=9.0, seed=None)
     job = inputs.build_job(spec)
     model, pts = kernel_model(job)
     print(f"# workload {args.workload}: {spec.n1}x{spec.n2}x{spec.ntheta}, J={spec.J}, K_max={spec.k_max}, "

@@ -139,6 +139,38 @@ __host__ __device__ constexpr int ct_rows(int n1, int nb, bool inv) {
 // its radix plan have 128..256 butterflies for the 256 threads.
 __host__ __device__ constexpr int ct_cols(int n2) { return (n2 <= 64 || n2 == 128) ? 16 : 8; }
 
+// First stage of the Coriolis mix for the XPT points of a thread and NB
+// blocks, in place: t_a(k) = phi(k, a) sum_b V[a][b] x_b(k).  Every entry
+// V[a][b] (a, b < nO; the eigenvector table is zero past the blocks present
+// at the slab) is read once and applied to all points, so one entry is live
+// at a time where the per-point form keeps all NB^2 of them in registers.
+template <int NB, int XPT, class VPtr, class Phi>
+__device__ __forceinline__ void mix_first_stage(double2 (&x)[NB][XPT], VPtr V, int nO, Phi phi) {
+  const int nOc = nO - 1;
+  double2 t[NB][XPT];
+#pragma unroll
+  for (int al = 0; al < NB; al++) {
+    double2 acc[XPT];
+#pragma unroll
+    for (int k = 0; k < XPT; k++) acc[k] = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      const double v = V[min(al, nOc) * nO + min(b, nOc)];
+#pragma unroll
+      for (int k = 0; k < XPT; k++) {
+        acc[k].x = fma(v, x[b][k].x, acc[k].x);
+        acc[k].y = fma(v, x[b][k].y, acc[k].y);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < XPT; k++) t[al][k] = cmul(phi(k, al), acc[k]);
+  }
+#pragma unroll
+  for (int al = 0; al < NB; al++)
+#pragma unroll
+    for (int k = 0; k < XPT; k++) x[al][k] = t[al][k];
+}
+
 // ---------------------------------------------------------------------------
 // R pass with the Coriolis mix (forward: mix then DFT; inverse: DFT, scale,
 // energies, mix)
@@ -184,6 +216,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
   // reloaded after every store of the mix).  The forward mix has no stores
   // and reads them from global memory, where the compiler keeps them scalar.
   constexpr bool VLDS = INV && NB > 5;
+  constexpr bool kMixBatched = INV && NB > 5 && NB <= 9 && N1 >= 256;
   __shared__ double Vs[VLDS ? NB * NB : 1];
   __shared__ double2 cms[VLDS ? NB : 1];
   const int job = locate_job(a.unit_base, a.n_jobs, blockIdx.x);
@@ -244,17 +277,19 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
   FbrSums<NB, NF> sm;
   sm.zero();
   if (!INV) {
+    {
 #pragma unroll
-    for (int k = 0; k < XPT; k++) {
-      const int e = ept(k);
-      if (eok(k)) {
-        double2 xp[NB];
+      for (int k = 0; k < XPT; k++) {
+        const int e = ept(k);
+        if (eok(k)) {
+          double2 xp[NB];
 #pragma unroll
-        for (int b = 0; b < NB; b++) xp[b] = x[b][k];
-        coriolis_point<false, NB, NF>(jb, xp, nb, l, e % N1, g * P + e / N1, V, sm, nullptr, LM,
-                                      (const double2 *)phs + (e % N1) * NB);
+          for (int b = 0; b < NB; b++) xp[b] = x[b][k];
+          coriolis_point<false, NB, NF>(jb, xp, nb, l, e % N1, g * P + e / N1, V, sm, nullptr, LM,
+                                        (const double2 *)phs + (e % N1) * NB);
 #pragma unroll
-        for (int b = 0; b < NB; b++) x[b][k] = xp[b];
+          for (int b = 0; b < NB; b++) x[b][k] = xp[b];
+        }
       }
     }
 #pragma unroll
@@ -335,27 +370,37 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
       if (b > 0) coriolis_energy(b - 1);
     }
     coriolis_energy(NB - 1);
-    // first stage, per point: t_a = phi(l, i, a) sum_b V_l[a][b] x_b, in place
+    // first stage, per point: t_a = phi(l, i, a) sum_b V_l[a][b] x_b, in
+    // place; on 256-point pencils with 6 to 9 blocks in the form that reads
+    // each entry of V once for all points of the thread (on 9 blocks of
+    // 256 x 192 x 96: 465 -> 383 us; on 192-point pencils it costs a wave per
+    // SIMD and no time is gained)
+    if constexpr (!kMixBatched) {
 #pragma unroll
-    for (int k = 0; k < XPT; k++) {
-      const int e = ept(k);
-      if (eok(k)) {
-        const double2 *ph = (const double2 *)phs + (e % N1) * NB;
-        double2 t[NB];
+      for (int k = 0; k < XPT; k++) {
+        const int e = ept(k);
+        if (eok(k)) {
+          const double2 *ph = (const double2 *)phs + (e % N1) * NB;
+          double2 t[NB];
 #pragma unroll
-        for (int al = 0; al < NB; al++) {
-          double2 acc = make_double2(0.0, 0.0);
+          for (int al = 0; al < NB; al++) {
+            double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
-          for (int b = 0; b < NB; b++) {
-            const double v = V[min(al, nOc) * jb.nO + min(b, nOc)];
-            acc.x = fma(v, x[b][k].x, acc.x);
-            acc.y = fma(v, x[b][k].y, acc.y);
+            for (int b = 0; b < NB; b++) {
+              const double v = V[min(al, nOc) * jb.nO + min(b, nOc)];
+              acc.x = fma(v, x[b][k].x, acc.x);
+              acc.y = fma(v, x[b][k].y, acc.y);
+            }
+            t[al] = cmul(ph[al], acc);
           }
-          t[al] = cmul(ph[al], acc);
-        }
 #pragma unroll
-        for (int al = 0; al < NB; al++) x[al][k] = t[al];
+          for (int al = 0; al < NB; al++) x[al][k] = t[al];
+        }
       }
+    } else {
+      mix_first_stage<NB, XPT>(x, V, jb.nO, [&](int k, int al) {
+        return ((const double2 *)phs)[(ept(k) % N1) * NB + al];
+      });
     }
     // second stage, block by block: y_b = rot2(l, j) sum_a V_l[a][b] t_a,
     // stored; rotational energy l(l+1)(I1 + I2)|y|^2, |y|^2 and conj(c0) y.
@@ -389,16 +434,35 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
 #pragma unroll
           for (int k = 0; k < XPT; k++) z[k] = c0[b * LM + (eok(k) ? ept(k) : 0)];
         }
+        // sum_a V[a][b] t_a for every point (batched: each entry of V read once)
+        double2 accs[kMixBatched ? XPT : 1];
+        if constexpr (kMixBatched) {
+#pragma unroll
+          for (int k = 0; k < XPT; k++) accs[k] = make_double2(0.0, 0.0);
+#pragma unroll
+          for (int al = 0; al < NB; al++) {
+            const double v = V[min(al, nOc) * nO + b];
+#pragma unroll
+            for (int k = 0; k < XPT; k++) {
+              accs[k].x = fma(v, x[al][k].x, accs[k].x);
+              accs[k].y = fma(v, x[al][k].y, accs[k].y);
+            }
+          }
+        }
 #pragma unroll
         for (int k = 0; k < XPT; k++) {
           const int e = ept(k);
           if (eok(k)) {
             double2 acc = make_double2(0.0, 0.0);
+            if constexpr (kMixBatched) {
+              acc = accs[k];
+            } else {
 #pragma unroll
-            for (int al = 0; al < NB; al++) {
-              const double v = V[min(al, nOc) * nO + b];
-              acc.x = fma(v, x[al][k].x, acc.x);
-              acc.y = fma(v, x[al][k].y, acc.y);
+              for (int al = 0; al < NB; al++) {
+                const double v = V[min(al, nOc) * nO + b];
+                acc.x = fma(v, x[al][k].x, acc.x);
+                acc.y = fma(v, x[al][k].y, acc.y);
+              }
             }
             const double2 y = cmul(rv[k], acc);
             base[b * LM + e] = y;
