@@ -914,24 +914,34 @@ bool leg_tiles_supported(int nt_tiles) {
   }
 }
 
-bool leg_paired_tiles(int nt_tiles) { return leg_paired() && nt_tiles <= 4; }
+// Paired-row kernels: up to 4 column tiles in both passes; the FP64 forward
+// pass also at 5 and 6 tiles for jobs of two or more blocks (9 blocks of
+// 256 x 192 x 96: 373 -> 340 us).  One block of 256 x 192 x 96 has too few
+// strips of 16 points to fill the CUs (59 against 55 us), the backward
+// pass spills beyond 4 tiles and so does the mixed forward pass.
+bool leg_fwd_paired(int nt_tiles, bool mixed, int n_blocks) {
+  return leg_paired() && (nt_tiles <= 4 || (!mixed && nt_tiles <= 6 && n_blocks >= 2));
+}
+bool leg_bwd_paired(int nt_tiles) { return leg_paired() && nt_tiles <= 4; }
 
-void launch_leg_fwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, int waves,
+void launch_leg_fwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, int waves, int paired,
                     hipStream_t s) {
-  if (leg_paired_tiles(nt_tiles)) {
+  if (paired) {
     switch (nt_tiles) {
       case 1: launch_leg_fwd_paired_nt<1>(a, total_units, kp_max, mixed, waves, s); return;
       case 2: launch_leg_fwd_paired_nt<2>(a, total_units, kp_max, mixed, waves, s); return;
       case 3: launch_leg_fwd_paired_nt<3>(a, total_units, kp_max, mixed, waves, s); return;
       case 4: launch_leg_fwd_paired_nt<4>(a, total_units, kp_max, mixed, waves, s); return;
+      case 5: launch_leg_fwd_paired_nt<5>(a, total_units, kp_max, mixed, waves, s); return;
+      case 6: launch_leg_fwd_paired_nt<6>(a, total_units, kp_max, mixed, waves, s); return;
       default: break;
     }
   }
   launch_leg<true>(a, total_units, nt_tiles, kp_max, mixed, waves, s);
 }
-void launch_leg_bwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, int waves,
+void launch_leg_bwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, int waves, int paired,
                     hipStream_t s) {
-  if (leg_paired_tiles(nt_tiles)) {   // 5 and 6 tiles spill in 8-wave workgroups
+  if (paired) {   // 5 and 6 tiles spill in 8-wave workgroups
     switch (nt_tiles) {
       case 1: launch_leg_bwd_paired_nt<1>(a, total_units, kp_max, mixed, waves, s); return;
       case 2: launch_leg_bwd_paired_nt<2>(a, total_units, kp_max, mixed, waves, s); return;
@@ -1022,7 +1032,7 @@ void init_attrs_gemm() {
   set_leg_attrs<10>(lim); set_leg_attrs<12>(lim); set_leg_attrs<15>(lim);
   (void)hipFuncSetAttribute((const void *)k_gemm_probe<8>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
   set_leg_paired_attrs<1>(lim); set_leg_paired_attrs<2>(lim); set_leg_paired_attrs<3>(lim);
-  set_leg_paired_attrs<4>(lim);
+  set_leg_paired_attrs<4>(lim); set_leg_paired_attrs<5>(lim); set_leg_paired_attrs<6>(lim);
 }
 
 }  // namespace qd

@@ -323,8 +323,10 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st, De
   // CUs (2 four-wave or 1 eight-wave workgroup per CU): each wave then runs
   // many pipelined tiles after paying the B staging and pipeline fill once.
   // the paired-row passes take strips of 16 complex points (32 rows)
-  const bool fwd_paired = leg_paired_tiles(d.fwd_np / 16);
-  const bool bwd_paired = leg_paired_tiles(d.bwd_np / 16);
+  const bool fwd_paired = leg_fwd_paired(d.fwd_np / 16, sh.mixed, nO);
+  const bool bwd_paired = leg_bwd_paired(d.bwd_np / 16);
+  d.leg_fwd_paired = fwd_paired ? 1 : 0;
+  d.leg_bwd_paired = bwd_paired ? 1 : 0;
   const long fwd_strips = fwd_paired ? ((long)M + 15) / 16 : strips;
   const long bwd_strips = bwd_paired ? ((long)M + 15) / 16 : strips;
   auto pick_waves = [&](int kp, int np, long nstrips) {
@@ -736,8 +738,7 @@ void Engine::set_jobs(const std::vector<DeviceJob *> &jobs) {
                "all jobs of one engine must share the GEMM workgroup size");
     // the strips of a job (16 points when paired, 16 rows otherwise) follow
     // its tile counts, and one launch runs one kernel for every job
-    QD_REQUIRE(leg_paired_tiles(d.fwd_np / 16) == leg_paired_tiles(jobs[0]->dev.fwd_np / 16) &&
-                   leg_paired_tiles(d.bwd_np / 16) == leg_paired_tiles(jobs[0]->dev.bwd_np / 16),
+    QD_REQUIRE(d.leg_fwd_paired == jobs[0]->dev.leg_fwd_paired && d.leg_bwd_paired == jobs[0]->dev.leg_bwd_paired,
                "all jobs of one engine must share the Legendre kernel form (paired or one-row strips)");
     descs[q] = d;
     for (int k = 0; k < KK_COUNT; k++) bases[(size_t)k * (n + 1) + q + 1] = bases[(size_t)k * (n + 1) + q] + d.units[k];
@@ -785,7 +786,8 @@ void Engine::enqueue(hipEvent_t *ev) {
     if (ev) QD_HIP(hipEventRecord(ev[2 * k + 1], s));
   };
   begin(QD_K_LEG_FWD);
-  launch_leg_fwd(args(KK_LEG_FWD), total_[KK_LEG_FWD], fwd_tiles_, fwd_kp_, mixed_, fwd_waves_, s);
+  launch_leg_fwd(args(KK_LEG_FWD), total_[KK_LEG_FWD], fwd_tiles_, fwd_kp_, mixed_, fwd_waves_,
+                 jobs_[0]->dev.leg_fwd_paired, s);
   end(QD_K_LEG_FWD);
   begin(QD_K_FFT_R_FWD);
   if (ct_n1_) launch_fft_R_ct(args(KK_FFT_R), total_[KK_FFT_R], 0, 0, ct_n1_, nb_max_, s);
@@ -800,7 +802,8 @@ void Engine::enqueue(hipEvent_t *ev) {
   else launch_fft_R_rt(args(KK_FFT_RI), total_[KK_FFT_RI], 1, any_acf_, nb_max_, lds_fftR_, s);
   end(QD_K_FFT_R_INV);
   begin(QD_K_LEG_BWD);
-  launch_leg_bwd(args(KK_LEG_BWD), total_[KK_LEG_BWD], bwd_tiles_, bwd_kp_, mixed_, bwd_waves_, s);
+  launch_leg_bwd(args(KK_LEG_BWD), total_[KK_LEG_BWD], bwd_tiles_, bwd_kp_, mixed_, bwd_waves_,
+                 jobs_[0]->dev.leg_bwd_paired, s);
   end(QD_K_LEG_BWD);
   if (any_flux_) {
     begin(QD_K_OBS);

@@ -53,6 +53,7 @@ struct JobDev {
   int gemm_strips, gemm_bwd_strips;   // strips per wave in the forward / backward Legendre GEMM
                                       // (16 rows, or 16 complex points in the paired-row passes)
   int gemm_fwd_waves, gemm_bwd_waves;   // waves per GEMM workgroup (4, or 8 to share a large B in LDS)
+  int leg_fwd_paired, leg_bwd_paired;   // 1: the Legendre pass runs the paired-row kernels
   int gemm_fwd_groups;       // workgroups per omega block (forward GEMM)
   int gemm_bwd_groups;       // workgroups per omega block (backward GEMM)
   int fwd_kp, fwd_np;        // padded K (mult of 4) and N (mult of 16), forward
@@ -196,13 +197,14 @@ void launch_vphase(double2 *vph, const double *pot, const double *d1, const doub
 bool leg_tiles_supported(int nt_tiles);
 // Legendre passes on paired rows (default; QD_LEG_PAIRED=0 turns them off)
 bool leg_paired();
-// whether a Legendre pass with nt_tiles column tiles runs the paired-row kernels
-bool leg_paired_tiles(int nt_tiles);
+// whether a job's forward / backward Legendre pass runs the paired-row kernels
+bool leg_fwd_paired(int nt_tiles, bool mixed, int n_blocks);
+bool leg_bwd_paired(int nt_tiles);
 // LDS bytes of a Legendre kernel for B of kp x np (np a multiple of 16)
 size_t leg_lds_bytes(int kp, int np, int mixed);
-void launch_leg_fwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, int waves,
+void launch_leg_fwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, int waves, int paired,
                     hipStream_t stream);
-void launch_leg_bwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, int waves,
+void launch_leg_bwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, int waves, int paired,
                     hipStream_t stream);
 // FFT along R fused with rotational . Coriolis . rotational (before the forward
 // transform, after the inverse one; the inverse pass also forms the energies),

@@ -35,8 +35,9 @@ double now_s() {
 // Jobs share a class when every kernel instantiation of the step is the same
 // for them: grid lengths (FFT kernels and tilings, including the R-pass rows,
 // which depend on whether a job has more than 5 omega blocks), Legendre tile
-// shapes and the precision.
-using ClassKey = std::tuple<int, int, int, int, int, int, int, int, int, int, int, int>;
+// shapes and kernel forms (workgroup waves, paired or one-row strips) and the
+// precision.
+using ClassKey = std::tuple<int, int, int, int, int, int, int, int, int, int, int, int, int, int>;
 
 struct ServerJob {
   int id = 0;
@@ -211,7 +212,8 @@ struct qd_server {
   ShapeClass &class_for(const DeviceJob &j) {
     const JobDev &d = j.dev;
     const ClassKey key{d.n1,      d.n2,     d.nt,       d.fwd_np,       d.bwd_np,           d.fwd_kp,
-                       d.bwd_kp,  d.mixed,  d.fft_rows, d.fft_rows_inv, d.gemm_fwd_waves,   d.gemm_bwd_waves};
+                       d.bwd_kp,  d.mixed,  d.fft_rows, d.fft_rows_inv, d.gemm_fwd_waves,   d.gemm_bwd_waves,
+                       d.leg_fwd_paired, d.leg_bwd_paired};
     for (auto &c : classes)
       if (c->key == key) return *c;
     std::unique_ptr<ShapeClass> c(new ShapeClass());

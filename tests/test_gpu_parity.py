@@ -122,6 +122,17 @@ def test_parity_nine_blocks():
     check_parity(a, b)
 
 
+def test_parity_nine_blocks_six_theta_tiles():
+    # 96 theta points and 9 blocks: the FP64 forward Legendre pass takes the
+    # paired-row kernel at 6 column tiles (multi-block jobs only), the
+    # backward pass the one-row kernel
+    spec = inputs.JobSpec(n1=96, n2=48, ntheta=96, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
+                          J=10, parity=0, k_max=8, omega0=0, j0=2, E_coll=0.02)
+    a, b = run_both(spec, 2, threads=16)
+    assert len(a.omegas) == 9
+    check_parity(a, b)
+
+
 def test_parity_nineteen_blocks():
     # 19 blocks: the widest R-pass instantiation (up to 24 blocks in registers and LDS)
     spec = inputs.JobSpec(n1=64, n2=32, ntheta=40, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
