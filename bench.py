@@ -58,7 +58,11 @@ def kernel_model(job):
     writes psi, 16 F_b + 16 G.  Total 40 G + 128 F_b.  Flops: the real-B
     Legendre product, 4 n1 n2 ntheta N_l per block and direction.  Kinds fused
     into these kernels (dvr, fbr1, fbr2) carry no traffic of their own; the
-    flux kernel runs only when reaction probabilities are requested."""
+    flux kernel runs only when reaction probabilities are requested.  The
+    fused Legendre step (leg_step: the backward pass of one step with the
+    forward pass of the next, psi kept on chip) reads the FBR state and V and
+    writes the FBR state, 8 G + 32 F_b, with both products' flops; a step
+    then moves 8 G + 128 F_b."""
     r1, r2, th, os_ = job["r1"], job["r2"], job["theta"], job["omega_states"]
     n1, n2, nt = r1["n"], r2["n"], th["n"]
     M = n1 * n2
@@ -81,6 +85,7 @@ def kernel_model(job):
         "leg_bwd": {"bytes": nO * 16 * G + 16 * Ft, "flops": gemm_flops},
         "flux": zero,
         "reduce": zero,
+        "leg_step": {"bytes": nO * 8 * G + 32 * Ft, "flops": 2 * gemm_flops},
     }, G * nO
 
 

@@ -219,7 +219,10 @@ enum {
   QD_K_LEG_BWD = 7,  /* backward Legendre transform (with the potential energy)    */
   QD_K_OBS = 8,      /* flux surface values (when CRP is requested)                */
   QD_K_REDUCE = 9,   /* final reduction of the per-step observables                */
-  QD_KERNEL_KINDS = 10
+  QD_K_LEG_STEP = 10,/* fused Legendre step: backward transform of the previous step,
+                        potential energy, DVR phase and forward transform of this one;
+                        where it runs, QD_K_LEG_FWD and QD_K_LEG_BWD read 0           */
+  QD_KERNEL_KINDS = 11
 };
 qd_status qd_profile_steps(qd_handle *h, int32_t n_steps, double *us);
 

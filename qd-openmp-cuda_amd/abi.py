@@ -16,11 +16,12 @@ QD_PREC_FP64, QD_PREC_MIXED = 0, 1
 QD_JOB_QUEUED, QD_JOB_RUNNING, QD_JOB_DONE, QD_JOB_FAILED = range(4)
 QD_OBS_FIELDS = 5
 OBS_NAMES = ("norm", "potential", "kinetic", "rotational", "coriolis")
-QD_KERNEL_KINDS = 10
+QD_KERNEL_KINDS = 11
 # kernel kinds of qd_profile_steps (dvr, fbr1 and fbr2 are fused into
-# leg_fwd, fft_R_fwd and fft_R_inv and read 0; flux runs only with CRP)
+# leg_fwd / leg_step, fft_R_fwd and fft_R_inv and read 0; flux runs only with
+# CRP; where the fused Legendre step runs, leg_fwd and leg_bwd read 0)
 KERNEL_NAMES = ("dvr", "leg_fwd", "fbr1", "fft_R_fwd", "fft_r", "fft_R_inv", "fbr2", "leg_bwd",
-                "flux", "reduce")
+                "flux", "reduce", "leg_step")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)

@@ -485,6 +485,8 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
       const double sum = (q < NV) ? (red[q] + red[NV + q]) + (red[2 * NV + q] + red[3 * NV + q]) : 0.0;
       pcor[(long)(5 * (q / NF) + q % NF) * ustride + u] = sum;
     }
+    // fused Legendre step: this step's backward transform is now pending
+    if (u == 0 && threadIdx.x == 0 && jb.leg_step) jb.counters[3] = 1;
   }
 }
 

@@ -77,8 +77,15 @@ class DeviceJob {
   JobDev dev{};          // descriptor (device pointers), uploaded by the engine
   size_t footprint = 0;  // bytes of device memory held
   int records_done = 0;  // records drained to the host
-  int steps = 0;         // host mirror of the device step counter
+  int steps = 0;         // host mirror of the device step counter: steps enqueued
   bool prephased = false;// step-0 V(-dt/2) phase applied
+  // host mirror of counters[3]: the last enqueued step ran the fused
+  // Legendre kernel's sequence and its backward transform, potential
+  // energy, flux values and record follow with the next step or finish
+  bool pending = false;
+  // observable records (and flux surface values) enqueued so far
+  int records_enqueued() const { return steps - shape.steps0 - (pending ? 1 : 0); }
+  int flux_steps() const { return steps - (pending ? 1 : 0); }
   std::vector<double> obs;      // [records][nO][5]
   std::vector<double> acf;      // [records][2]
 
@@ -87,11 +94,12 @@ class DeviceJob {
   void copy_psi(double *dst, hipStream_t s) const;
   // before the first step of a job whose step counter is 0: apply exp(+i dt/2 V)
   void prephase_if_first(hipStream_t s);
-  // flux: after each enqueued step (steps already advanced) fold a full
-  // window into the energy accumulators; flux_to_crp folds the open window
-  // and writes P(E) for every CRP energy (synchronous)
+  // flux: after each enqueued step or finish (steps and pending already
+  // updated) fold a full window into the energy accumulators; flux_to_crp
+  // folds the open window and writes P(E) for every CRP energy (synchronous,
+  // after the last step's finish)
   void flux_after_step(hipStream_t s);
-  // steps left before the flux window must be flushed (INT_MAX without flux)
+  // flux surface writes that still fit the open window (INT_MAX without flux)
   int steps_to_flux_flush() const;
   void flux_flush(hipStream_t s);
   void flux_to_crp(double *crp, hipStream_t s);
@@ -122,6 +130,12 @@ class Engine {
   // events) records ev[2k], ev[2k+1] around the launch of each kind k in
   // timed_kinds()
   void step(hipEvent_t *ev = nullptr);
+  // Whether the steps run the fused Legendre kernel (every job or none).
+  // Then a step leaves each job's backward transform pending, and finish
+  // runs it (the separate backward pass), with the flux surface values and
+  // the reduction of that last step, for every job (q < 0) or job q only.
+  bool fused() const { return fused_ != 0; }
+  void finish(int q = -1);
   unsigned timed_kinds() const;
   const std::vector<DeviceJob *> &jobs() const { return jobs_; }
   hipStream_t stream() const { return stream_; }
@@ -139,7 +153,7 @@ class Engine {
   hipStream_t stream_;
   std::vector<DeviceJob *> jobs_;
   JobDev *d_jobs_ = nullptr;
-  int *d_bases_ = nullptr;      // [KK_COUNT][n_jobs + 1]
+  int *d_bases_ = nullptr;      // [KK_COUNT][n_jobs + 1], then [n_jobs][KK_COUNT][2] for single jobs
   int n_alloc_jobs_ = 0;
   // host sources of the descriptor uploads, kept until upload_done_ passes
   std::vector<JobDev> descs_;
@@ -151,9 +165,12 @@ class Engine {
   int fwd_tiles_ = 0, bwd_tiles_ = 0, fwd_kp_ = 0, bwd_kp_ = 0, nb_max_ = 1;
   int fwd_waves_ = 4, bwd_waves_ = 4;   // waves per GEMM workgroup (the same for every job)
   int mixed_ = 0, any_flux_ = 0, any_acf_ = 0;
+  int fused_ = 0, step_waves_ = 4, step_ttiles_ = 0, step_ltiles_ = 0;
+  size_t lds_step_ = 0;
   hipGraph_t graph_[2] = {};           // [0] one step, [1] kGraphSteps steps
   hipGraphExec_t graph_exec_[2] = {};
-  StepArgs args(int kind) const;
+  StepArgs args(int kind, int gate = 0) const;
+  StepArgs args_one(int q, int kind) const;   // job q alone
   void enqueue(hipEvent_t *ev);
 };
 

@@ -144,7 +144,11 @@ __global__ __launch_bounds__(kBlock) void k_fft_R(StepArgs a) {
     double2 *dst = jb.fbr + ((long)b * L + l) * M + (long)g * PN;
     for (int e = threadIdx.x; e < PN; e += kBlock) dst[e] = X[b * PN + e];
   }
-  if (INV) store_fbr_partials<NB, NF>(jb, u, sm, red);
+  if (INV) {
+    store_fbr_partials<NB, NF>(jb, u, sm, red);
+    // fused Legendre step: this step's backward transform is now pending
+    if (u == 0 && threadIdx.x == 0 && jb.leg_step) jb.counters[3] = 1;
+  }
 }
 
 void launch_fft_R_rt(const StepArgs &a, int total_units, int inverse, int acf, int nb_max, size_t lds,

@@ -24,7 +24,8 @@ __global__ __launch_bounds__(kBlock) void k_flux_surface(StepArgs a) {
   const int q = locate_job(a.unit_base, a.n_jobs, u);
   const QD_G JobDev &jb = a.jobs[q];
   const int s = (u - a.unit_base[q]) * kBlock + threadIdx.x;
-  if (!jb.flux_on || s >= jb.flux_S) return;
+  // in the fused step sequence: only jobs whose last step is finishing now
+  if (!jb.flux_on || s >= jb.flux_S || (a.gate && jb.counters[3] == 0)) return;
   const int n1 = jb.n1, nt = jb.nt, M = jb.M, ng = jb.flux_ng, half = ng / 2;
   const int i = s % n1, k = (s / n1) % nt, b = s / (n1 * nt);
   const QD_G double2 *p = jb.psi + ((long)b * nt + k) * M + (long)(jb.flux_ns - half) * n1 + i;

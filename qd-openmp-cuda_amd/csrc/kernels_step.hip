@@ -107,14 +107,22 @@ void launch_wpot(double *wpot, const double *pot, const double *w, int nt, long 
 // record: no workgroup waits for another, so the kernel needs no
 // device-scope fence (each of which writes back an XCD's L2 on this chip).
 // Workgroup (0, 0) sets the pending-advance flag; one workgroup of the next
-// step's first kernel advances the step counters (advance_counters), so no
+// Legendre kernel advances the step counters (advance_counters), so no
 // workgroup here can read a counter that another has already advanced.
+// Gated (the fused step sequence): only jobs whose backward transform was
+// pending, i.e. ran in the fused kernel just before, have a step to record.
+// Ungated (after a separate backward pass): every job, and workgroup (0, 0)
+// clears the pending flag (no workgroup of an ungated launch reads it).
+// Jobs stepped with the fused kernel take their potential-energy partials
+// from its workgroups (fused or backward-only form), others from those of
+// the backward Legendre pass.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_reduce(StepArgs a) {
   __shared__ double red[4];
   const int job = locate_job(a.unit_base, a.n_jobs, blockIdx.x);
   const JobDev &jb = a.jobs[job];
   const int u = blockIdx.x - a.unit_base[job];
+  if (a.gate && jb.counters[3] == 0) return;   // uniform over the workgroup
   const int nO = jb.nO;
   const int b = u / kRedFields, f = u % kRedFields;
   const QD_G double *src;
@@ -131,7 +139,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce(StepArgs a) {
     n = jb.units[KK_FFT_RI];
     src = jb.part_cor + (5L * b + (f - 1)) * n;
   } else {        // potential energy partials of the backward Legendre epilogue: [b][group]
-    n = jb.gemm_bwd_groups;
+    n = jb.leg_step ? jb.step_groups : jb.gemm_bwd_groups;   // fused kernel (both forms) or backward pass
     src = jb.part_obs + (long)b * n;
   }
   // eight loads in flight per thread, summed in element order
@@ -163,7 +171,10 @@ __global__ __launch_bounds__(kBlock) void k_reduce(StepArgs a) {
       case 5: acf[1] = sum * area; break;
       default: rec[1] = sum * area; break;                 // potential
     }
-    if (u == 0) jb.counters[2] = 1;
+    if (u == 0) {
+      jb.counters[2] = 1;
+      if (!a.gate) jb.counters[3] = 0;
+    }
   }
 }
 

@@ -93,6 +93,7 @@ int pick_divisor(int n, const int *cands, int ncand, int limit_elems, int other)
 
 void init_attrs_fft();
 void init_attrs_gemm();
+void init_attrs_leg_step();
 
 void init_kernel_attributes() {
   // once per process, also when several host threads create handles at once
@@ -100,6 +101,7 @@ void init_kernel_attributes() {
   std::call_once(once, [] {
     init_attrs_fft();
     init_attrs_gemm();
+    init_attrs_leg_step();
     (void)hipGetLastError();   // an unsupported attribute must not leak into later error checks
   });
 }
@@ -352,6 +354,19 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st, De
                             ((long)d.gemm_fwd_waves * d.gemm_strips));
   d.gemm_bwd_groups = (int)((bwd_strips + (long)d.gemm_bwd_waves * d.gemm_bwd_strips - 1) /
                             ((long)d.gemm_bwd_waves * d.gemm_bwd_strips));
+  // Fused Legendre step (FP64, paired rows, both tables in LDS): its waves
+  // hold both products' accumulators and run one per SIMD, so one 4-wave
+  // workgroup per CU, all in one round
+  d.leg_step = (leg_step_supported(d.bwd_np / 16, d.fwd_np / 16, sh.mixed) &&
+                leg_step_lds_bytes(d.bwd_np / 16, d.fwd_np / 16) <= kLdsLimit) ? 1 : 0;
+  d.step_waves = 4;
+  {
+    const long pstrips = ((long)M + 15) / 16, slots = kCUs;
+    long S = std::max(1L, (pstrips * nO + d.step_waves * slots - 1) / (d.step_waves * slots));
+    while (nO * ((pstrips + d.step_waves * S - 1) / (d.step_waves * S)) > slots) S++;
+    d.step_strips = (int)S;
+    d.step_groups = (int)((pstrips + d.step_waves * S - 1) / (d.step_waves * S));
+  }
   static const int rows_c[] = {8, 4, 2, 1};
   static const int cols_c[] = {16, 8, 4, 2, 1};
   // compile-time FFT kernels where the lengths allow; otherwise the runtime-size
@@ -373,6 +388,7 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st, De
   d.units[KK_FFT_r] = (int)(slabs * (n1 / d.fft_cols));
   d.units[KK_REDUCE] = nO * kRedFields;
   d.units[KK_FLUX] = 0;
+  d.units[KK_LEG_STEP] = d.leg_step ? nO * d.step_groups : 0;
 
   // ---- host tables
   std::vector<double> r1(n1), r2(n2), I1(n1), I2(n2), kinE1(n1), kinE2(n2);
@@ -509,7 +525,8 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st, De
   d.twr = upload(twiddles(n2), st);
   d.part_cor = (double *)alloc(sizeof(double) * (size_t)d.units[KK_FFT_RI] * nO * 5);
   d.part_ke = (double *)alloc(sizeof(double) * (size_t)d.units[KK_FFT_r]);
-  d.part_obs = (double *)alloc(sizeof(double) * (size_t)nO * d.gemm_bwd_groups);
+  // potential-energy partials of the backward pass or of the fused step
+  d.part_obs = (double *)alloc(sizeof(double) * (size_t)nO * std::max(d.gemm_bwd_groups, d.step_groups));
   if (sh.acf) {
     // Autocorrelation <psi(t_0)|psi(t_n)>.  The state stored after a step
     // lacks the trailing V(dt/2) of the leapfrogged potential
@@ -537,8 +554,8 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st, De
   d.ring = kRing;
   d.obs_ring = (double *)alloc(sizeof(double) * (size_t)kRing * nO * 5);
   d.acf_ring = (double2 *)alloc(sizeof(double2) * (size_t)kRing * nO);
-  d.counters = (int *)alloc(3 * sizeof(int));
-  const int cnt[3] = {sh.steps0, 0, 0};
+  d.counters = (int *)alloc(4 * sizeof(int));
+  const int cnt[4] = {sh.steps0, 0, 0, 0};
   QD_HIP(hipMemcpyAsync(d.counters, cnt, sizeof(cnt), hipMemcpyHostToDevice, st));
   d.flux_on = 0;
   d.units[KK_FLUX] = 0;
@@ -636,7 +653,7 @@ void DeviceJob::prephase_if_first(hipStream_t s) {
 }
 
 void DeviceJob::flux_flush(hipStream_t s) {
-  if (!dev.flux_on || steps == flux_window_start_) return;
+  if (!dev.flux_on || flux_steps() == flux_window_start_) return;
   FluxWindow w;
   w.W = dev.flux_W;
   w.A = flux_A_;
@@ -648,22 +665,23 @@ void DeviceJob::flux_flush(hipStream_t s) {
   w.ne = dev.flux_ne;
   w.T = dev.flux_T;
   w.s0 = flux_window_start_;
-  w.n = steps - flux_window_start_;
+  w.n = flux_steps() - flux_window_start_;
   launch_flux_window(w, s);
   QD_HIP(hipGetLastError());
-  flux_window_start_ = steps;
+  flux_window_start_ = flux_steps();
 }
 
 void DeviceJob::flux_after_step(hipStream_t s) {
-  if (dev.flux_on && steps - flux_window_start_ >= dev.flux_T) flux_flush(s);
+  if (dev.flux_on && flux_steps() - flux_window_start_ >= dev.flux_T) flux_flush(s);
 }
 
 int DeviceJob::steps_to_flux_flush() const {
-  return dev.flux_on ? dev.flux_T - (steps - flux_window_start_) : std::numeric_limits<int>::max();
+  return dev.flux_on ? dev.flux_T - (flux_steps() - flux_window_start_) : std::numeric_limits<int>::max();
 }
 
 void DeviceJob::flux_to_crp(double *crp, hipStream_t s) {
   if (!dev.flux_on) return;
+  QD_REQUIRE(!pending, "reaction probabilities need the last step finished");
   flux_flush(s);
   // P = Im(sum w conj(A) B) dr1 / (2 pi mu_r eta)
   const double scale = shape.dr1 / (2.0 * M_PI * shape.m2);
@@ -713,12 +731,13 @@ void Engine::set_jobs(const std::vector<DeviceJob *> &jobs) {
     d_bases_ = nullptr;
     const int cap = std::max(n, 8);
     QD_HIP(hipMalloc(&d_jobs_, sizeof(JobDev) * cap));
-    QD_HIP(hipMalloc(&d_bases_, sizeof(int) * KK_COUNT * (cap + 1)));
+    QD_HIP(hipMalloc(&d_bases_, sizeof(int) * (KK_COUNT * (cap + 1) + 2 * KK_COUNT * cap)));
     n_alloc_jobs_ = cap;
   }
   if (n == 0) return;
   descs_.assign(n, JobDev{});
-  bases_.assign((size_t)KK_COUNT * (n + 1), 0);
+  // prefix tables of all jobs, then per job the single-job tables {0, units}
+  bases_.assign((size_t)KK_COUNT * (n + 1) + 2 * KK_COUNT * n, 0);
   std::vector<JobDev> &descs = descs_;
   std::vector<int> &bases = bases_;
   lds_fftR_ = lds_fftr_ = 0;
@@ -731,8 +750,21 @@ void Engine::set_jobs(const std::vector<DeviceJob *> &jobs) {
   bwd_waves_ = jobs[0]->dev.gemm_bwd_waves;
   any_flux_ = 0;
   any_acf_ = 0;
+  fused_ = jobs[0]->dev.leg_step;
+  step_waves_ = jobs[0]->dev.step_waves;
+  step_ttiles_ = step_ltiles_ = 0;
+  lds_step_ = 0;
   for (int q = 0; q < n; q++) {
     const JobDev &d = jobs[q]->dev;
+    QD_REQUIRE(d.leg_step == fused_ && d.step_waves == step_waves_,
+               "all jobs of one engine must share the Legendre step form");
+    if (fused_) {
+      step_ttiles_ = std::max(step_ttiles_, d.bwd_np / 16);
+      step_ltiles_ = std::max(step_ltiles_, d.fwd_np / 16);
+      lds_step_ = std::max(lds_step_, leg_step_lds_bytes(d.bwd_np / 16, d.fwd_np / 16));
+    }
+    int *single = bases.data() + (size_t)KK_COUNT * (n + 1) + (size_t)2 * KK_COUNT * q;
+    for (int k = 0; k < KK_COUNT; k++) single[2 * k + 1] = d.units[k];
     QD_REQUIRE(d.mixed == mixed_, "all jobs of one engine must share a precision");
     QD_REQUIRE(d.gemm_fwd_waves == fwd_waves_ && d.gemm_bwd_waves == bwd_waves_,
                "all jobs of one engine must share the GEMM workgroup size");
@@ -767,11 +799,21 @@ void Engine::set_jobs(const std::vector<DeviceJob *> &jobs) {
   QD_HIP(hipEventRecord(upload_done_, stream_));
 }
 
-StepArgs Engine::args(int kind) const {
+StepArgs Engine::args(int kind, int gate) const {
   StepArgs a;
   a.jobs = d_jobs_;
   a.unit_base = d_bases_ + (size_t)kind * (jobs_.size() + 1);
   a.n_jobs = (int)jobs_.size();
+  a.gate = gate;
+  return a;
+}
+
+StepArgs Engine::args_one(int q, int kind) const {
+  StepArgs a;
+  a.jobs = d_jobs_ + q;
+  a.unit_base = d_bases_ + (size_t)KK_COUNT * (jobs_.size() + 1) + (size_t)2 * KK_COUNT * q + 2 * kind;
+  a.n_jobs = 1;
+  a.gate = 0;
   return a;
 }
 
@@ -785,10 +827,27 @@ void Engine::enqueue(hipEvent_t *ev) {
   auto end = [&](int k) {
     if (ev) QD_HIP(hipEventRecord(ev[2 * k + 1], s));
   };
-  begin(QD_K_LEG_FWD);
-  launch_leg_fwd(args(KK_LEG_FWD), total_[KK_LEG_FWD], fwd_tiles_, fwd_kp_, mixed_, fwd_waves_,
-                 jobs_[0]->dev.leg_fwd_paired, s);
-  end(QD_K_LEG_FWD);
+  if (fused_) {
+    // the previous step's backward transform with this step's forward one;
+    // then that previous step's flux values and record, for the jobs that
+    // had one pending
+    begin(QD_K_LEG_STEP);
+    launch_leg_step(args(KK_LEG_STEP), total_[KK_LEG_STEP], step_ttiles_, step_ltiles_, lds_step_, 0, s);
+    end(QD_K_LEG_STEP);
+    if (any_flux_) {
+      begin(QD_K_OBS);
+      launch_flux(args(KK_FLUX, 1), total_[KK_FLUX], s);
+      end(QD_K_OBS);
+    }
+    begin(QD_K_REDUCE);
+    launch_reduce(args(KK_REDUCE, 1), total_[KK_REDUCE], s);
+    end(QD_K_REDUCE);
+  } else {
+    begin(QD_K_LEG_FWD);
+    launch_leg_fwd(args(KK_LEG_FWD), total_[KK_LEG_FWD], fwd_tiles_, fwd_kp_, mixed_, fwd_waves_,
+                   jobs_[0]->dev.leg_fwd_paired, s);
+    end(QD_K_LEG_FWD);
+  }
   begin(QD_K_FFT_R_FWD);
   if (ct_n1_) launch_fft_R_ct(args(KK_FFT_R), total_[KK_FFT_R], 0, 0, ct_n1_, nb_max_, s);
   else launch_fft_R_rt(args(KK_FFT_R), total_[KK_FFT_R], 0, 0, nb_max_, lds_fftR_, s);
@@ -801,24 +860,45 @@ void Engine::enqueue(hipEvent_t *ev) {
   if (ct_n1_) launch_fft_R_ct(args(KK_FFT_RI), total_[KK_FFT_RI], 1, any_acf_, ct_n1_, nb_max_, s);
   else launch_fft_R_rt(args(KK_FFT_RI), total_[KK_FFT_RI], 1, any_acf_, nb_max_, lds_fftR_, s);
   end(QD_K_FFT_R_INV);
-  begin(QD_K_LEG_BWD);
-  launch_leg_bwd(args(KK_LEG_BWD), total_[KK_LEG_BWD], bwd_tiles_, bwd_kp_, mixed_, bwd_waves_,
-                 jobs_[0]->dev.leg_bwd_paired, s);
-  end(QD_K_LEG_BWD);
-  if (any_flux_) {
-    begin(QD_K_OBS);
-    launch_flux(args(KK_FLUX), total_[KK_FLUX], s);
-    end(QD_K_OBS);
+  if (!fused_) {
+    begin(QD_K_LEG_BWD);
+    launch_leg_bwd(args(KK_LEG_BWD), total_[KK_LEG_BWD], bwd_tiles_, bwd_kp_, mixed_, bwd_waves_,
+                   jobs_[0]->dev.leg_bwd_paired, s);
+    end(QD_K_LEG_BWD);
+    if (any_flux_) {
+      begin(QD_K_OBS);
+      launch_flux(args(KK_FLUX), total_[KK_FLUX], s);
+      end(QD_K_OBS);
+    }
+    begin(QD_K_REDUCE);
+    launch_reduce(args(KK_REDUCE), total_[KK_REDUCE], s);
+    end(QD_K_REDUCE);
   }
-  begin(QD_K_REDUCE);
-  launch_reduce(args(KK_REDUCE), total_[KK_REDUCE], s);
-  end(QD_K_REDUCE);
+  QD_HIP(hipGetLastError());
+}
+
+void Engine::finish(int q) {
+  if (jobs_.empty() || !fused_) return;
+  hipStream_t s = stream_;
+  // the fused kernel's backward-only form: the same workgroups and partial
+  // sums as the fused steps, so a record does not depend on where a run was cut
+  if (q < 0) {
+    launch_leg_step(args(KK_LEG_STEP), total_[KK_LEG_STEP], step_ttiles_, step_ltiles_, lds_step_, 1, s);
+    if (any_flux_) launch_flux(args(KK_FLUX), total_[KK_FLUX], s);
+    launch_reduce(args(KK_REDUCE), total_[KK_REDUCE], s);
+  } else {
+    const JobDev &d = jobs_[q]->dev;
+    launch_leg_step(args_one(q, KK_LEG_STEP), d.units[KK_LEG_STEP], d.bwd_np / 16, d.fwd_np / 16,
+                    leg_step_lds_bytes(d.bwd_np / 16, d.fwd_np / 16), 1, s);
+    if (d.flux_on) launch_flux(args_one(q, KK_FLUX), d.units[KK_FLUX], s);
+    launch_reduce(args_one(q, KK_REDUCE), d.units[KK_REDUCE], s);
+  }
   QD_HIP(hipGetLastError());
 }
 
 unsigned Engine::timed_kinds() const {
-  unsigned m = (1u << QD_K_LEG_FWD) | (1u << QD_K_FFT_R_FWD) | (1u << QD_K_FFT_r) | (1u << QD_K_FFT_R_INV) |
-               (1u << QD_K_LEG_BWD) | (1u << QD_K_REDUCE);
+  unsigned m = (1u << QD_K_FFT_R_FWD) | (1u << QD_K_FFT_r) | (1u << QD_K_FFT_R_INV) | (1u << QD_K_REDUCE);
+  m |= fused_ ? (1u << QD_K_LEG_STEP) : ((1u << QD_K_LEG_FWD) | (1u << QD_K_LEG_BWD));
   if (any_flux_) m |= 1u << QD_K_OBS;
   return m;
 }

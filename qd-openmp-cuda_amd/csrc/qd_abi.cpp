@@ -25,7 +25,6 @@ struct qd_handle {
   std::unique_ptr<Engine> engine;
   const qd_inputs *in = nullptr;   // borrowed, must outlive the handle
   std::string err;
-  int records_enqueued = 0;
   int last_checked = 0;
   bool use_graph = true;
 };
@@ -70,11 +69,22 @@ void print_records(qd_handle *h, int from, int to) {
   std::fflush(stdout);
 }
 
+// With the fused Legendre step, the last enqueued step's backward transform,
+// flux values and record are pending until the next step; run them now.
+void finish_pending(qd_handle *h) {
+  DeviceJob &j = *h->job;
+  if (!j.pending) return;
+  j.flux_after_step(h->stream);   // a full window is folded before the last surface write
+  h->engine->finish();
+  j.pending = false;
+  j.flux_after_step(h->stream);
+}
+
 // drain the ring, check the norm of each new record, publish the step counter
 qd_status sync_and_check(qd_handle *h) {
   QD_HIP(hipStreamSynchronize(h->stream));
   DeviceJob &j = *h->job;
-  j.drain(h->records_enqueued);
+  j.drain(j.records_enqueued());
   const int nO = j.shape.nO;
   // the reference aborts on norm > 1 with zero tolerance (cudaOpenMP2.cu:437-440);
   // allow the rounding of each form: FP64 sums, or FP32-accumulated Legendre products
@@ -144,32 +154,36 @@ qd_status qd_run(qd_handle *h, int32_t n_steps) {
     const qd_options &op = h->in->options;
     const bool copy = (op.wave_to_matlab || op.wave_callback) && op.steps_to_copy_psi_from_device_to_host > 0;
     const int N = op.steps_to_copy_psi_from_device_to_host;
+    Engine &e = *h->engine;
     for (int s = 0; s < n_steps;) {
       j.prephase_if_first(h->stream);   // V(-dt/2) of step 0 (cudaOpenMP2.cu:344)
       // Steps in this launch: kGraphSteps as one graph when no host-side
       // event falls inside them (a copy-back follows the step whose
       // pre-step counter is a multiple of N, the flux window fills, the
-      // observable ring fills), otherwise one.
+      // observable ring fills), otherwise one.  With the fused Legendre
+      // step, k steps from a pending state add k records and flux writes; a
+      // step from a finished state runs alone (it adds none).
       int k = 1;
-      if (h->use_graph) {
+      if (h->use_graph && (!e.fused() || j.pending)) {
         int lim = std::min(n_steps - s, Engine::kGraphSteps);
         lim = std::min(lim, j.steps_to_flux_flush());
-        lim = std::min(lim, j.dev.ring - (h->records_enqueued - j.records_done));
+        lim = std::min(lim, j.dev.ring - (j.records_enqueued() - j.records_done));
         if (copy) {
           const int r = j.steps % N;
           lim = std::min(lim, r == 0 ? 1 : N - r + 1);
         }
         if (lim == Engine::kGraphSteps) k = lim;
       }
-      if (h->use_graph) h->engine->step_graph(k);
-      else h->engine->step();
-      h->records_enqueued += k;
+      if (h->use_graph) e.step_graph(k);
+      else e.step();
       j.steps += k;
+      if (e.fused()) j.pending = true;
       s += k;
       const int steps_before = j.steps - 1;   // pre-step counter of the launch's last step
       j.flux_after_step(h->stream);
-      const bool ring_full = h->records_enqueued - j.records_done >= j.dev.ring;
       const bool copy_now = copy && steps_before % N == 0;
+      if (copy_now || s == n_steps) finish_pending(h);
+      const bool ring_full = j.records_enqueued() - j.records_done >= j.dev.ring;
       if (ring_full || copy_now || s == n_steps) {
         qd_status st = sync_and_check(h);
         if (st != QD_OK) return st;
@@ -190,11 +204,12 @@ qd_status qd_synchronize(qd_handle *h) {
   QD_GUARD(h, { return sync_and_check(h); })
 }
 
-int32_t qd_steps_done(const qd_handle *h) { return h ? h->records_enqueued : 0; }
+int32_t qd_steps_done(const qd_handle *h) { return h ? h->job->steps - h->job->shape.steps0 : 0; }
 
 qd_status qd_copy_wavepackets(qd_handle *h, double *dst) {
   if (!h || !dst) return fail(h, QD_ERR_INVALID, "NULL argument");
   QD_GUARD(h, {
+    finish_pending(h);
     h->job->copy_psi(dst, h->stream);
     return QD_OK;
   })
@@ -232,6 +247,7 @@ qd_status qd_get_crp(qd_handle *h, double *crp) {
   if (!h || !crp) return fail(h, QD_ERR_INVALID, "NULL argument");
   QD_GUARD(h, {
     if (!h->job->shape.crp) return fail(h, QD_ERR_STATE, "CRP was not requested (calculate_CRP == 0)");
+    finish_pending(h);
     h->job->flux_to_crp(crp, h->stream);
     return QD_OK;
   })
@@ -245,25 +261,36 @@ qd_status qd_profile_steps(qd_handle *h, int32_t n_steps, double *us) {
     const unsigned timed = h->engine->timed_kinds();
     double acc[QD_KERNEL_KINDS] = {};
     qd_status st = QD_OK;
+    DeviceJob &j = *h->job;
+    int timed_steps = 0;
     for (int s = 0; s < n_steps && st == QD_OK; s++) {
-      h->job->prephase_if_first(h->stream);
-      h->engine->step(ev);
-      h->records_enqueued++;
-      h->job->steps++;
-      h->job->flux_after_step(h->stream);
+      j.prephase_if_first(h->stream);
+      // with the fused Legendre step, a step from a finished state runs the
+      // forward transform alone: untimed unless it is the only step
+      const bool time_it = !h->engine->fused() || j.pending || n_steps == 1;
+      h->engine->step(time_it ? ev : nullptr);
+      j.steps++;
+      if (h->engine->fused()) j.pending = true;
+      j.flux_after_step(h->stream);
       QD_HIP(hipStreamSynchronize(h->stream));
-      for (int k = 0; k < QD_KERNEL_KINDS; k++) {
-        if (!(timed & (1u << k))) continue;
-        float ms = 0.f;
-        QD_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
-        acc[k] += ms * 1000.0;
+      if (time_it) {
+        timed_steps++;
+        for (int k = 0; k < QD_KERNEL_KINDS; k++) {
+          if (!(timed & (1u << k))) continue;
+          float ms = 0.f;
+          QD_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
+          acc[k] += ms * 1000.0;
+        }
       }
-      if (h->records_enqueued - h->job->records_done >= h->job->dev.ring) st = sync_and_check(h);
+      if (j.records_enqueued() - j.records_done >= j.dev.ring) st = sync_and_check(h);
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
-    if (st == QD_OK) st = sync_and_check(h);
+    if (st == QD_OK) {
+      finish_pending(h);
+      st = sync_and_check(h);
+    }
     // mean per launch; the R pass launches twice per step, as two kinds
-    for (int k = 0; k < QD_KERNEL_KINDS; k++) us[k] = acc[k] / n_steps;
+    for (int k = 0; k < QD_KERNEL_KINDS; k++) us[k] = timed_steps ? acc[k] / timed_steps : 0.0;
     return st;
   })
 }

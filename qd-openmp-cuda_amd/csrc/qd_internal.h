@@ -41,6 +41,7 @@ enum KernelKind {
   KK_LEG_BWD,
   KK_REDUCE,
   KK_FLUX,
+  KK_LEG_STEP,  // backward Legendre transform of one step fused with the forward one of the next
   KK_COUNT
 };
 
@@ -60,6 +61,9 @@ struct JobDev {
   int bwd_kp, bwd_np;        // padded K and N, backward
   int fft_rows, fft_rows_inv; // pencils per workgroup along R (forward, inverse R pass)
   int fft_cols;              // pencils per workgroup along r
+  // fused Legendre step (KK_LEG_STEP): 1 when the job's steps run it; its
+  // waves per workgroup, strips (16 points) per wave and workgroups per block
+  int leg_step, step_waves, step_strips, step_groups;
   int units[KK_COUNT];       // work units (workgroups) per kernel for this job
   // physics
   double dt, dr1, dr2;
@@ -94,14 +98,17 @@ struct JobDev {
   // reduction scratch (per-workgroup partial sums)
   QD_G double *part_cor;          // [nO][5][units FFT_R] Coriolis, rotational energy, norm, acf re/im
   QD_G double *part_ke;           // [units FFT_r]       kinetic energy of one (block, slab, column group)
-  QD_G double *part_obs;          // [nO][gemm_bwd_groups] w V |psi|^2 (potential energy)
+  QD_G double *part_obs;          // [nO][gemm_bwd_groups or step_groups] w V |psi|^2 (potential energy)
   QD_G double *obs_ring;          // [ring][nO][5] per-step observables
   QD_G double2 *acf_ring;         // [ring][nO] autocorrelation contribution of each block
   int ring;                  // ring capacity (records)
   QD_G int *counters;             // device: [0] step counter (the reference's time.steps),
                              //         [1] records written, [2] advance pending: set by
                              //         the step's reduction, applied by advance_counters
-                             //         at the end of the next step's first kernel
+                             //         at the end of the next Legendre kernel;
+                             //         [3] backward transform pending (leg_step jobs): set
+                             //         by the inverse R pass, cleared by the ungated
+                             //         reduction that follows a separate backward pass
   // flux at the dividing surface r2 = r_s (index flux_ns): per step the
   // surface kernel writes the values and the ng-point r-derivative of the
   // state with V(dt/2) restored into window slot (step % flux_T); every
@@ -130,6 +137,10 @@ struct StepArgs {
   const QD_G JobDev *jobs;
   const QD_G int *unit_base;      // [n_jobs + 1] prefix of units for this kernel kind
   int n_jobs;
+  // flux surface and reduction kernels: 1 in the fused step sequence, where
+  // they finish the previous step of each job whose backward transform was
+  // pending (counters[3]) and skip the others; 0 after a separate backward pass
+  int gate;
 };
 
 // Locate the job owning workgroup `u` (binary search over the prefix table).
@@ -174,11 +185,12 @@ __device__ __forceinline__ double wave_sum64(double x) {
 }
 #endif
 
-// Applies the step-counter advance flagged by the previous step's reduction;
-// called, after its own work, by one thread of the job's first workgroup of
-// each step's first kernel (the forward Legendre GEMM, which reads no
-// counter).  Kernel boundaries order it after that reduction and before this
-// step's readers of the counters (flux surface kernel, reduction).
+// Applies the step-counter advance flagged by the last reduction; called,
+// after its own work, by one thread of the job's first workgroup of each
+// Legendre kernel (forward, fused or backward; none reads a counter).  A
+// reduction is always followed by such a kernel before the next flux surface
+// kernel or reduction, and kernel boundaries order the advance between them;
+// a second call finds nothing pending.
 __device__ __forceinline__ void advance_counters(const JobDev &jb) {
   if (jb.counters[2]) {
     jb.counters[0] += 1;
@@ -204,6 +216,21 @@ bool leg_bwd_paired(int nt_tiles);
 size_t leg_lds_bytes(int kp, int np, int mixed);
 void launch_leg_fwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, int waves, int paired,
                     hipStream_t stream);
+// Fused Legendre step (FP64, paired rows): for each job whose backward
+// transform is pending (counters[3]), the backward product of its last step,
+// the potential-energy partials, the absorber and potential phase and the
+// forward product of its next step, strip by strip, the DVR state held in
+// registers; for the others the forward product alone.  theta_tiles = bwd_np
+// / 16 and l_tiles = fwd_np / 16 (both supported by leg_step_supported).
+bool leg_step_supported(int theta_tiles, int l_tiles, bool mixed);
+// LDS bytes of the fused kernel: both Legendre tables, padded to whole
+// chunks of 16 rows
+size_t leg_step_lds_bytes(int theta_tiles, int l_tiles);
+// finish: the backward transform alone, for the jobs with one pending (the
+// separate backward pass that ends a fused step sequence, with the same
+// partial sums of the potential energy)
+void launch_leg_step(const StepArgs &a, int total_units, int theta_tiles, int l_tiles, size_t lds, int finish,
+                     hipStream_t stream);
 void launch_leg_bwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max, int mixed, int waves, int paired,
                     hipStream_t stream);
 // FFT along R fused with rotational . Coriolis . rotational (before the forward

@@ -35,9 +35,9 @@ double now_s() {
 // Jobs share a class when every kernel instantiation of the step is the same
 // for them: grid lengths (FFT kernels and tilings, including the R-pass rows,
 // which depend on whether a job has more than 5 omega blocks), Legendre tile
-// shapes and kernel forms (workgroup waves, paired or one-row strips) and the
-// precision.
-using ClassKey = std::tuple<int, int, int, int, int, int, int, int, int, int, int, int, int, int>;
+// shapes and kernel forms (workgroup waves, paired or one-row strips, the
+// fused Legendre step) and the precision.
+using ClassKey = std::tuple<int, int, int, int, int, int, int, int, int, int, int, int, int, int, int, int>;
 
 struct ServerJob {
   int id = 0;
@@ -213,7 +213,7 @@ struct qd_server {
     const JobDev &d = j.dev;
     const ClassKey key{d.n1,      d.n2,     d.nt,       d.fwd_np,       d.bwd_np,           d.fwd_kp,
                        d.bwd_kp,  d.mixed,  d.fft_rows, d.fft_rows_inv, d.gemm_fwd_waves,   d.gemm_bwd_waves,
-                       d.leg_fwd_paired, d.leg_bwd_paired};
+                       d.leg_fwd_paired, d.leg_bwd_paired, d.leg_step, d.step_waves};
     for (auto &c : classes)
       if (c->key == key) return *c;
     std::unique_ptr<ShapeClass> c(new ShapeClass());
@@ -341,7 +341,9 @@ struct qd_server {
     try {
       QD_HIP(hipStreamWaitEvent(admit_stream, after, 0));
       QD_HIP(hipEventSynchronize(after));
-      sj->job->drain(sj->enqueued);
+      // every record (a job retired early for a failed norm check may leave
+      // its last step unfinished: it writes no outputs)
+      sj->job->drain(sj->job->records_enqueued());
       ok = check_norms(sj, msg);
       if (ok) write_outputs(sj, admit_stream);
     } catch (const std::exception &e) {
@@ -401,7 +403,7 @@ struct qd_server {
   void drain_class(ShapeClass &c) {
     QD_HIP(hipStreamSynchronize(c.stream));
     for (ServerJob *sj : c.members) {
-      sj->job->drain(sj->enqueued);
+      sj->job->drain(sj->job->records_enqueued());
       std::string msg;
       if (!check_norms(sj, msg)) {
         std::lock_guard<std::mutex> lk(mu);
@@ -478,8 +480,20 @@ struct qd_server {
         DeviceJob &j = *sj->job;
         j.steps += k;
         sj->enqueued += k;
+        if (c.engine->fused()) j.pending = true;
         j.flux_after_step(c.stream);
-        drain |= sj->enqueued < sj->target && sj->enqueued - j.records_done >= kDrainEvery;
+        drain |= sj->enqueued < sj->target && j.records_enqueued() - j.records_done >= kDrainEvery;
+      }
+      // jobs at their target run their last step's backward transform, flux
+      // values and record alone (the engine's job q is members[q])
+      for (size_t q = 0; q < c.members.size(); q++) {
+        ServerJob *sj = c.members[q];
+        DeviceJob &j = *sj->job;
+        if (sj->enqueued >= sj->target && j.pending) {
+          c.engine->finish((int)q);
+          j.pending = false;
+          j.flux_after_step(c.stream);
+        }
       }
       hipEvent_t e = class_event(c);
       QD_HIP(hipEventRecord(e, c.stream));
