@@ -19,7 +19,8 @@ OBS_NAMES = ("norm", "potential", "kinetic", "rotational", "coriolis")
 QD_KERNEL_KINDS = 11
 # kernel kinds of qd_profile_steps (dvr, fbr1 and fbr2 are fused into
 # leg_fwd / leg_step, fft_R_fwd and fft_R_inv and read 0; flux runs only with
-# CRP; where the fused Legendre step runs, leg_fwd and leg_bwd read 0)
+# CRP; where the fused Legendre step runs, leg_fwd and leg_bwd read 0 and the
+# reduction runs inside fft_R_fwd, so reduce reads 0 as well)
 KERNEL_NAMES = ("dvr", "leg_fwd", "fbr1", "fft_R_fwd", "fft_r", "fft_R_inv", "fbr2", "leg_bwd",
                 "flux", "reduce", "leg_step")
 

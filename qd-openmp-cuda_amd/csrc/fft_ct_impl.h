@@ -14,6 +14,7 @@
 #include <type_traits>
 
 #include "fft_common.h"
+#include "step_reduce.h"
 
 namespace qd {
 
@@ -219,9 +220,11 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
   constexpr bool kMixBatched = INV && NB > 5 && NB <= 9 && N1 >= 256;
   __shared__ double Vs[VLDS ? NB * NB : 1];
   __shared__ double2 cms[VLDS ? NB : 1];
-  const int job = locate_job(a.unit_base, a.n_jobs, blockIdx.x);
+  if (!INV && reduce_carried(a, red)) return;   // leading workgroups: the previous step's reduction
+  const int bid = blockIdx.x - (INV ? 0 : a.red_units);
+  const int job = locate_job(a.unit_base, a.n_jobs, bid);
   const JobDev &jb = a.jobs[job];
-  const int u = blockIdx.x - a.unit_base[job];
+  const int u = bid - a.unit_base[job];
   const int M = jb.M;
   const long LM = (long)jb.L * M;
   const int groups = jb.n2 / P;
@@ -617,7 +620,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_r_ct(StepArgs a) {
 // ---------------------------------------------------------------------------
 template <bool INV, int N1, bool ACF>
 void launch_R_nb(const StepArgs &a, int units, int nb_max, hipStream_t s) {
-  const dim3 g(units), t(kBlock);
+  const dim3 g(units + (INV ? 0 : a.red_units)), t(kBlock);
   if (nb_max <= 1) hipLaunchKernelGGL((k_fft_R_ct<INV, 1, N1, ACF>), g, t, 0, s, a);
   else if (nb_max <= 2) hipLaunchKernelGGL((k_fft_R_ct<INV, 2, N1, ACF>), g, t, 0, s, a);
   else if (nb_max <= 3) hipLaunchKernelGGL((k_fft_R_ct<INV, 3, N1, ACF>), g, t, 0, s, a);

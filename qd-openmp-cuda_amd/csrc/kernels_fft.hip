@@ -20,6 +20,7 @@
 #include <mutex>
 
 #include "fft_common.h"
+#include "step_reduce.h"
 #include "host_internal.h"
 
 namespace qd {
@@ -86,9 +87,11 @@ __global__ __launch_bounds__(kBlock) void k_fft_R(StepArgs a) {
   constexpr int NF = ACF ? 5 : 3;
   extern __shared__ double2 lds[];
   __shared__ double red[4 * NF * NB];
-  const int job = locate_job(a.unit_base, a.n_jobs, blockIdx.x);
+  if (!INV && reduce_carried(a, red)) return;   // leading workgroups: the previous step's reduction
+  const int bid = blockIdx.x - (INV ? 0 : a.red_units);
+  const int job = locate_job(a.unit_base, a.n_jobs, bid);
   const JobDev &jb = a.jobs[job];
-  const int u = blockIdx.x - a.unit_base[job];
+  const int u = bid - a.unit_base[job];
   const int n1 = jb.n1, P = INV ? jb.fft_rows_inv : jb.fft_rows, M = jb.M, L = jb.L;
   const long LM = (long)L * M;
   const int groups = jb.n2 / P;
@@ -153,7 +156,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_R(StepArgs a) {
 
 void launch_fft_R_rt(const StepArgs &a, int total_units, int inverse, int acf, int nb_max, size_t lds,
                      hipStream_t stream) {
-  const dim3 g(total_units), t(kBlock);
+  const dim3 g(total_units + (inverse ? 0 : a.red_units)), t(kBlock);
 #define QD_FFTR_CASE(NB)                                                                    \
   if (nb_max <= NB) {                                                                       \
     if (!inverse) hipLaunchKernelGGL((k_fft_R<false, NB, false>), g, t, lds, stream, a);    \

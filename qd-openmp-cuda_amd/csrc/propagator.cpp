@@ -805,6 +805,8 @@ StepArgs Engine::args(int kind, int gate) const {
   a.unit_base = d_bases_ + (size_t)kind * (jobs_.size() + 1);
   a.n_jobs = (int)jobs_.size();
   a.gate = gate;
+  a.red_base = nullptr;
+  a.red_units = 0;
   return a;
 }
 
@@ -814,6 +816,8 @@ StepArgs Engine::args_one(int q, int kind) const {
   a.unit_base = d_bases_ + (size_t)KK_COUNT * (jobs_.size() + 1) + (size_t)2 * KK_COUNT * q + 2 * kind;
   a.n_jobs = 1;
   a.gate = 0;
+  a.red_base = nullptr;
+  a.red_units = 0;
   return a;
 }
 
@@ -839,9 +843,7 @@ void Engine::enqueue(hipEvent_t *ev) {
       launch_flux(args(KK_FLUX, 1), total_[KK_FLUX], s);
       end(QD_K_OBS);
     }
-    begin(QD_K_REDUCE);
-    launch_reduce(args(KK_REDUCE, 1), total_[KK_REDUCE], s);
-    end(QD_K_REDUCE);
+    // the reduction of that step rides on the forward R pass below
   } else {
     begin(QD_K_LEG_FWD);
     launch_leg_fwd(args(KK_LEG_FWD), total_[KK_LEG_FWD], fwd_tiles_, fwd_kp_, mixed_, fwd_waves_,
@@ -849,8 +851,18 @@ void Engine::enqueue(hipEvent_t *ev) {
     end(QD_K_LEG_FWD);
   }
   begin(QD_K_FFT_R_FWD);
-  if (ct_n1_) launch_fft_R_ct(args(KK_FFT_R), total_[KK_FFT_R], 0, 0, ct_n1_, nb_max_, s);
-  else launch_fft_R_rt(args(KK_FFT_R), total_[KK_FFT_R], 0, 0, nb_max_, lds_fftR_, s);
+  {
+    // fused sequence: the previous step's (gated) reduction as the leading
+    // workgroups of the forward R pass, which reads and writes no partial
+    // sum and no counter (one launch less per step)
+    StepArgs ar = args(KK_FFT_R);
+    if (fused_) {
+      ar.red_base = d_bases_ + (size_t)KK_REDUCE * (jobs_.size() + 1);
+      ar.red_units = total_[KK_REDUCE];
+    }
+    if (ct_n1_) launch_fft_R_ct(ar, total_[KK_FFT_R], 0, 0, ct_n1_, nb_max_, s);
+    else launch_fft_R_rt(ar, total_[KK_FFT_R], 0, 0, nb_max_, lds_fftR_, s);
+  }
   end(QD_K_FFT_R_FWD);
   begin(QD_K_FFT_r);
   if (ct_n2_) launch_fft_r_ct(args(KK_FFT_r), total_[KK_FFT_r], ct_n2_, s);
@@ -897,8 +909,8 @@ void Engine::finish(int q) {
 }
 
 unsigned Engine::timed_kinds() const {
-  unsigned m = (1u << QD_K_FFT_R_FWD) | (1u << QD_K_FFT_r) | (1u << QD_K_FFT_R_INV) | (1u << QD_K_REDUCE);
-  m |= fused_ ? (1u << QD_K_LEG_STEP) : ((1u << QD_K_LEG_FWD) | (1u << QD_K_LEG_BWD));
+  unsigned m = (1u << QD_K_FFT_R_FWD) | (1u << QD_K_FFT_r) | (1u << QD_K_FFT_R_INV);
+  m |= fused_ ? (1u << QD_K_LEG_STEP) : ((1u << QD_K_LEG_FWD) | (1u << QD_K_LEG_BWD) | (1u << QD_K_REDUCE));
   if (any_flux_) m |= 1u << QD_K_OBS;
   return m;
 }

@@ -141,6 +141,11 @@ struct StepArgs {
   // they finish the previous step of each job whose backward transform was
   // pending (counters[3]) and skip the others; 0 after a separate backward pass
   int gate;
+  // forward R pass of the fused step sequence: its first red_units
+  // workgroups run the (gated) reduction of the previous step, with the
+  // reduction's prefix table red_base (0: none)
+  const QD_G int *red_base;
+  int red_units;
 };
 
 // Locate the job owning workgroup `u` (binary search over the prefix table).
