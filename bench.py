@@ -45,7 +45,7 @@ def _load_package():
     return mod
 
 
-def kernel_model(job):
+def kernel_model(job, form=2):
     """Algorithmic bytes and flops per launch of each kernel kind for one step.
 
     G = n1 n2 ntheta DVR points per block, F_b = n1 n2 (l_max - omega_b + 1)
@@ -62,7 +62,9 @@ def kernel_model(job):
     fused Legendre step (leg_step: the backward pass of one step with the
     forward pass of the next, psi kept on chip) reads the FBR state and V and
     writes the FBR state, 8 G + 32 F_b, with both products' flops; a step
-    then moves 8 G + 128 F_b."""
+    then moves 8 G + 128 F_b.  In its parity-split form (`form` 2, the
+    default on Gauss-Legendre grids) each product runs over half the theta
+    grid, so both together take 4 n1 n2 ntheta N_l per block."""
     r1, r2, th, os_ = job["r1"], job["r2"], job["theta"], job["omega_states"]
     n1, n2, nt = r1["n"], r2["n"], th["n"]
     M = n1 * n2
@@ -85,7 +87,7 @@ def kernel_model(job):
         "leg_bwd": {"bytes": nO * 16 * G + 16 * Ft, "flops": gemm_flops},
         "flux": zero,
         "reduce": zero,
-        "leg_step": {"bytes": nO * 8 * G + 32 * Ft, "flops": 2 * gemm_flops},
+        "leg_step": {"bytes": nO * 8 * G + 32 * Ft, "flops": (1 if form == 2 else 2) * gemm_flops},
     }, G * nO
 
 
@@ -168,12 +170,19 @@ def mixed_crp_error(api, abi, inputs, steps):
     return out
 
 
-def step_model_seconds(job):
-    """Model time of one step: sum over the kernels of max(bytes / HBM peak,
+def step_kinds(form):
+    """Kernel kinds of one step: the fused Legendre step (forms 1, 2) or the
+    separate forward and backward passes (form 0), the three FFT passes."""
+    leg = ("leg_step",) if form else ("leg_fwd", "leg_bwd")
+    return leg + ("fft_R_fwd", "fft_r", "fft_R_inv")
+
+
+def step_model_seconds(job, form):
+    """Model time of one step: sum over its kernels of max(bytes / HBM peak,
     flops / FP64 MFMA peak), with the per-kernel figures of kernel_model."""
-    model, _ = kernel_model(job)
-    return sum(max(m["bytes"] / (HBM_PEAK_GBS * 1e9), m["flops"] / (FP64_MFMA_PEAK_TFLOPS * 1e12))
-               for m in model.values())
+    model, _ = kernel_model(job, form)
+    return sum(max(model[k]["bytes"] / (HBM_PEAK_GBS * 1e9), model[k]["flops"] / (FP64_MFMA_PEAK_TFLOPS * 1e12))
+               for k in step_kinds(form))
 
 
 def other_configs(api, abi, inputs, cpu_threads, with_cpu):
@@ -195,7 +204,9 @@ def other_configs(api, abi, inputs, cpu_threads, with_cpu):
             el, _ = measure(api, abi, job, prec, gpu_steps, 20, profile_steps=0)
             rec[prec_name] = {"steps": gpu_steps, "ms_per_step": el / gpu_steps * 1e3,
                               "grid_point_steps_per_s": pts * gpu_steps / el}
-        rec["fp64"]["model_frac"] = step_model_seconds(job) / (rec["fp64"]["ms_per_step"] * 1e-3)
+        form = api.legendre_form(abi.Inputs(job, total_steps=1))
+        rec["fp64"]["model_frac"] = step_model_seconds(job, form) / (rec["fp64"]["ms_per_step"] * 1e-3)
+        rec["fp64"]["legendre_form"] = form
         if name == "c1":
             # the latency-bound small job batched: 16 copies stepped together by
             # the server (one launch per kernel for all 16)
@@ -335,7 +346,8 @@ def main():
         spec = dataclasses.replace(spec, n1=n1, n2=n2, ntheta=nt, J=2 * (nb - 1), parity=0, k_max=nb - 1,
                                    R0=9.0, seed=None)
     job = inputs.build_job(spec)
-    model, pts = kernel_model(job)
+    form = api.legendre_form(abi.Inputs(job, total_steps=1))
+    model, pts = kernel_model(job, form)
     print(f"# workload {args.workload}: {spec.n1}x{spec.n2}x{spec.ntheta}, J={spec.J}, K_max={spec.k_max}, "
           f"{len(job['omega_states']['omegas'])} blocks; steps={args.steps} warmup={args.warmup}",
           file=sys.stderr, flush=True)
@@ -366,12 +378,13 @@ def main():
                 "frac": achieved / HBM_PEAK_GBS}
     roof["traffic"] = read_traffic(dom)
     roof["kernel"] = dom
+    roof["legendre_form"] = ("fused step, parity-split products" if form == 2 else
+                             "fused step, direct products" if form == 1 else "separate passes")
     roof["kernel_us"] = us
     roof["algorithmic_bytes"] = m["bytes"]
     roof["algorithmic_flops"] = m["flops"]
     # whole step against the same per-kernel model: sum of max(bytes/BW, flops/peak)
-    t_model = sum(max(mk["bytes"] / (HBM_PEAK_GBS * 1e9), mk["flops"] / (FP64_MFMA_PEAK_TFLOPS * 1e12))
-                  for mk in model.values())
+    t_model = step_model_seconds(job, form)
     roof["step_model_us"] = t_model * 1e6
     roof["step_model_frac"] = t_model / (el64 / args.steps)
     per_kernel = {}

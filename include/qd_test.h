@@ -51,6 +51,14 @@ qd_status qd_probe_copy(int64_t bytes, int32_t repeats, double *us_per_copy);
  * counters.  No counterpart in the reference. */
 qd_status qd_probe_bandwidth(int64_t bytes, int32_t repeats, int32_t kind, double *us_per_pass);
 
+/* Which form of the Legendre transforms a job built from `in` would run (host
+ * only: validates the inputs, touches no device): 0 the separate forward and
+ * backward passes, 1 the fused step (backward transform of one step with the
+ * forward transform of the next), 2 the fused step with parity-split products
+ * (the theta grid and tables symmetric under x -> -x).  No counterpart in the
+ * reference. */
+qd_status qd_probe_legendre_form(const qd_inputs *in, int32_t *form);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,7 @@ _DECLS = {
     "qd_probe_coriolis": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
     "qd_probe_copy": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
     "qd_probe_bandwidth": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
+    "qd_probe_legendre_form": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "qd_server_create": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_void_p)]),
     "qd_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32)]),
     "qd_server_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
@@ -219,6 +220,15 @@ class Server:
 # ---------------------------------------------------------------------------
 # probes (include/qd_test.h)
 # ---------------------------------------------------------------------------
+
+def legendre_form(inp) -> int:
+    """Form of the Legendre transforms a job would run (host only): 0 the
+    separate passes, 1 the fused step, 2 the fused step with parity-split
+    products."""
+    form = C.c_int32(-1)
+    _check(lib().qd_probe_legendre_form(C.addressof(inp.struct), C.byref(form)))
+    return form.value
+
 
 def probe_gemm(A: np.ndarray, B: np.ndarray) -> np.ndarray:
     A = np.asfortranarray(A, dtype=np.float64)

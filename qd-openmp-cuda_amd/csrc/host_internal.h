@@ -49,6 +49,9 @@ struct JobShape {
 
 // Throws Error(QD_ERR_INVALID) on any inconsistency; reads only host memory.
 JobShape validate_inputs(const qd_inputs *in);
+// Form of the Legendre transforms of a validated job: 0 separate passes,
+// 1 fused step, 2 fused step with parity-split products (host only).
+int legendre_step_form(const qd_inputs *in, const JobShape &sh);
 
 // Where a job's device memory and its large host<->device copies come from.
 // The plain form allocates with hipMalloc, frees with hipFree and copies
@@ -165,12 +168,13 @@ class Engine {
   int fwd_tiles_ = 0, bwd_tiles_ = 0, fwd_kp_ = 0, bwd_kp_ = 0, nb_max_ = 1;
   int fwd_waves_ = 4, bwd_waves_ = 4;   // waves per GEMM workgroup (the same for every job)
   int mixed_ = 0, any_flux_ = 0, any_acf_ = 0;
-  int fused_ = 0, step_waves_ = 4, step_ttiles_ = 0, step_ltiles_ = 0;
+  int fused_ = 0, split_ = 0, step_waves_ = 4, step_ttiles_ = 0, step_ltiles_ = 0;
   size_t lds_step_ = 0;
   hipGraph_t graph_[2] = {};           // [0] one step, [1] kGraphSteps steps
   hipGraphExec_t graph_exec_[2] = {};
   StepArgs args(int kind, int gate = 0) const;
   StepArgs args_one(int q, int kind) const;   // job q alone
+  void launch_step_kernel(const StepArgs &a, int units, int finish, hipStream_t s) const;
   void enqueue(hipEvent_t *ev);
 };
 

@@ -305,12 +305,17 @@ __global__ __launch_bounds__(4 * 64) void k_leg_step_paired(StepArgs a) {
   if (u == 0 && threadIdx.x == 0) advance_counters(jb);
 }
 
-bool leg_step_supported(int theta_tiles, int l_tiles, bool mixed) {
+// QD_LEG_STEP=0 keeps the separate Legendre passes (tests, comparisons)
+bool leg_step_enabled() {
   static const bool on = [] {
     const char *e = std::getenv("QD_LEG_STEP");
     return !(e && std::atoi(e) == 0);
   }();
-  return on && !mixed && leg_paired() && theta_tiles >= 1 && theta_tiles <= 4 && l_tiles >= 1 &&
+  return on && leg_paired();
+}
+
+bool leg_step_supported(int theta_tiles, int l_tiles, bool mixed) {
+  return leg_step_enabled() && !mixed && theta_tiles >= 1 && theta_tiles <= 4 && l_tiles >= 1 &&
          l_tiles <= theta_tiles;
 }
 

@@ -94,6 +94,7 @@ int pick_divisor(int n, const int *cands, int ncand, int limit_elems, int other)
 void init_attrs_fft();
 void init_attrs_gemm();
 void init_attrs_leg_step();
+void init_attrs_leg_split();
 
 void init_kernel_attributes() {
   // once per process, also when several host threads create handles at once
@@ -102,6 +103,7 @@ void init_kernel_attributes() {
     init_attrs_fft();
     init_attrs_gemm();
     init_attrs_leg_step();
+    init_attrs_leg_split();
     (void)hipGetLastError();   // an unsupported attribute must not leak into later error checks
   });
 }
@@ -281,6 +283,57 @@ static size_t gemm_pad_slabs_fbr(const JobShape &s, const JobDev &d) {
   return (size_t)std::max(0, om_last + d.bwd_kp - s.L);
 }
 
+// QD_LEG_SPLIT=0 keeps the direct form of the fused Legendre step (tests).
+static bool leg_split_enabled() {
+  static const bool on = [] {
+    const char *e = std::getenv("QD_LEG_SPLIT");
+    return !(e && std::atoi(e) == 0);
+  }();
+  return on;
+}
+
+// Whether the caller's theta grid and Legendre tables have the symmetry the
+// parity-split Legendre products use: nodes x_k' = -x_k and weights w_k' = w_k
+// (k' = nt - 1 - k), and in every block P_l(x_k') = (-1)^(l - omega) P_l(x_k),
+// all to 1e-12 of the largest value (Gauss-Legendre nodes and normalised
+// associated Legendre functions satisfy it to rounding).
+static bool legendre_symmetric(const qd_inputs *in, const JobShape &s) {
+  const int nt = s.nt;
+  const double *x = in->theta.x, *w = in->theta.w;
+  double wmax = 0.0;
+  for (int k = 0; k < nt; k++) wmax = std::max(wmax, std::fabs(w[k]));
+  for (int k = 0; k < nt / 2; k++) {
+    const int kq = nt - 1 - k;
+    if (!(std::fabs(x[k] + x[kq]) <= 1e-12) || !(std::fabs(w[k] - w[kq]) <= 1e-12 * wmax)) return false;
+  }
+  const double *Pin = in->omega_states.associated_legendres;
+  for (int b = 0; b < s.nO; b++) {
+    const int nl = s.lmax - (s.omin + b) + 1;
+    const double *pb = Pin + (size_t)b * nt * s.legendre_n_max;
+    for (int c = 0; c < nl; c++) {
+      const double *p = pb + (size_t)c * nt;
+      double pmax = 0.0;
+      for (int k = 0; k < nt; k++) pmax = std::max(pmax, std::fabs(p[k]));
+      const double sg = c % 2 == 0 ? 1.0 : -1.0;
+      for (int k = 0; k < nt / 2; k++)
+        if (!(std::fabs(p[nt - 1 - k] - sg * p[k]) <= 1e-12 * pmax)) return false;
+    }
+  }
+  return true;
+}
+
+int legendre_step_form(const qd_inputs *in, const JobShape &sh) {
+  const int nt = sh.nt, nl_max = sh.lmax - sh.omin + 1;
+  const int theta_tiles = round_up(nt, 16) / 16, l_tiles = round_up(nl_max, 16) / 16;
+  const int th = (nt / 2 + 15) / 16, te = ((nl_max + 1) / 2 + 15) / 16, to = (nl_max / 2 + 15) / 16;
+  if (leg_step_enabled() && leg_split_enabled() && !sh.mixed && nt % 2 == 0 && nl_max >= 2 &&
+      leg_split_supported(th, te, to) && leg_split_lds_bytes(th, te, to) <= kLdsLimit && legendre_symmetric(in, sh))
+    return 2;
+  if (leg_step_supported(theta_tiles, l_tiles, sh.mixed) && leg_step_lds_bytes(theta_tiles, l_tiles) <= kLdsLimit)
+    return 1;
+  return 0;
+}
+
 size_t DeviceJob::estimate_footprint(const JobShape &s, int n_energies) {
   const size_t G = (size_t)s.M * s.nt;
   size_t b = 0;
@@ -356,9 +409,14 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st, De
                             ((long)d.gemm_bwd_waves * d.gemm_bwd_strips));
   // Fused Legendre step (FP64, paired rows, both tables in LDS): its waves
   // hold both products' accumulators and run one per SIMD, so one 4-wave
-  // workgroup per CU, all in one round
-  d.leg_step = (leg_step_supported(d.bwd_np / 16, d.fwd_np / 16, sh.mixed) &&
-                leg_step_lds_bytes(d.bwd_np / 16, d.fwd_np / 16) <= kLdsLimit) ? 1 : 0;
+  // workgroup per CU, all in one round.  Its parity-split form where the
+  // theta grid and the tables are symmetric under x -> -x.
+  const int form = legendre_step_form(in, sh);
+  d.sp_th = (nt / 2 + 15) / 16;
+  d.sp_te = ((nl_max + 1) / 2 + 15) / 16;
+  d.sp_to = (nl_max / 2 + 15) / 16;
+  d.leg_split = form == 2 ? 1 : 0;
+  d.leg_step = form >= 1 ? 1 : 0;
   d.step_waves = 4;
   {
     const long pstrips = ((long)M + 15) / 16, slots = kCUs;
@@ -501,6 +559,34 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st, De
   }
   d.wP = upload(wP, st);
   d.Pt = upload(Pt, st);
+  if (d.leg_split) {
+    // parity-split tables (kernels_leg_split.hip), from the first half of the
+    // theta grid; l = omega_b + 2 m (even) and omega_b + 2 m + 1 (odd)
+    const int nh = nt / 2, TH = 16 * d.sp_th, TE = 16 * d.sp_te, TO = 16 * d.sp_to;
+    std::vector<double> pe((size_t)nO * TE * TH, 0.0), po((size_t)nO * TO * TH, 0.0);
+    std::vector<double> wpe((size_t)nO * TH * TE, 0.0), wpo((size_t)nO * TH * TO, 0.0);
+    for (int b = 0; b < nO; b++) {
+      const int nl = sh.lmax - (sh.omin + b) + 1;
+      const double *pb = Pin + (size_t)b * nt * sh.legendre_n_max;
+      for (int c = 0; c < nl; c++) {
+        const int m = c / 2;
+        for (int k = 0; k < nh; k++) {
+          const double p = pb[(size_t)c * nt + k], wp = in->theta.w[k] * p;
+          if (c % 2 == 0) {
+            pe[((size_t)b * TE + m) * TH + k] = p;
+            wpe[((size_t)b * TH + k) * TE + m] = wp;
+          } else {
+            po[((size_t)b * TO + m) * TH + k] = p;
+            wpo[((size_t)b * TH + k) * TO + m] = wp;
+          }
+        }
+      }
+    }
+    d.sp_PE = upload(pe, st);
+    d.sp_PO = upload(po, st);
+    d.sp_wPE = upload(wpe, st);
+    d.sp_wPO = upload(wpo, st);
+  }
   if (sh.mixed) {
     // FP32 hi parts for the hi*hi products, bfloat16 tables for the compensation terms
     std::vector<float> h(wP.size()), h2(Pt.size());
@@ -751,17 +837,23 @@ void Engine::set_jobs(const std::vector<DeviceJob *> &jobs) {
   any_flux_ = 0;
   any_acf_ = 0;
   fused_ = jobs[0]->dev.leg_step;
+  split_ = jobs[0]->dev.leg_split;
   step_waves_ = jobs[0]->dev.step_waves;
   step_ttiles_ = step_ltiles_ = 0;
   lds_step_ = 0;
   for (int q = 0; q < n; q++) {
     const JobDev &d = jobs[q]->dev;
-    QD_REQUIRE(d.leg_step == fused_ && d.step_waves == step_waves_,
+    QD_REQUIRE(d.leg_step == fused_ && d.step_waves == step_waves_ && d.leg_split == split_,
                "all jobs of one engine must share the Legendre step form");
+    // the split kernel's tile counts address the tables: one set per engine
+    QD_REQUIRE(!split_ || (d.sp_th == jobs[0]->dev.sp_th && d.sp_te == jobs[0]->dev.sp_te &&
+                           d.sp_to == jobs[0]->dev.sp_to),
+               "all jobs of one engine must share the parity-split tile counts");
     if (fused_) {
       step_ttiles_ = std::max(step_ttiles_, d.bwd_np / 16);
       step_ltiles_ = std::max(step_ltiles_, d.fwd_np / 16);
-      lds_step_ = std::max(lds_step_, leg_step_lds_bytes(d.bwd_np / 16, d.fwd_np / 16));
+      lds_step_ = std::max(lds_step_, split_ ? leg_split_lds_bytes(d.sp_th, d.sp_te, d.sp_to)
+                                             : leg_step_lds_bytes(d.bwd_np / 16, d.fwd_np / 16));
     }
     int *single = bases.data() + (size_t)KK_COUNT * (n + 1) + (size_t)2 * KK_COUNT * q;
     for (int k = 0; k < KK_COUNT; k++) single[2 * k + 1] = d.units[k];
@@ -810,6 +902,17 @@ StepArgs Engine::args(int kind, int gate) const {
   return a;
 }
 
+// the fused Legendre kernel of this engine's jobs (direct or parity-split
+// form), fused steps or (finish) the backward transform alone
+void Engine::launch_step_kernel(const StepArgs &a, int units, int finish, hipStream_t s) const {
+  if (split_) {
+    const JobDev &d = jobs_[0]->dev;
+    launch_leg_split(a, units, d.sp_th, d.sp_te, d.sp_to, lds_step_, finish, s);
+  } else {
+    launch_leg_step(a, units, step_ttiles_, step_ltiles_, lds_step_, finish, s);
+  }
+}
+
 StepArgs Engine::args_one(int q, int kind) const {
   StepArgs a;
   a.jobs = d_jobs_ + q;
@@ -836,7 +939,7 @@ void Engine::enqueue(hipEvent_t *ev) {
     // then that previous step's flux values and record, for the jobs that
     // had one pending
     begin(QD_K_LEG_STEP);
-    launch_leg_step(args(KK_LEG_STEP), total_[KK_LEG_STEP], step_ttiles_, step_ltiles_, lds_step_, 0, s);
+    launch_step_kernel(args(KK_LEG_STEP), total_[KK_LEG_STEP], 0, s);
     end(QD_K_LEG_STEP);
     if (any_flux_) {
       begin(QD_K_OBS);
@@ -895,13 +998,12 @@ void Engine::finish(int q) {
   // the fused kernel's backward-only form: the same workgroups and partial
   // sums as the fused steps, so a record does not depend on where a run was cut
   if (q < 0) {
-    launch_leg_step(args(KK_LEG_STEP), total_[KK_LEG_STEP], step_ttiles_, step_ltiles_, lds_step_, 1, s);
+    launch_step_kernel(args(KK_LEG_STEP), total_[KK_LEG_STEP], 1, s);
     if (any_flux_) launch_flux(args(KK_FLUX), total_[KK_FLUX], s);
     launch_reduce(args(KK_REDUCE), total_[KK_REDUCE], s);
   } else {
     const JobDev &d = jobs_[q]->dev;
-    launch_leg_step(args_one(q, KK_LEG_STEP), d.units[KK_LEG_STEP], d.bwd_np / 16, d.fwd_np / 16,
-                    leg_step_lds_bytes(d.bwd_np / 16, d.fwd_np / 16), 1, s);
+    launch_step_kernel(args_one(q, KK_LEG_STEP), d.units[KK_LEG_STEP], 1, s);
     if (d.flux_on) launch_flux(args_one(q, KK_FLUX), d.units[KK_FLUX], s);
     launch_reduce(args_one(q, KK_REDUCE), d.units[KK_REDUCE], s);
   }

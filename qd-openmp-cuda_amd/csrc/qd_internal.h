@@ -64,6 +64,10 @@ struct JobDev {
   // fused Legendre step (KK_LEG_STEP): 1 when the job's steps run it; its
   // waves per workgroup, strips (16 points) per wave and workgroups per block
   int leg_step, step_waves, step_strips, step_groups;
+  // parity-split form of the fused step (theta grid and tables symmetric
+  // under x -> -x): 1 when used; tiles of 16 for the half theta grid and the
+  // even / odd l columns
+  int leg_split, sp_th, sp_te, sp_to;
   int units[KK_COUNT];       // work units (workgroups) per kernel for this job
   // physics
   double dt, dr1, dr2;
@@ -83,6 +87,11 @@ struct JobDev {
   // entry (4 c + g, n) = bfloat16 (B_lo(k_q, n), q < 4; B_hi(k_q, n), q < 4)
   // with k_q = 16 c + g + 4 q and B_hi = fp32(B), B_lo = fp32(B - B_hi)
   const QD_G uint4 *wP_x, *Pt_x;
+  // parity-split tables, padded to whole tiles with zeros (k < nt / 2,
+  // m: l = omega + 2 m (E) or omega + 2 m + 1 (O)):
+  //   sp_PE [nO][16 sp_te][16 sp_th] = P_l(x_k),  sp_PO [nO][16 sp_to][16 sp_th]
+  //   sp_wPE [nO][16 sp_th][16 sp_te] = w_k P_l(x_k),  sp_wPO [nO][16 sp_th][16 sp_to]
+  const QD_G double *sp_PE, *sp_PO, *sp_wPE, *sp_wPO;
   const QD_G double *corV;        // [L][nO][nO] eigenvectors of C_l: corV[(l nO + a) nO + b] = v_a(omega_b)
   const QD_G double2 *corPhi;     // [L][n1][nO] exp(-i dt/2 l(l+1) I1_i) exp(-i dt/2 e_a I1_i)
   const QD_G double2 *rot2;       // [L][n2] exp(-i dt/2 l(l+1) / (2 m2 r_j^2))
@@ -227,7 +236,15 @@ void launch_leg_fwd(const StepArgs &a, int total_units, int nt_tiles, int kp_max
 // forward product of its next step, strip by strip, the DVR state held in
 // registers; for the others the forward product alone.  theta_tiles = bwd_np
 // / 16 and l_tiles = fwd_np / 16 (both supported by leg_step_supported).
+bool leg_step_enabled();
 bool leg_step_supported(int theta_tiles, int l_tiles, bool mixed);
+// Parity-split form of the fused kernel (kernels_leg_split.hip): supported
+// tile counts (half theta grid, even and odd l), LDS bytes, launch (finish as
+// launch_leg_step).
+bool leg_split_supported(int th, int te, int to);
+size_t leg_split_lds_bytes(int th, int te, int to);
+void launch_leg_split(const StepArgs &a, int total_units, int th, int te, int to, size_t lds, int finish,
+                      hipStream_t stream);
 // LDS bytes of the fused kernel: both Legendre tables, padded to whole
 // chunks of 16 rows
 size_t leg_step_lds_bytes(int theta_tiles, int l_tiles);

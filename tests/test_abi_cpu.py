@@ -159,3 +159,37 @@ def test_fixture_header_roundtrip(tmp_path):
     expect = 8 + 17 * 4 + 7 * 8 + 8 * (2 * nt + n1 * n2 * nt) + 4 * nO + 8 * nt * lnmax * nO \
         + 16 * n1 * n2 * nt * nO + 8 * (n1 + n2) + 8 * 2 * 3
     assert len(raw) == expect
+
+
+def test_legendre_form_choice():
+    """The Legendre transforms a job runs (host-side choice, no GPU): the fused
+    step with parity-split products where the theta grid and the tables are
+    symmetric under x -> -x (Gauss-Legendre nodes, normalised associated
+    Legendre functions), the direct fused step where they are not, the
+    separate passes where the fused kernels have no instantiation (96 theta
+    points: 6 column tiles) and for the mixed form."""
+    job = inputs.build_job(inputs.config_spec("c3"))
+    assert api.legendre_form(abi.Inputs(job, total_steps=1)) == 2
+
+    def altered(x=None, P=None):
+        j = dict(job)
+        j["theta"] = dict(job["theta"], x=job["theta"]["x"] if x is None else x)
+        if P is not None:
+            j["omega_states"] = dict(job["omega_states"], associated_legendres=P)
+        return abi.Inputs(j, total_steps=1)
+
+    # one node moved by 1e-6: not symmetric, direct form
+    x = np.array(job["theta"]["x"], dtype=np.float64)
+    x[0] += 1e-6
+    assert api.legendre_form(altered(x=x)) == 1
+    # one table entry off its mirror image by 1e-9 of the row's largest value
+    P = np.array(job["omega_states"]["associated_legendres"], dtype=np.float64, order="F")
+    P[3, 5, 1] += 1e-9 * np.abs(P[:, 5, 1]).max()
+    assert api.legendre_form(altered(P=P)) == 1
+    # a difference at rounding level keeps the split form
+    P = np.array(job["omega_states"]["associated_legendres"], dtype=np.float64, order="F")
+    P[3, 5, 1] *= 1.0 + 1e-15
+    assert api.legendre_form(altered(P=P)) == 2
+    assert api.legendre_form(abi.Inputs(job, total_steps=1, precision=abi.QD_PREC_MIXED)) == 0
+    big = inputs.build_job(inputs.config_spec("c2"))
+    assert api.legendre_form(abi.Inputs(big, total_steps=1)) == 0

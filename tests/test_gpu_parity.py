@@ -391,11 +391,14 @@ def test_command_line_driver_matches_library(tmp_path):
     assert np.array_equal(got["crp"], ref.crp)
 
 
-def test_unpaired_legendre_kernels_match_oracle(tmp_path):
-    """QD_LEG_PAIRED=0 (read once per process, so in a child: the C++ driver)
-    selects the one-row-per-lane Legendre kernels for FP64 jobs; on the c3
-    block structure they match the oracle to the same bars as the default
-    paired-row kernels."""
+@pytest.mark.parametrize("switch", ["QD_LEG_PAIRED", "QD_LEG_STEP", "QD_LEG_SPLIT"])
+def test_legendre_kernel_forms_match_oracle(tmp_path, switch):
+    """The forms of the Legendre transforms other than the default (the fused
+    step with parity-split products), each selected by a switch read once per
+    process, so in a child (the C++ driver): QD_LEG_PAIRED=0 the one-row-per-
+    lane kernels as separate passes, QD_LEG_STEP=0 the paired-row separate
+    passes, QD_LEG_SPLIT=0 the fused step with direct products.  On the c3
+    block structure each matches the oracle to the same bars as the default."""
     import os
     import subprocess
     spec = dataclasses.replace(inputs.config_spec("c3"), n1=96, n2=64, ntheta=32)
@@ -405,8 +408,8 @@ def test_unpaired_legendre_kernels_match_oracle(tmp_path):
     inputs.write_fixture(job, str(fx), total_steps=steps)
     drv = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "qd-openmp-cuda_amd",
                        "qd_fixture_run")
-    _say("case unpaired Legendre kernels 96x64x32 J=20 K_max=4, 3 steps (child process)")
-    env = dict(os.environ, QD_LEG_PAIRED="0")
+    _say(f"case {switch}=0 Legendre kernels 96x64x32 J=20 K_max=4, 3 steps (child process)")
+    env = dict(os.environ, **{switch: "0"})
     r = subprocess.run([drv, str(fx), str(out)], capture_output=True, text=True, timeout=120, env=env)
     assert r.returncode == 0, r.stdout + r.stderr
     got = inputs.read_fixture_output(str(out), job, steps)
@@ -416,6 +419,23 @@ def test_unpaired_legendre_kernels_match_oracle(tmp_path):
     e_obs = float(np.abs(got["obs"] - ref.obs).max()) / max(1.0, float(np.abs(ref.obs).max()))
     _say(f"  psi rel {e_psi:.3e}  obs {e_obs:.3e}")
     assert e_psi <= PSI_TOL and e_obs <= OBS_TOL
+
+
+def test_parity_asymmetric_theta_grid():
+    """A theta grid whose nodes are not symmetric under x -> -x (one node moved
+    by 1e-6; the tables as given) takes the fused step with direct products
+    instead of the parity-split ones; GPU and oracle run the same inputs."""
+    spec = dataclasses.replace(inputs.config_spec("c3"), n1=96, n2=64, ntheta=32)
+    job = inputs.build_job(spec)
+    job["theta"]["x"] = np.array(job["theta"]["x"], dtype=np.float64)
+    job["theta"]["x"][0] += 1e-6
+    assert api.legendre_form(abi.Inputs(job, total_steps=1)) == 1
+    _say("case asymmetric theta grid 96x64x32 J=20 K_max=4, 4 steps")
+    a = abi.Inputs(job, total_steps=4)
+    b = abi.Inputs(job, total_steps=4)
+    oracle.evolve(a, 8)
+    api.evolve(b)
+    check_parity(a, b)
 
 
 def test_parity_c2_production_shape():
