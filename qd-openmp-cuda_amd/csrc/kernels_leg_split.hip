@@ -18,6 +18,14 @@
 // nodes, weights and tables for this symmetry (DeviceJob) and uses the
 // direct form otherwise.  The results agree with the direct sums to rounding.
 //
+// Mixed form (MIXED): every product as FP32 hi*hi MFMAs plus one bfloat16
+// compensation MFMA per 16 k (paired_mac), flushed into FP64 accumulators.
+// The FP32 16x16x4 accumulator holds column 16 t + 4 lk + it instead of
+// 16 t + lk + 4 it, so a lane's backward outputs are theta rows
+// acc_col<true>; the forward tables are stored with their rows in the order
+// the forward MFMAs consume them (row 16 c + 4 q + g holds node
+// 16 c + 4 g + q), so the hand-off stays in the lane.
+//
 // Modes, as the direct fused kernel: fused (backward product of the pending
 // step, potential energy, flux stencil rows, phase, forward product), FIN
 // (backward product alone, the DVR state stored, finishing a step sequence),
@@ -26,8 +34,53 @@
 
 namespace qd {
 
+// One k-chunk (4 k-steps, 16 k) of a mixed product from B fragments already
+// read from LDS: FP32 hi parts bh[q][t0 + t] of k-step q and the bfloat16
+// compensation entries bx[t0 + t] (paired_mac's arithmetic: hi*hi summed in
+// FP32 over the chunk, hi*lo + lo*hi on one bf16 MFMA, both added into the
+// FP64 accumulators).
+template <int NT, int NB>
+__device__ __forceinline__ void split_mac_mixed(const float (&bh)[4][NB], const v4u (&bx)[NB], int t0,
+                                                const double (&ar)[4], const double (&ai)[4], PairAcc<NT> &acc) {
+  unsigned hrb[4], lrb[4], hib[4], lib[4];
+  v4f hr[NT], hi[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) hr[t] = hi[t] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const float arh = (float)ar[q], aih = (float)ai[q];
+    hrb[q] = bf16_bits(arh);
+    lrb[q] = bf16_bits((float)(ar[q] - (double)arh));
+    hib[q] = bf16_bits(aih);
+    lib[q] = bf16_bits((float)(ai[q] - (double)aih));
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      hr[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(bh[q][t0 + t], arh, hr[t], 0, 0, 0);
+      hi[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(bh[q][t0 + t], aih, hi[t], 0, 0, 0);
+    }
+  }
+  const v8bf xa_r = bf16_operand(hrb, lrb);
+  const v8bf xa_i = bf16_operand(hib, lib);
+  const v4f zero = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const v8bf xb = __builtin_bit_cast(v8bf, bx[t0 + t]);
+    const v4f xr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xb, xa_r, zero, 0, 0, 0);
+    const v4f xi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xb, xa_i, zero, 0, 0, 0);
+#pragma unroll
+    for (int it = 0; it < 4; it++) {
+      acc.r[t][it] += (double)hr[t][it] + (double)xr[it];
+      acc.i[t][it] += (double)hi[t][it] + (double)xi[it];
+    }
+  }
+}
+
 // Device view of one block of a split job.
 struct SplitView {
+  // mixed form: FP32 hi tables and bfloat16 compensation tables in LDS
+  // (same rows and columns as the FP64 ones; compensation rows of 16 k)
+  const float *BE1h, *BO1h, *BE2h, *BO2h;
+  const v4u *BE1x, *BO1x, *BE2x, *BO2x;
   const double *fbr;        // FBR slab l = omega_b of block b, doubles, rows of 2M
   int nl;                   // l_max - omega_b + 1 (operand rows; rows past it load as zero)
   QD_G double2 *psi;        // DVR block b, [nt][M]
@@ -47,7 +100,7 @@ struct SplitView {
 // tiles to a multiple of lcm(D, 2), unrolled and fenced tile by tile; each
 // fence region issues tile tau's MFMAs with tile tau + 1's preparation (B
 // fragments from LDS, operands from the ring), as step_wave_paired does.
-template <int TH, int TE, int TO, int D, bool FIN, typename Stage>
+template <int TH, int TE, int TO, int D, bool FIN, bool MIXED, typename Stage>
 __device__ __forceinline__ void split_wave(const SplitView &v, long strip0, int nstrips, int sstride, double &ob,
                                            Stage &&stage) {
   constexpr int kCh = 4;
@@ -72,14 +125,17 @@ __device__ __forceinline__ void split_wave(const SplitView &v, long strip0, int 
   };
   struct Op {
     double ar[kCh], ai[kCh], br[kCh], bi[kCh];   // backward: FBR (a); forward: even (a) and odd (b) operands
-    double bv[kCh][NB];
+    double bv[kCh][NB];   // FP64 B fragments
+    float bh[kCh][NB];    // mixed: FP32 hi fragments and compensation entries
+    v4u bx[NB];
   };
   Slot ring[D];
   Op op[2];
   const __amdgpu_buffer_rsrc_t rA = make_rsrc(v.fbr, (uint32_t)((unsigned long)v.nl * lda * 8ul));
   const __amdgpu_buffer_rsrc_t rT = make_rsrc((const void *)v.vph, (uint32_t)((unsigned long)v.nt * lda * 8ul));
-  // theta rows of forward k-step rows: k = 16 c + 4 q + lk and its partner
-  auto kk = [&](int c, int q) { return 16 * c + 4 * q + lk; };
+  // theta row of item q of a lane's backward tile c (the operand of forward
+  // k-step 4 c + q) and its partner
+  auto kk = [&](int c, int q) { return acc_col<MIXED>(c, lk, q); };
   auto kp = [&](int c, int q) { return 2 * v.nh - 1 - kk(c, q); };
   auto load = [&](int tau, long st, Slot &r) {
     const long p = st * 16 + lr;
@@ -120,8 +176,19 @@ __device__ __forceinline__ void split_wave(const SplitView &v, long strip0, int 
       for (int q = 0; q < kCh; q++) {
         o.ar[q] = r.x[q].x;
         o.ai[q] = r.x[q].y;
+        if (!MIXED) {
 #pragma unroll
-        for (int t = 0; t < TH; t++) o.bv[q][t] = B[(16 * c + 4 * q + lk) * WsH + lr + 16 * t];
+          for (int t = 0; t < TH; t++) o.bv[q][t] = B[(16 * c + 4 * q + lk) * WsH + lr + 16 * t];
+        } else {
+          const float *Bh = tau < TE ? v.BE1h : v.BO1h;
+#pragma unroll
+          for (int t = 0; t < TH; t++) o.bh[q][t] = Bh[(16 * c + 4 * q + lk) * WsH + lr + 16 * t];
+        }
+      }
+      if (MIXED) {
+        const v4u *Bx = tau < TE ? v.BE1x : v.BO1x;
+#pragma unroll
+        for (int t = 0; t < TH; t++) o.bx[t] = Bx[(4 * c + lk) * (16 * TH) + lr + 16 * t];
       }
     } else if (tau < T0) {
       const int c = tau - TE - TO;
@@ -145,11 +212,24 @@ __device__ __forceinline__ void split_wave(const SplitView &v, long strip0, int 
           o.ai[q] = fi + gi;
           o.br[q] = fr - gr;
           o.bi[q] = fi - gi;
+          if (!MIXED) {
 #pragma unroll
-          for (int t = 0; t < TE; t++) o.bv[q][t] = v.BE2[(16 * c + 4 * q + lk) * WsE + lr + 16 * t];
+            for (int t = 0; t < TE; t++) o.bv[q][t] = v.BE2[(16 * c + 4 * q + lk) * WsE + lr + 16 * t];
 #pragma unroll
-          for (int t = 0; t < TO; t++) o.bv[q][TE + t] = v.BO2[(16 * c + 4 * q + lk) * WsO + lr + 16 * t];
+            for (int t = 0; t < TO; t++) o.bv[q][TE + t] = v.BO2[(16 * c + 4 * q + lk) * WsO + lr + 16 * t];
+          } else {   // table row 16 c + 4 q + lk holds node kk(c, q)
+#pragma unroll
+            for (int t = 0; t < TE; t++) o.bh[q][t] = v.BE2h[(16 * c + 4 * q + lk) * WsE + lr + 16 * t];
+#pragma unroll
+            for (int t = 0; t < TO; t++) o.bh[q][TE + t] = v.BO2h[(16 * c + 4 * q + lk) * WsO + lr + 16 * t];
+          }
         }
+      }
+      if (MIXED && !FIN) {
+#pragma unroll
+        for (int t = 0; t < TE; t++) o.bx[t] = v.BE2x[(4 * c + lk) * (16 * TE) + lr + 16 * t];
+#pragma unroll
+        for (int t = 0; t < TO; t++) o.bx[TE + t] = v.BO2x[(4 * c + lk) * (16 * TO) + lr + 16 * t];
       }
     }
   };
@@ -160,30 +240,39 @@ __device__ __forceinline__ void split_wave(const SplitView &v, long strip0, int 
         aE.reset();
         aO.reset();
       }
+      if constexpr (MIXED) {
+        split_mac_mixed<TH, NB>(o.bh, o.bx, 0, o.ar, o.ai, acc);
+      } else {
 #pragma unroll
-      for (int q = 0; q < kCh; q++)
+        for (int q = 0; q < kCh; q++)
 #pragma unroll
-        for (int t = 0; t < TH; t++) {
-          acc.r[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(o.bv[q][t], o.ar[q], acc.r[t], 0, 0, 0);
-          acc.i[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(o.bv[q][t], o.ai[q], acc.i[t], 0, 0, 0);
-        }
+          for (int t = 0; t < TH; t++) {
+            acc.r[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(o.bv[q][t], o.ar[q], acc.r[t], 0, 0, 0);
+            acc.i[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(o.bv[q][t], o.ai[q], acc.i[t], 0, 0, 0);
+          }
+      }
     } else if (!FIN && tau < T0) {
       const int c = tau - TE - TO;
       if (c == 0) {
         bE.reset();
         bO.reset();
       }
+      if constexpr (MIXED) {
+        split_mac_mixed<TE, NB>(o.bh, o.bx, 0, o.ar, o.ai, bE);
+        split_mac_mixed<TO, NB>(o.bh, o.bx, TE, o.br, o.bi, bO);
+      } else {
 #pragma unroll
-      for (int q = 0; q < kCh; q++) {
+        for (int q = 0; q < kCh; q++) {
 #pragma unroll
-        for (int t = 0; t < TE; t++) {
-          bE.r[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(o.bv[q][t], o.ar[q], bE.r[t], 0, 0, 0);
-          bE.i[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(o.bv[q][t], o.ai[q], bE.i[t], 0, 0, 0);
-        }
+          for (int t = 0; t < TE; t++) {
+            bE.r[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(o.bv[q][t], o.ar[q], bE.r[t], 0, 0, 0);
+            bE.i[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(o.bv[q][t], o.ai[q], bE.i[t], 0, 0, 0);
+          }
 #pragma unroll
-        for (int t = 0; t < TO; t++) {
-          bO.r[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(o.bv[q][TE + t], o.br[q], bO.r[t], 0, 0, 0);
-          bO.i[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(o.bv[q][TE + t], o.bi[q], bO.i[t], 0, 0, 0);
+          for (int t = 0; t < TO; t++) {
+            bO.r[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(o.bv[q][TE + t], o.br[q], bO.r[t], 0, 0, 0);
+            bO.i[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(o.bv[q][TE + t], o.bi[q], bO.i[t], 0, 0, 0);
+          }
         }
       }
       if (c == TH - 1) {   // streaming 16-byte stores of both parities
@@ -193,7 +282,7 @@ __device__ __forceinline__ void split_wave(const SplitView &v, long strip0, int 
         for (int t = 0; t < TE; t++)
 #pragma unroll
           for (int it = 0; it < 4; it++) {
-            const int m = acc_col<false>(t, lk, it);
+            const int m = acc_col<MIXED>(t, lk, it);
             if (pok && m < v.ne)
               __builtin_nontemporal_store(v2d{bE.re(t, it), bE.im(t, it)},
                                           (v2d *)(v.fbr + (long)(2 * m) * lda + 2 * p));
@@ -202,7 +291,7 @@ __device__ __forceinline__ void split_wave(const SplitView &v, long strip0, int 
         for (int t = 0; t < TO; t++)
 #pragma unroll
           for (int it = 0; it < 4; it++) {
-            const int m = acc_col<false>(t, lk, it);
+            const int m = acc_col<MIXED>(t, lk, it);
             if (pok && m < v.no)
               __builtin_nontemporal_store(v2d{bO.re(t, it), bO.im(t, it)},
                                           (v2d *)(v.fbr + (long)(2 * m + 1) * lda + 2 * p));
@@ -239,12 +328,13 @@ __device__ __forceinline__ void split_wave(const SplitView &v, long strip0, int 
 
 // Forward product alone (a job whose state is in DVR): per strip the TH
 // chunks load psi and the phase at k and k', form the even and odd operands
-// as the fused mode does, and feed both forward products.  Two-deep ring;
-// arithmetic identical to the fused mode's forward half.
-template <int TH, int TE, int TO>
+// as the fused mode does, and feed both forward products; arithmetic
+// identical to the fused mode's forward half.
+template <int TH, int TE, int TO, bool MIXED>
 __device__ __forceinline__ void split_forward_wave(const SplitView &v, long strip0, int nstrips, int sstride) {
   constexpr int kCh = 4;
   constexpr int WsE = lds_stride(TE), WsO = lds_stride(TO);
+  constexpr int NB = TE + TO;
   const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
   const long npts = v.M;
   const uint32_t lda = 2u * (uint32_t)v.M;
@@ -266,32 +356,57 @@ __device__ __forceinline__ void split_forward_wave(const SplitView &v, long stri
       v2d x1[kCh], x2[kCh], t1[kCh], t2[kCh];
 #pragma unroll
       for (int q = 0; q < kCh; q++) {
-        const uint32_t k1 = (uint32_t)(16 * c + 4 * q + lk), k2 = (uint32_t)(2 * v.nh - 1) - k1;
+        const uint32_t k1 = (uint32_t)acc_col<MIXED>(c, lk, q), k2 = (uint32_t)(2 * v.nh - 1) - k1;
         x1[q] = buffer_load_v2d(rP, (k1 * lda + 2u * pc) * 8u);
         x2[q] = buffer_load_v2d(rP, (k2 * lda + 2u * pc) * 8u);
         t1[q] = buffer_load_v2d(rT, (k1 * lda + 2u * pc) * 8u);
         t2[q] = buffer_load_v2d(rT, (k2 * lda + 2u * pc) * 8u);
       }
+      double ar[kCh], ai[kCh], br[kCh], bi[kCh];
 #pragma unroll
       for (int q = 0; q < kCh; q++) {
         // rows k >= nh are zero in the fused mode (E and O vanish there)
-        const bool kv = 16 * c + 4 * q + lk < v.nh;
+        const bool kv = acc_col<MIXED>(c, lk, q) < v.nh;
         const double pr = kv ? x1[q].x : 0.0, pi = kv ? x1[q].y : 0.0;
         const double qr = kv ? x2[q].x : 0.0, qi = kv ? x2[q].y : 0.0;
         const double fr = fma(-pi, t1[q].y, pr * t1[q].x), fi = fma(pr, t1[q].y, pi * t1[q].x);
         const double gr = fma(-qi, t2[q].y, qr * t2[q].x), gi = fma(qr, t2[q].y, qi * t2[q].x);
-        const double ar = fr + gr, ai = fi + gi, br = fr - gr, bi = fi - gi;
+        ar[q] = fr + gr;
+        ai[q] = fi + gi;
+        br[q] = fr - gr;
+        bi[q] = fi - gi;
+      }
+      if constexpr (MIXED) {
+        float bh[kCh][NB];
+        v4u bx[NB];
 #pragma unroll
-        for (int t = 0; t < TE; t++) {
-          const double bv = v.BE2[(16 * c + 4 * q + lk) * WsE + lr + 16 * t];
-          bE.r[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, ar, bE.r[t], 0, 0, 0);
-          bE.i[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, ai, bE.i[t], 0, 0, 0);
+        for (int q = 0; q < kCh; q++) {
+#pragma unroll
+          for (int t = 0; t < TE; t++) bh[q][t] = v.BE2h[(16 * c + 4 * q + lk) * WsE + lr + 16 * t];
+#pragma unroll
+          for (int t = 0; t < TO; t++) bh[q][TE + t] = v.BO2h[(16 * c + 4 * q + lk) * WsO + lr + 16 * t];
         }
 #pragma unroll
-        for (int t = 0; t < TO; t++) {
-          const double bv = v.BO2[(16 * c + 4 * q + lk) * WsO + lr + 16 * t];
-          bO.r[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, br, bO.r[t], 0, 0, 0);
-          bO.i[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, bi, bO.i[t], 0, 0, 0);
+        for (int t = 0; t < TE; t++) bx[t] = v.BE2x[(4 * c + lk) * (16 * TE) + lr + 16 * t];
+#pragma unroll
+        for (int t = 0; t < TO; t++) bx[TE + t] = v.BO2x[(4 * c + lk) * (16 * TO) + lr + 16 * t];
+        split_mac_mixed<TE, NB>(bh, bx, 0, ar, ai, bE);
+        split_mac_mixed<TO, NB>(bh, bx, TE, br, bi, bO);
+      } else {
+#pragma unroll
+        for (int q = 0; q < kCh; q++) {
+#pragma unroll
+          for (int t = 0; t < TE; t++) {
+            const double bv = v.BE2[(16 * c + 4 * q + lk) * WsE + lr + 16 * t];
+            bE.r[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, ar[q], bE.r[t], 0, 0, 0);
+            bE.i[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, ai[q], bE.i[t], 0, 0, 0);
+          }
+#pragma unroll
+          for (int t = 0; t < TO; t++) {
+            const double bv = v.BO2[(16 * c + 4 * q + lk) * WsO + lr + 16 * t];
+            bO.r[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, br[q], bO.r[t], 0, 0, 0);
+            bO.i[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, bi[q], bO.i[t], 0, 0, 0);
+          }
         }
       }
     }
@@ -299,7 +414,7 @@ __device__ __forceinline__ void split_forward_wave(const SplitView &v, long stri
     for (int t = 0; t < TE; t++)
 #pragma unroll
       for (int it = 0; it < 4; it++) {
-        const int m = acc_col<false>(t, lk, it);
+        const int m = acc_col<MIXED>(t, lk, it);
         if (pok && m < v.ne)
           __builtin_nontemporal_store(v2d{bE.re(t, it), bE.im(t, it)}, (v2d *)(v.fbr + (long)(2 * m) * lda + 2 * p));
       }
@@ -307,7 +422,7 @@ __device__ __forceinline__ void split_forward_wave(const SplitView &v, long stri
     for (int t = 0; t < TO; t++)
 #pragma unroll
       for (int it = 0; it < 4; it++) {
-        const int m = acc_col<false>(t, lk, it);
+        const int m = acc_col<MIXED>(t, lk, it);
         if (pok && m < v.no)
           __builtin_nontemporal_store(v2d{bO.re(t, it), bO.im(t, it)},
                                       (v2d *)(v.fbr + (long)(2 * m + 1) * lda + 2 * p));
@@ -317,8 +432,10 @@ __device__ __forceinline__ void split_forward_wave(const SplitView &v, long stri
 
 // Workgroup (block b, group g): 4 waves, one per SIMD, the strips
 // g S 4 + wave + 4 s of block b (the fused kernel's decomposition).  LDS: the
-// forward tables wP_E, wP_O, then (fused, FIN) the backward tables P_E, P_O.
-template <int TH, int TE, int TO, bool FIN>
+// forward tables wP_E, wP_O, then (fused, FIN) the backward tables P_E, P_O;
+// mixed: the four compensation tables (16-byte entries, rows of 16 k), then
+// the four FP32 hi tables.
+template <int TH, int TE, int TO, bool FIN, bool MIXED>
 __global__ __launch_bounds__(4 * 64) void k_leg_split(StepArgs a) {
   constexpr int WAVES = 4, TB = WAVES * 64;
   constexpr int WsH = lds_stride(TH), WsE = lds_stride(TE), WsO = lds_stride(TO);
@@ -334,24 +451,51 @@ __global__ __launch_bounds__(4 * 64) void k_leg_split(StepArgs a) {
   const long G = M * jb.nt;
   const bool fused = jb.counters[3] != 0;   // the job's backward transform is pending
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  SplitView v;
   double *BE2 = lds_d;                       // [16 TH][WsE]
   double *BO2 = BE2 + 16 * TH * WsE;         // [16 TH][WsO]
   double *BE1 = BO2 + 16 * TH * WsO;         // [16 TE][WsH]
   double *BO1 = BE1 + 16 * TE * WsH;         // [16 TO][WsH]
+  v4u *BE2x = (v4u *)lds_d;                  // [4 TH][16 TE]
+  v4u *BO2x = BE2x + 4 * TH * 16 * TE;       // [4 TH][16 TO]
+  v4u *BE1x = BO2x + 4 * TH * 16 * TO;       // [4 TE][16 TH]
+  v4u *BO1x = BE1x + 4 * TE * 16 * TH;       // [4 TO][16 TH]
+  float *BE2h = (float *)(BO1x + 4 * TO * 16 * TH);   // as the FP64 tables
+  float *BO2h = BE2h + 16 * TH * WsE;
+  float *BE1h = BO2h + 16 * TH * WsO;
+  float *BO1h = BE1h + 16 * TE * WsH;
   const bool need_fwd = !FIN, need_bwd = fused;
   auto stage = [&]() {
     // tables are stored padded to whole 16-row / 16-column chunks
-    if (need_fwd) {
-      stage_b<TB, 16 * TE>(BE2, jb.sp_wPE + (long)b * 16 * TH * 16 * TE, 16 * TH, 16 * TE, WsE);
-      stage_b<TB, 16 * TO>(BO2, jb.sp_wPO + (long)b * 16 * TH * 16 * TO, 16 * TH, 16 * TO, WsO);
-    }
-    if (need_bwd) {
-      stage_b<TB, 16 * TH>(BE1, jb.sp_PE + (long)b * 16 * TE * 16 * TH, 16 * TE, 16 * TH, WsH);
-      stage_b<TB, 16 * TH>(BO1, jb.sp_PO + (long)b * 16 * TO * 16 * TH, 16 * TO, 16 * TH, WsH);
+    if constexpr (!MIXED) {
+      if (need_fwd) {
+        stage_b<TB, 16 * TE>(BE2, jb.sp_wPE + (long)b * 16 * TH * 16 * TE, 16 * TH, 16 * TE, WsE);
+        stage_b<TB, 16 * TO>(BO2, jb.sp_wPO + (long)b * 16 * TH * 16 * TO, 16 * TH, 16 * TO, WsO);
+      }
+      if (need_bwd) {
+        stage_b<TB, 16 * TH>(BE1, jb.sp_PE + (long)b * 16 * TE * 16 * TH, 16 * TE, 16 * TH, WsH);
+        stage_b<TB, 16 * TH>(BO1, jb.sp_PO + (long)b * 16 * TO * 16 * TH, 16 * TO, 16 * TH, WsH);
+      }
+    } else {
+      if (need_fwd) {
+        stage_b<TB, 16 * TE>(BE2h, jb.sp_hwPE + (long)b * 16 * TH * 16 * TE, 16 * TH, 16 * TE, WsE);
+        stage_b<TB, 16 * TO>(BO2h, jb.sp_hwPO + (long)b * 16 * TH * 16 * TO, 16 * TH, 16 * TO, WsO);
+        stage_b<TB, 16 * TE>(BE2x, (const QD_G v4u *)jb.sp_xwPE + (long)b * 4 * TH * 16 * TE, 4 * TH, 16 * TE,
+                             16 * TE);
+        stage_b<TB, 16 * TO>(BO2x, (const QD_G v4u *)jb.sp_xwPO + (long)b * 4 * TH * 16 * TO, 4 * TH, 16 * TO,
+                             16 * TO);
+      }
+      if (need_bwd) {
+        stage_b<TB, 16 * TH>(BE1h, jb.sp_hPE + (long)b * 16 * TE * 16 * TH, 16 * TE, 16 * TH, WsH);
+        stage_b<TB, 16 * TH>(BO1h, jb.sp_hPO + (long)b * 16 * TO * 16 * TH, 16 * TO, 16 * TH, WsH);
+        stage_b<TB, 16 * TH>(BE1x, (const QD_G v4u *)jb.sp_xPE + (long)b * 4 * TE * 16 * TH, 4 * TE, 16 * TH,
+                             16 * TH);
+        stage_b<TB, 16 * TH>(BO1x, (const QD_G v4u *)jb.sp_xPO + (long)b * 4 * TO * 16 * TH, 4 * TO, 16 * TH,
+                             16 * TH);
+      }
     }
     __syncthreads();
   };
-  SplitView v;
   v.fbr = (const double *)(jb.fbr + ((long)b * jb.L + om) * M);
   v.nl = nl;
   v.psi = jb.psi + b * G;
@@ -369,16 +513,24 @@ __global__ __launch_bounds__(4 * 64) void k_leg_split(StepArgs a) {
   v.BO1 = BO1;
   v.BE2 = BE2;
   v.BO2 = BO2;
+  v.BE1h = BE1h;
+  v.BO1h = BO1h;
+  v.BE2h = BE2h;
+  v.BO2h = BO2h;
+  v.BE1x = BE1x;
+  v.BO1x = BO1x;
+  v.BE2x = BE2x;
+  v.BO2x = BO2x;
   const int S = jb.step_strips;
   const long strip0 = (long)g * S * WAVES + wave;
   if (!fused) {
     if (!FIN) {
       stage();
-      split_forward_wave<TH, TE, TO>(v, strip0, S, WAVES);
+      split_forward_wave<TH, TE, TO, MIXED>(v, strip0, S, WAVES);
     }
   } else {
     double ob = 0.0;
-    split_wave<TH, TE, TO, 3, FIN>(v, strip0, S, WAVES, ob, stage);
+    split_wave<TH, TE, TO, 3, FIN, MIXED>(v, strip0, S, WAVES, ob, stage);
     ob = wave_sum64(ob);
     if (lane == 0) red[wave] = ob;
     __syncthreads();
@@ -401,18 +553,28 @@ bool leg_split_supported(int th, int te, int to) {
   }
 }
 
-size_t leg_split_lds_bytes(int th, int te, int to) {
-  return (size_t)16 * ((size_t)th * lds_stride(te) + (size_t)th * lds_stride(to) + (size_t)te * lds_stride(th) +
-                       (size_t)to * lds_stride(th)) * sizeof(double);
+size_t leg_split_lds_bytes(int th, int te, int to, int mixed) {
+  const size_t f = (size_t)16 * ((size_t)th * lds_stride(te) + (size_t)th * lds_stride(to) +
+                                 (size_t)te * lds_stride(th) + (size_t)to * lds_stride(th));
+  if (!mixed) return f * sizeof(double);
+  // compensation entries: 4 rows of 16-byte entries per 16 k, all columns
+  const size_t x = (size_t)4 * 16 * ((size_t)th * te + (size_t)th * to + (size_t)te * th + (size_t)to * th);
+  return x * 16 + f * sizeof(float);
 }
 
-void launch_leg_split(const StepArgs &a, int total_units, int th, int te, int to, size_t lds, int finish,
+void launch_leg_split(const StepArgs &a, int total_units, int th, int te, int to, int mixed, size_t lds, int finish,
                       hipStream_t s) {
+  const dim3 grid(total_units), block(256);
   switch (th * 64 + te * 8 + to) {
-#define QD_SPLIT_LAUNCH(H, E, O)                                                                            \
-  case H * 64 + E * 8 + O:                                                                                  \
-    if (finish) hipLaunchKernelGGL((k_leg_split<H, E, O, true>), dim3(total_units), dim3(256), lds, s, a);  \
-    else hipLaunchKernelGGL((k_leg_split<H, E, O, false>), dim3(total_units), dim3(256), lds, s, a);        \
+#define QD_SPLIT_LAUNCH(H, E, O)                                                         \
+  case H * 64 + E * 8 + O:                                                               \
+    if (mixed) {                                                                         \
+      if (finish) hipLaunchKernelGGL((k_leg_split<H, E, O, true, true>), grid, block, lds, s, a);   \
+      else hipLaunchKernelGGL((k_leg_split<H, E, O, false, true>), grid, block, lds, s, a);         \
+    } else {                                                                             \
+      if (finish) hipLaunchKernelGGL((k_leg_split<H, E, O, true, false>), grid, block, lds, s, a);  \
+      else hipLaunchKernelGGL((k_leg_split<H, E, O, false, false>), grid, block, lds, s, a);        \
+    }                                                                                    \
     break;
     QD_SPLIT_CASES(QD_SPLIT_LAUNCH)
 #undef QD_SPLIT_LAUNCH
@@ -422,10 +584,12 @@ void launch_leg_split(const StepArgs &a, int total_units, int th, int te, int to
 
 void init_attrs_leg_split() {
   const int lim = 160 * 1024;
-#define QD_SPLIT_ATTR(H, E, O)                                                                             \
-  (void)hipFuncSetAttribute((const void *)k_leg_split<H, E, O, false>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                            lim);                                                                          \
-  (void)hipFuncSetAttribute((const void *)k_leg_split<H, E, O, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+  const hipFuncAttribute at = hipFuncAttributeMaxDynamicSharedMemorySize;
+#define QD_SPLIT_ATTR(H, E, O)                                                            \
+  (void)hipFuncSetAttribute((const void *)k_leg_split<H, E, O, false, false>, at, lim);   \
+  (void)hipFuncSetAttribute((const void *)k_leg_split<H, E, O, true, false>, at, lim);    \
+  (void)hipFuncSetAttribute((const void *)k_leg_split<H, E, O, false, true>, at, lim);    \
+  (void)hipFuncSetAttribute((const void *)k_leg_split<H, E, O, true, true>, at, lim);
   QD_SPLIT_CASES(QD_SPLIT_ATTR)
 #undef QD_SPLIT_ATTR
 }

@@ -326,8 +326,8 @@ int legendre_step_form(const qd_inputs *in, const JobShape &sh) {
   const int nt = sh.nt, nl_max = sh.lmax - sh.omin + 1;
   const int theta_tiles = round_up(nt, 16) / 16, l_tiles = round_up(nl_max, 16) / 16;
   const int th = (nt / 2 + 15) / 16, te = ((nl_max + 1) / 2 + 15) / 16, to = (nl_max / 2 + 15) / 16;
-  if (leg_step_enabled() && leg_split_enabled() && !sh.mixed && nt % 2 == 0 && nl_max >= 2 &&
-      leg_split_supported(th, te, to) && leg_split_lds_bytes(th, te, to) <= kLdsLimit && legendre_symmetric(in, sh))
+  if (leg_step_enabled() && leg_split_enabled() && nt % 2 == 0 && nl_max >= 2 && leg_split_supported(th, te, to) &&
+      leg_split_lds_bytes(th, te, to, sh.mixed) <= kLdsLimit && legendre_symmetric(in, sh))
     return 2;
   if (leg_step_supported(theta_tiles, l_tiles, sh.mixed) && leg_step_lds_bytes(theta_tiles, l_tiles) <= kLdsLimit)
     return 1;
@@ -586,6 +586,33 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st, De
     d.sp_PO = upload(po, st);
     d.sp_wPE = upload(wpe, st);
     d.sp_wPO = upload(wpo, st);
+    if (sh.mixed) {
+      // forward tables in the FP32 MFMA's row order (row 16 c + 4 q + g holds
+      // node 16 c + 4 g + q), backward tables as they are
+      auto permuted = [&](const std::vector<double> &t, int cols) {
+        std::vector<double> o(t.size());
+        for (int b = 0; b < nO; b++)
+          for (int r = 0; r < TH; r++) {
+            const int k = (r & ~15) + 4 * (r & 3) + ((r >> 2) & 3);
+            std::memcpy(&o[((size_t)b * TH + r) * cols], &t[((size_t)b * TH + k) * cols], sizeof(double) * cols);
+          }
+        return o;
+      };
+      auto hi = [](const std::vector<double> &t) {
+        std::vector<float> h(t.size());
+        for (size_t q = 0; q < t.size(); q++) h[q] = (float)t[q];
+        return h;
+      };
+      const std::vector<double> wpe_p = permuted(wpe, TE), wpo_p = permuted(wpo, TO);
+      d.sp_hPE = upload(hi(pe), st);
+      d.sp_hPO = upload(hi(po), st);
+      d.sp_hwPE = upload(hi(wpe_p), st);
+      d.sp_hwPO = upload(hi(wpo_p), st);
+      d.sp_xPE = upload(bf16_compensation_table(pe, nO, TE, TH), st);
+      d.sp_xPO = upload(bf16_compensation_table(po, nO, TO, TH), st);
+      d.sp_xwPE = upload(bf16_compensation_table(wpe_p, nO, TH, TE), st);
+      d.sp_xwPO = upload(bf16_compensation_table(wpo_p, nO, TH, TO), st);
+    }
   }
   if (sh.mixed) {
     // FP32 hi parts for the hi*hi products, bfloat16 tables for the compensation terms
@@ -852,7 +879,7 @@ void Engine::set_jobs(const std::vector<DeviceJob *> &jobs) {
     if (fused_) {
       step_ttiles_ = std::max(step_ttiles_, d.bwd_np / 16);
       step_ltiles_ = std::max(step_ltiles_, d.fwd_np / 16);
-      lds_step_ = std::max(lds_step_, split_ ? leg_split_lds_bytes(d.sp_th, d.sp_te, d.sp_to)
+      lds_step_ = std::max(lds_step_, split_ ? leg_split_lds_bytes(d.sp_th, d.sp_te, d.sp_to, d.mixed)
                                              : leg_step_lds_bytes(d.bwd_np / 16, d.fwd_np / 16));
     }
     int *single = bases.data() + (size_t)KK_COUNT * (n + 1) + (size_t)2 * KK_COUNT * q;
@@ -907,7 +934,7 @@ StepArgs Engine::args(int kind, int gate) const {
 void Engine::launch_step_kernel(const StepArgs &a, int units, int finish, hipStream_t s) const {
   if (split_) {
     const JobDev &d = jobs_[0]->dev;
-    launch_leg_split(a, units, d.sp_th, d.sp_te, d.sp_to, lds_step_, finish, s);
+    launch_leg_split(a, units, d.sp_th, d.sp_te, d.sp_to, mixed_, lds_step_, finish, s);
   } else {
     launch_leg_step(a, units, step_ttiles_, step_ltiles_, lds_step_, finish, s);
   }

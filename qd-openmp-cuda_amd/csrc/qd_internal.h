@@ -92,6 +92,12 @@ struct JobDev {
   //   sp_PE [nO][16 sp_te][16 sp_th] = P_l(x_k),  sp_PO [nO][16 sp_to][16 sp_th]
   //   sp_wPE [nO][16 sp_th][16 sp_te] = w_k P_l(x_k),  sp_wPO [nO][16 sp_th][16 sp_to]
   const QD_G double *sp_PE, *sp_PO, *sp_wPE, *sp_wPO;
+  // mixed form of the split tables: FP32 hi parts (same layout) and bfloat16
+  // compensation tables (bf16_compensation_table layout, k = the table's rows).
+  // The forward tables' rows are permuted within each 16-row chunk: row
+  // 16 c + 4 q + g holds node 16 c + 4 g + q (the FP32 MFMA's output order)
+  const QD_G float *sp_hPE, *sp_hPO, *sp_hwPE, *sp_hwPO;
+  const QD_G uint4 *sp_xPE, *sp_xPO, *sp_xwPE, *sp_xwPO;
   const QD_G double *corV;        // [L][nO][nO] eigenvectors of C_l: corV[(l nO + a) nO + b] = v_a(omega_b)
   const QD_G double2 *corPhi;     // [L][n1][nO] exp(-i dt/2 l(l+1) I1_i) exp(-i dt/2 e_a I1_i)
   const QD_G double2 *rot2;       // [L][n2] exp(-i dt/2 l(l+1) / (2 m2 r_j^2))
@@ -240,10 +246,10 @@ bool leg_step_enabled();
 bool leg_step_supported(int theta_tiles, int l_tiles, bool mixed);
 // Parity-split form of the fused kernel (kernels_leg_split.hip): supported
 // tile counts (half theta grid, even and odd l), LDS bytes, launch (finish as
-// launch_leg_step).
+// launch_leg_step); mixed: the FP32 hi / bfloat16 compensation form.
 bool leg_split_supported(int th, int te, int to);
-size_t leg_split_lds_bytes(int th, int te, int to);
-void launch_leg_split(const StepArgs &a, int total_units, int th, int te, int to, size_t lds, int finish,
+size_t leg_split_lds_bytes(int th, int te, int to, int mixed);
+void launch_leg_split(const StepArgs &a, int total_units, int th, int te, int to, int mixed, size_t lds, int finish,
                       hipStream_t stream);
 // LDS bytes of the fused kernel: both Legendre tables, padded to whole
 // chunks of 16 rows

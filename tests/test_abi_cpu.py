@@ -167,16 +167,17 @@ def test_legendre_form_choice():
     symmetric under x -> -x (Gauss-Legendre nodes, normalised associated
     Legendre functions), the direct fused step where they are not, the
     separate passes where the fused kernels have no instantiation (96 theta
-    points: 6 column tiles) and for the mixed form."""
+    points: 6 column tiles).  The mixed form has the split fused step only:
+    on an asymmetric grid it runs the separate passes."""
     job = inputs.build_job(inputs.config_spec("c3"))
     assert api.legendre_form(abi.Inputs(job, total_steps=1)) == 2
 
-    def altered(x=None, P=None):
+    def altered(x=None, P=None, precision=abi.QD_PREC_FP64):
         j = dict(job)
         j["theta"] = dict(job["theta"], x=job["theta"]["x"] if x is None else x)
         if P is not None:
             j["omega_states"] = dict(job["omega_states"], associated_legendres=P)
-        return abi.Inputs(j, total_steps=1)
+        return abi.Inputs(j, total_steps=1, precision=precision)
 
     # one node moved by 1e-6: not symmetric, direct form
     x = np.array(job["theta"]["x"], dtype=np.float64)
@@ -190,6 +191,7 @@ def test_legendre_form_choice():
     P = np.array(job["omega_states"]["associated_legendres"], dtype=np.float64, order="F")
     P[3, 5, 1] *= 1.0 + 1e-15
     assert api.legendre_form(altered(P=P)) == 2
-    assert api.legendre_form(abi.Inputs(job, total_steps=1, precision=abi.QD_PREC_MIXED)) == 0
+    assert api.legendre_form(abi.Inputs(job, total_steps=1, precision=abi.QD_PREC_MIXED)) == 2
+    assert api.legendre_form(altered(x=x, precision=abi.QD_PREC_MIXED)) == 0
     big = inputs.build_job(inputs.config_spec("c2"))
     assert api.legendre_form(abi.Inputs(big, total_steps=1)) == 0
