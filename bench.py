@@ -31,6 +31,8 @@ PKG_DIR = os.path.join(ROOT, "qd-openmp-cuda_amd")
 
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFLOPS = 78.6
+# the Legendre transforms a job runs (api.legendre_form)
+LEGENDRE_FORMS = {0: "separate passes", 1: "fused step, direct products", 2: "fused step, parity-split products"}
 FP32_MFMA_PEAK_TFLOPS = 157.3
 
 
@@ -138,34 +140,42 @@ def cpu_baseline(job, abi, n_steps=2):
             "single_thread": {"value": pts / dt1, "sample": f"1 step on 1 thread, {dt1:.2f} s"}}
 
 
-def mixed_crp_error(api, abi, inputs, steps):
-    """Reaction probabilities of the mixed form against the FP64 form on the
-    256x192x96 J=0 problem at 200 energies (FP64 parity with the oracle is
-    established by the test suite).  Energies where eta_sq is below 1e-8 of
-    its maximum carry no flux information (P = flux / eta_sq is then
-    dominated by rounding in either form) and are reported separately."""
-    spec = inputs.config_spec("c4", total_steps=steps)
-    job = inputs.build_job(spec)
+def mixed_crp_error(api, abi, inputs):
+    """Reaction probabilities of both forms against the FP64 oracle's P(E) on
+    the 256x192x96 J=0 problem at 200 energies, 2000 steps (the committed
+    fixture tests/golden/c4_oracle_crp.npz, made by
+    tests/golden/make_c4_oracle.py; the flux has crossed the surface by
+    then).  Per energy: E_coll, eta_sq / max, the oracle's P and |dP| of
+    each form; and the largest |dP| of each form over the energies where
+    eta_sq is at least 1e-2, 1e-3, 1e-4 and 1e-6 of its maximum (below
+    that, P = flux / eta_sq divides rounding noise by eta_sq)."""
+    gold = np.load(os.path.join(ROOT, "tests", "golden", "c4_oracle_crp.npz"))
+    steps = int(gold["steps"])
+    job = inputs.build_job(inputs.config_spec("c4", total_steps=steps))
+    assert np.array_equal(job["crp"]["energies"], gold["energies"])
+    ref = gold["crp"]
     out = {}
-    crp = {}
+    err = {}
     for name, prec in (("fp64", abi.QD_PREC_FP64), ("mixed", abi.QD_PREC_MIXED)):
         inp = abi.Inputs(job, precision=prec, want_acf=False, want_observables=True, final_copy=False)
+        out[f"{name}_legendre_form"] = LEGENDRE_FORMS[api.legendre_form(inp)]
         t0 = time.perf_counter()
         api.evolve(inp)
         out[f"{name}_s"] = time.perf_counter() - t0
-        crp[name] = inp.crp.copy()
+        err[name] = np.abs(inp.crp - ref)
         out[f"{name}_final_norm"] = float(inp.obs[-1, :, 0].sum())
-    eta = job["crp"]["eta_sq"]
-    ok = eta >= 1e-8 * eta.max()
-    err = np.abs(crp["mixed"] - crp["fp64"])
+    out["oracle_final_norm"] = float(gold["norm_final"])
+    eta = job["crp"]["eta_sq"] / job["crp"]["eta_sq"].max()
     E = job["crp"]["energies"] - job["meta"]["e_int"]
     out.update({
         "workload": f"256x192x96, J=0, dt=10 au, {steps} steps, 200 collision energies in [0.01, 0.3] hartree",
-        "energies_resolved": int(ok.sum()),
-        "max_abs_err_resolved": float(err[ok].max()) if ok.any() else None,
-        "max_abs_err_all": float(err.max()),
-        "per_energy": [{"E_coll": float(e), "P_fp64": float(p), "abs_err": float(d), "eta_sq": float(h)}
-                       for e, p, d, h in zip(E, crp["fp64"], err, eta)],
+        "reference": "FP64 oracle P(E), tests/golden/c4_oracle_crp.npz",
+        "by_eta": [{"eta_over_max_at_least": f, "energies": int((eta >= f).sum()),
+                    "max_abs_err_fp64": float(err["fp64"][eta >= f].max()),
+                    "max_abs_err_mixed": float(err["mixed"][eta >= f].max())} for f in (1e-2, 1e-3, 1e-4, 1e-6)],
+        "per_energy": [{"E_coll": float(e), "eta_over_max": float(h), "P_oracle": float(p),
+                        "abs_err_fp64": float(d64), "abs_err_mixed": float(dmx)}
+                       for e, h, p, d64, dmx in zip(E, eta, ref, err["fp64"], err["mixed"])],
     })
     return out
 
@@ -293,7 +303,6 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--c4", action="store_true",
                     help="only run the mixed-precision CRP comparison (256x192x96, J=0, 200 energies) and exit")
-    ap.add_argument("--c4-steps", type=int, default=20000)
     ap.add_argument("--serve-jobs", type=int, default=64)
     ap.add_argument("--serve-steps", type=int, default=200)
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -324,7 +333,7 @@ def main():
         return
 
     if args.c4:
-        print(json.dumps(mixed_crp_error(api, abi, inputs, args.c4_steps)))
+        print(json.dumps(mixed_crp_error(api, abi, inputs)))
         return
 
     if args.serve_only:
@@ -378,8 +387,7 @@ def main():
                 "frac": achieved / HBM_PEAK_GBS}
     roof["traffic"] = read_traffic(dom)
     roof["kernel"] = dom
-    roof["legendre_form"] = ("fused step, parity-split products" if form == 2 else
-                             "fused step, direct products" if form == 1 else "separate passes")
+    roof["legendre_form"] = LEGENDRE_FORMS[form]
     roof["kernel_us"] = us
     roof["algorithmic_bytes"] = m["bytes"]
     roof["algorithmic_flops"] = m["flops"]
