@@ -20,6 +20,16 @@ __host__ __device__ constexpr int fft_stage_count(int n) {
   return s;
 }
 
+// Store of one complex value of an FFT pass's output, which the next pass
+// reads once: streaming (nontemporal) when the job's arrays are larger than
+// the Infinity Cache (stream, uniform), a plain store otherwise, so that the
+// next pass finds it in the cache.
+__device__ __forceinline__ void store_out(QD_G double2 *p, double2 v, int stream) {
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  if (stream) __builtin_nontemporal_store(d2{v.x, v.y}, (QD_G d2 *)p);
+  else *p = v;
+}
+
 template <bool INV>
 __device__ __forceinline__ double2 rot_i(double2 a) {   // multiply by -i (forward) / +i (inverse)
   return INV ? cmul_pi(a) : cmul_mi(a);

@@ -33,11 +33,58 @@ struct PlanEnds {
   }
 };
 
+// R-pass LDS layout (SW).  16-byte LDS stores are served 8 lanes at a time
+// over 128 bytes (32 banks), loads 16 lanes at a time over 256 bytes; a
+// wave's access is conflict-free when every group spreads over distinct
+// 16-byte slots (slot mod 8 for stores, mod 16 for loads), measured with
+// SQ_LDS_BANK_CONFLICT on fixed lane patterns.
+//  - Tile element i sits in slot swz(i): bits 0-1 XOR bits 3-4 and bit 2 XOR
+//    bit 4.  The stores of the radix-4 stages, at lane stride 4 (stride-1
+//    stage: 4 i + q) and in runs of 4 at stride 16 (stride-4 stage:
+//    16 (i / 4) + i % 4 + 4 q), which unswizzled fall on 2 and 4 of the 8
+//    slots, cover all 8; each aligned group of 8 elements keeps its slots, so
+//    the unit-stride loads (every stage's inputs, the last stage, the point
+//    map) stay conflict-free.
+//  - Twiddles in stage order (tw_stage_offset): stage NS holds w^(q k N / (NS R))
+//    at (q - 1) NS + k, so a stage reads them at unit stride in k instead of at
+//    stride q N / (NS R).
+template <bool SW> __device__ __forceinline__ int swz(int i) {
+  return SW ? i ^ ((i >> 3) & 3) ^ (((i >> 4) & 1) << 2) : i;
+}
+// Offset of stage NS's twiddles in the stage-ordered table (stages of
+// stride 1 have none); tw_stage_offset(n, n) is the table's length.
+template <class PLAN> __host__ __device__ constexpr int tw_stage_offset(int n, int ns_stop) {
+  int off = 0;
+  for (int ns = 1; ns < ns_stop; ns *= PLAN::radix(n, ns))
+    if (ns > 1) off += (PLAN::radix(n, ns) - 1) * ns;
+  return off;
+}
+// Index into the natural twiddle table (w^j, j < N) of entry e of the
+// stage-ordered one (stages unrolled at compile time).
+template <class PLAN, int N, int NS = 1, int OFF = 0>
+__device__ __forceinline__ int tw_stage_source(int e) {
+  if constexpr (NS >= N) {
+    return 0;
+  } else {
+    constexpr int R = PLAN::radix(N, NS);
+    if constexpr (NS == 1) {
+      return tw_stage_source<PLAN, N, NS * R, OFF>(e);
+    } else {
+      if (e < OFF + (R - 1) * NS) {
+        const int d = e - OFF;
+        return (d / NS + 1) * (d % NS) * (N / (NS * R));
+      }
+      return tw_stage_source<PLAN, N, NS * R, OFF + (R - 1) * NS>(e);
+    }
+  }
+}
+
 // Stockham stages NS.. (up to, not including, the stage of stride NSEND) of
 // an in-place transform of NPEN pencils; pencil p element e is
-// A[p * PS + e * ES].  PF maps consecutive threads to consecutive pencils.
-// Entry and exit are synchronised.
-template <bool INV, int N, int NS, int NPEN, int PS, int ES, bool PF, class PLAN = PlanStd, int NSEND = N>
+// A[p * PS + e * ES] (slot swz<SW> of it).  PF maps consecutive threads to
+// consecutive pencils.  Entry and exit are synchronised.
+template <bool INV, int N, int NS, int NPEN, int PS, int ES, bool PF, class PLAN = PlanStd, int NSEND = N,
+          bool SW = false>
 __device__ __forceinline__ void stages_ip(double2 *A, const double2 *tw) {
   if constexpr (NS < NSEND) {
     constexpr int R = PLAN::radix(N, NS);
@@ -45,6 +92,7 @@ __device__ __forceinline__ void stages_ip(double2 *A, const double2 *tw) {
     constexpr int TOTAL = NPEN * NB;
     constexpr int MB = (TOTAL + kBlock - 1) / kBlock;
     constexpr int TWS = N / (NS * R);
+    constexpr int TWO = tw_stage_offset<PLAN>(N, NS);
     double2 X[MB][R];
 #pragma unroll
     for (int m = 0; m < MB; m++) {
@@ -54,12 +102,12 @@ __device__ __forceinline__ void stages_ip(double2 *A, const double2 *tw) {
         const int idx = PF ? t / NPEN : t % NB;
         double2 v[R];
 #pragma unroll
-        for (int q = 0; q < R; q++) v[q] = A[p * PS + (idx + q * NB) * ES];
+        for (int q = 0; q < R; q++) v[q] = A[swz<SW>(p * PS + (idx + q * NB) * ES)];
         if constexpr (NS > 1) {
           const int k = idx % NS;
 #pragma unroll
           for (int q = 1; q < R; q++) {
-            double2 w = tw[q * k * TWS];
+            double2 w = SW ? tw[TWO + (q - 1) * NS + k] : tw[q * k * TWS];
             if (INV) w.y = -w.y;
             v[q] = cmul(v[q], w);
           }
@@ -76,11 +124,11 @@ __device__ __forceinline__ void stages_ip(double2 *A, const double2 *tw) {
         const int idx = PF ? t / NPEN : t % NB;
         const int d = (idx / NS) * NS * R + idx % NS;
 #pragma unroll
-        for (int q = 0; q < R; q++) A[p * PS + (d + q * NS) * ES] = X[m][q];
+        for (int q = 0; q < R; q++) A[swz<SW>(p * PS + (d + q * NS) * ES)] = X[m][q];
       }
     }
     __syncthreads();
-    stages_ip<INV, N, NS * R, NPEN, PS, ES, PF, PLAN, NSEND>(A, tw);
+    stages_ip<INV, N, NS * R, NPEN, PS, ES, PF, PLAN, NSEND, SW>(A, tw);
   }
 }
 
@@ -99,9 +147,10 @@ __host__ __device__ constexpr int last_stage_ns(int n) {
 // Its inputs are exactly its outputs' positions, so a thread that writes
 // back only these positions needs no barrier before it.  Entry must be
 // synchronised.
-template <bool INV, int N, int NS, int NPEN, int XPT>
+template <bool INV, int N, int NS, int NPEN, int XPT, bool SW = false>
 __device__ __forceinline__ void last_stage_own(const double2 *A, const double2 *tw, double2 (&y)[XPT]) {
   constexpr int R = N / NS;
+  constexpr int TWO = tw_stage_offset<PlanStd>(N, NS);
   constexpr int MBL = XPT / R;
   constexpr int TL = NPEN * NS;
 #pragma unroll
@@ -111,11 +160,11 @@ __device__ __forceinline__ void last_stage_own(const double2 *A, const double2 *
     const int p = t / NS, idx = t % NS;
     double2 v[R], X[R];
 #pragma unroll
-    for (int q = 0; q < R; q++) v[q] = A[p * N + idx + q * NS];
+    for (int q = 0; q < R; q++) v[q] = A[swz<SW>(p * N + idx + q * NS)];
     if constexpr (NS > 1) {
 #pragma unroll
       for (int q = 1; q < R; q++) {
-        double2 w = tw[q * idx];
+        double2 w = SW ? tw[TWO + (q - 1) * NS + idx] : tw[q * idx];
         if (INV) w.y = -w.y;
         v[q] = cmul(v[q], w);
       }
@@ -208,8 +257,11 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
     if constexpr (OWN) return TL % kBlock == 0 || (int)threadIdx.x + (k / RL) * kBlock < TL;
     else return PN % kBlock == 0 || (int)threadIdx.x + k * kBlock < PN;
   };
+  // swizzled tile slots where the pencils are whole groups of 64 points
+  constexpr bool SW = N1 % 64 == 0;
+  constexpr int NTWS = SW ? tw_stage_offset<PlanStd>(N1, N1) : N1;
   __shared__ double2 A[PN];
-  __shared__ double2 tw[N1];
+  __shared__ double2 tw[NTWS];   // natural order, or stage order (SW)
   __shared__ double2 phs[N1 * NB];   // corPhi(l, i, a) of this slab, reused by the P rows
   __shared__ double red[4 * NF * NB];
   // inverse pass with more than 5 blocks: the slab's eigenvectors and C_l
@@ -224,6 +276,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
   const int bid = blockIdx.x - (INV ? 0 : a.red_units);
   const int job = locate_job(a.unit_base, a.n_jobs, bid);
   const JobDev &jb = a.jobs[job];
+  const int stream = jb.stream_out;   // read before any store (stores may alias the descriptor)
   const int u = bid - a.unit_base[job];
   const int M = jb.M;
   const long LM = (long)jb.L * M;
@@ -252,12 +305,15 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
     // the twiddles and the slab's phases: every load first (clamped in-range
     // indices), then the LDS writes, so the staging costs one round trip
     // rather than one per loop trip
-    constexpr int NTW = (N1 + kBlock - 1) / kBlock;
+    constexpr int NTW = (NTWS + kBlock - 1) / kBlock;
     constexpr int NPH = (N1 * NB + kBlock - 1) / kBlock;
     const QD_G double2 *cp = jb.corPhi + (long)l * N1 * jb.nO;
     double2 tv[NTW], pv[NPH];
 #pragma unroll
-    for (int it = 0; it < NTW; it++) tv[it] = jb.twR[min((int)threadIdx.x + it * kBlock, N1 - 1)];
+    for (int it = 0; it < NTW; it++) {
+      const int e = min((int)threadIdx.x + it * kBlock, NTWS - 1);
+      tv[it] = jb.twR[SW ? tw_stage_source<PlanStd, N1>(e) : e];
+    }
 #pragma unroll
     for (int it = 0; it < NPH; it++) {
       const int e = min((int)threadIdx.x + it * kBlock, N1 * NB - 1);
@@ -269,7 +325,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
     }
 #pragma unroll
     for (int it = 0; it < NTW; it++)
-      if ((int)threadIdx.x + it * kBlock < N1) tw[threadIdx.x + it * kBlock] = tv[it];
+      if ((int)threadIdx.x + it * kBlock < NTWS) tw[threadIdx.x + it * kBlock] = tv[it];
 #pragma unroll
     for (int it = 0; it < NPH; it++) {
       const int e = threadIdx.x + it * kBlock;
@@ -300,20 +356,20 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
       if (b < nb) {
 #pragma unroll
         for (int k = 0; k < XPT; k++)
-          if (eok(k)) A[ept(k)] = x[b][k];
+          if (eok(k)) A[swz<SW>(ept(k))] = x[b][k];
         __syncthreads();
         if constexpr (OWN) {
-          stages_ip<false, N1, 1, P, N1, 1, false, PlanStd, NSL>(A, tw);
+          stages_ip<false, N1, 1, P, N1, 1, false, PlanStd, NSL, SW>(A, tw);
           double2 y[XPT];
-          last_stage_own<false, N1, NSL, P, XPT>(A, tw, y);
+          last_stage_own<false, N1, NSL, P, XPT, SW>(A, tw, y);
 #pragma unroll
           for (int k = 0; k < XPT; k++)
-            if (eok(k)) base[b * LM + ept(k)] = y[k];
+            if (eok(k)) store_out(base + b * LM + ept(k), y[k], stream);
         } else {
-          stages_ip<false, N1, 1, P, N1, 1, false>(A, tw);
+          stages_ip<false, N1, 1, P, N1, 1, false, PlanStd, N1, SW>(A, tw);
 #pragma unroll
           for (int k = 0; k < XPT; k++)
-            if (eok(k)) base[b * LM + ept(k)] = A[ept(k)];
+            if (eok(k)) store_out(base + b * LM + ept(k), A[swz<SW>(ept(k))], stream);
         }
       }
     }
@@ -353,16 +409,16 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
       if (b < nb) {
 #pragma unroll
         for (int k = 0; k < XPT; k++)
-          if (eok(k)) A[ept(k)] = x[b][k];
+          if (eok(k)) A[swz<SW>(ept(k))] = x[b][k];
         __syncthreads();
         if constexpr (OWN) {
-          stages_ip<true, N1, 1, P, N1, 1, false, PlanStd, NSL>(A, tw);
-          last_stage_own<true, N1, NSL, P, XPT>(A, tw, x[b]);
+          stages_ip<true, N1, 1, P, N1, 1, false, PlanStd, NSL, SW>(A, tw);
+          last_stage_own<true, N1, NSL, P, XPT, SW>(A, tw, x[b]);
         } else {
-          stages_ip<true, N1, 1, P, N1, 1, false>(A, tw);
+          stages_ip<true, N1, 1, P, N1, 1, false, PlanStd, N1, SW>(A, tw);
 #pragma unroll
           for (int k = 0; k < XPT; k++)
-            if (eok(k)) x[b][k] = A[ept(k)];
+            if (eok(k)) x[b][k] = A[swz<SW>(ept(k))];
         }
 #pragma unroll
         for (int k = 0; k < XPT; k++) x[b][k] = eok(k) ? cscale(x[b][k], s) : make_double2(0.0, 0.0);
@@ -468,7 +524,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
               }
             }
             const double2 y = cmul(rv[k], acc);
-            base[b * LM + e] = y;
+            store_out(base + b * LM + e, y, stream);
             const double q = cnorm(y);
             sb[0] += iv[k] * q;
             sb[1] += q;
@@ -515,6 +571,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_r_ct(StepArgs a) {
   __shared__ double red[4];
   const int job = locate_job(a.unit_base, a.n_jobs, blockIdx.x);
   const JobDev &jb = a.jobs[job];
+  const int stream = jb.stream_out;   // read before any store (stores may alias the descriptor)
   const int u = blockIdx.x - a.unit_base[job];
   const int n1 = jb.n1;
   int b, l, g;
@@ -606,7 +663,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_r_ct(StepArgs a) {
       }
       dft_small<true, R1>(v, y);
 #pragma unroll
-      for (int q = 0; q < R1; q++) base[(long)(idx + q * NSL) * n1 + p] = y[q];
+      for (int q = 0; q < R1; q++) store_out(base + (long)(idx + q * NSL) * n1 + p, y[q], stream);
     }
   }
   ke = wave_sum64(ke);

@@ -416,6 +416,10 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st, De
   d.sp_te = ((nl_max + 1) / 2 + 15) / 16;
   d.sp_to = (nl_max / 2 + 15) / 16;
   d.leg_split = form == 2 ? 1 : 0;
+  // 192 x 128 x 64 with 5 blocks (126 MB): plain stores, read back from the
+  // cache (R inverse pass 56 us against 62 us streamed); 256 x 192 x 96 with
+  // 9 blocks (680 MB): streamed (r pass 275 -> 261 us, R inverse 373 -> 364 us)
+  d.stream_out = 16.0 * (double)G * nO > 256.0 * (1 << 20) ? 1 : 0;
   d.leg_step = form >= 1 ? 1 : 0;
   d.step_waves = 4;
   {

@@ -68,6 +68,10 @@ struct JobDev {
   // under x -> -x): 1 when used; tiles of 16 for the half theta grid and the
   // even / odd l columns
   int leg_split, sp_th, sp_te, sp_to;
+  // FFT passes store their output with streaming stores: set when one
+  // wavepacket array (16 M nt nO bytes) is larger than the 256 MB Infinity
+  // Cache; smaller ones are left there for the next pass to read
+  int stream_out;
   int units[KK_COUNT];       // work units (workgroups) per kernel for this job
   // physics
   double dt, dr1, dr2;

