@@ -225,8 +225,13 @@ def other_configs(api, abi, inputs, cpu_threads, with_cpu):
             prepared = [(0.0, abi.Inputs(job, total_steps=steps16, want_acf=False, final_copy=False,
                                          calculate_crp=False)) for _ in range(copies)]
             r = serve.run_stream(prepared, time_scale=0.0)
+            # per job-step: the wall time over the 16 x 400 job-steps; against the
+            # same per-step model as a single job's model_frac
+            per_job_step = r["wall_s"] / (copies * steps16)
             rec["fp64_served_x16"] = {"jobs": copies, "steps": steps16, "wall_s": r["wall_s"],
-                                      "grid_point_steps_per_s": r["grid_point_steps_per_s"]}
+                                      "grid_point_steps_per_s": r["grid_point_steps_per_s"],
+                                      "ms_per_job_step": per_job_step * 1e3,
+                                      "model_frac": step_model_seconds(job, form) / per_job_step}
         if with_cpu:
             from oracle import oracle
             if cpu_warm:
