@@ -96,19 +96,28 @@ struct SplitView {
 // one wave per SIMD.  A strip's tiles: TE k-chunks of the even backward
 // product, TO of the odd one (FBR rows l = omega + 2 m and omega + 2 m + 1),
 // then TH chunks that form psi_k, psi_k' from E and O (potential energy,
-// stored rows) and, fused, feed both forward products.  Padded with idle
-// tiles to a multiple of lcm(D, 2), unrolled and fenced tile by tile; each
-// fence region issues tile tau's MFMAs with tile tau + 1's preparation (B
-// fragments from LDS, operands from the ring), as step_wave_paired does.
+// stored rows) and, fused, feed both forward products.  With 3 or more
+// theta tiles per half (TWO) the forward products take two sweeps over the
+// theta chunks, the even one with the potential energy and stored rows,
+// then the odd one, which forms psi and phi again from E and O: one set of
+// forward accumulators is live at a time, at the cost of reading the phase
+// rows twice.  Padded with idle tiles to a multiple of lcm(D, 2), unrolled
+// and fenced tile by tile; each fence region issues tile tau's MFMAs with
+// tile tau + 1's preparation (B fragments from LDS, operands from the
+// ring), as step_wave_paired does.
 template <int TH, int TE, int TO, int D, bool FIN, bool MIXED, typename Stage>
 __device__ __forceinline__ void split_wave(const SplitView &v, long strip0, int nstrips, int sstride, double &ob,
                                            Stage &&stage) {
   constexpr int kCh = 4;
-  constexpr int T0 = TE + TO + TH;
+  constexpr bool TWO = TH >= 3;
+  constexpr int TB = TE + TO;                                // backward tiles
+  constexpr int T0 = TB + TH + (TWO && !FIN ? TH : 0);       // tiles of a strip
   constexpr int LCM = D % 2 == 0 ? D : 2 * D;
   constexpr int T = (T0 + LCM - 1) / LCM * LCM;
   constexpr int WsH = lds_stride(TH), WsE = lds_stride(TE), WsO = lds_stride(TO);
-  constexpr int NB = TE + TO > TH ? TE + TO : TH;
+  constexpr int NF = TWO ? (TE > TO ? TE : TO) : TE + TO;   // B fragment tiles of a forward tile
+  constexpr int NB = NF > TH ? NF : TH;
+  constexpr int OO = TWO ? 0 : TE;                           // first fragment tile of the odd product
   const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
   const long npts = v.M;
   const uint32_t lda = 2u * (uint32_t)v.M;   // doubles per row of the FBR, phase and psi arrays
@@ -137,10 +146,13 @@ __device__ __forceinline__ void split_wave(const SplitView &v, long strip0, int 
   // k-step 4 c + q) and its partner
   auto kk = [&](int c, int q) { return acc_col<MIXED>(c, lk, q); };
   auto kp = [&](int c, int q) { return 2 * v.nh - 1 - kk(c, q); };
+  // forward tile tau: second sweep (the odd product, TWO) and theta chunk
+  auto second = [&](int tau) { return TWO && tau >= TB + TH; };
+  auto chunk = [&](int tau) { return tau - TB - (TWO && tau >= TB + TH ? TH : 0); };
   auto load = [&](int tau, long st, Slot &r) {
     const long p = st * 16 + lr;
     const uint32_t pc = (uint32_t)(p < npts ? p : npts - 1);   // points past the end reread the last one
-    if (tau < TE + TO) {
+    if (tau < TB) {
       const int c = tau < TE ? tau : tau - TE, par = tau < TE ? 0 : 1;
 #pragma unroll
       for (int q = 0; q < kCh; q++) {
@@ -148,7 +160,7 @@ __device__ __forceinline__ void split_wave(const SplitView &v, long strip0, int 
         r.x[q] = buffer_load_v2d(rA, (row * lda + 2u * pc) * 8u);
       }
     } else if (tau < T0) {
-      const int c = tau - TE - TO;
+      const int c = chunk(tau);
 #pragma unroll
       for (int q = 0; q < kCh; q++) {
         // partner rows below 0 (k past nh on small grids) wrap to huge
@@ -157,26 +169,26 @@ __device__ __forceinline__ void split_wave(const SplitView &v, long strip0, int 
           r.x[q] = buffer_load_v2d(rT, ((uint32_t)kk(c, q) * lda + 2u * pc) * 8u);
           r.y[q] = buffer_load_v2d(rT, ((uint32_t)kp(c, q) * lda + 2u * pc) * 8u);
         }
-        const int k1 = kk(c, q), k2 = max(kp(c, q), 0);
-        r.w[q] = v.wpot[(long)k1 * v.M + pc];
-        r.w[kCh + q] = v.wpot[(long)k2 * v.M + pc];
+        if (!second(tau)) {
+          const int k1 = kk(c, q), k2 = max(kp(c, q), 0);
+          r.w[q] = v.wpot[(long)k1 * v.M + pc];
+          r.w[kCh + q] = v.wpot[(long)k2 * v.M + pc];
+        }
       }
     }
   };
   PairAcc<TH> aE, aO;   // E_k, O_k: item it of tile t at k = 16 t + lk + 4 it
   PairAcc<TE> bE;       // forward outputs, l = omega + 2 m, m = 16 t + lk + 4 it
   PairAcc<TO> bO;       // l = omega + 2 m + 1
-  auto prep = [&](int tau, long st, bool surf, const Slot &r, Op &o) {
-    const long p = st * 16 + lr;
-    const bool pok = p < npts;
-    if (tau < TE + TO) {
+  // B fragments of tile tau from LDS: read one tile ahead by prep, or (TWO:
+  // fewer registers) in the tile's own fence region by mma
+  auto frag = [&](int tau, Op &o) {
+    if (tau < TB) {
       const int c = tau < TE ? tau : tau - TE;
-      const double *B = tau < TE ? v.BE1 : v.BO1;
 #pragma unroll
       for (int q = 0; q < kCh; q++) {
-        o.ar[q] = r.x[q].x;
-        o.ai[q] = r.x[q].y;
         if (!MIXED) {
+          const double *B = tau < TE ? v.BE1 : v.BO1;
 #pragma unroll
           for (int t = 0; t < TH; t++) o.bv[q][t] = B[(16 * c + 4 * q + lk) * WsH + lr + 16 * t];
         } else {
@@ -190,51 +202,92 @@ __device__ __forceinline__ void split_wave(const SplitView &v, long strip0, int 
 #pragma unroll
         for (int t = 0; t < TH; t++) o.bx[t] = Bx[(4 * c + lk) * (16 * TH) + lr + 16 * t];
       }
+    } else if (!FIN && tau < T0) {
+      const bool s1 = second(tau), even = !s1, odd = !TWO || s1;
+      const int c = chunk(tau);
+#pragma unroll
+      for (int q = 0; q < kCh; q++) {
+        if (!MIXED) {
+          if (even) {
+#pragma unroll
+            for (int t = 0; t < TE; t++) o.bv[q][t] = v.BE2[(16 * c + 4 * q + lk) * WsE + lr + 16 * t];
+          }
+          if (odd) {
+#pragma unroll
+            for (int t = 0; t < TO; t++) o.bv[q][OO + t] = v.BO2[(16 * c + 4 * q + lk) * WsO + lr + 16 * t];
+          }
+        } else {   // table row 16 c + 4 q + lk holds node kk(c, q)
+          if (even) {
+#pragma unroll
+            for (int t = 0; t < TE; t++) o.bh[q][t] = v.BE2h[(16 * c + 4 * q + lk) * WsE + lr + 16 * t];
+          }
+          if (odd) {
+#pragma unroll
+            for (int t = 0; t < TO; t++) o.bh[q][OO + t] = v.BO2h[(16 * c + 4 * q + lk) * WsO + lr + 16 * t];
+          }
+        }
+      }
+      if (MIXED) {
+        if (even) {
+#pragma unroll
+          for (int t = 0; t < TE; t++) o.bx[t] = v.BE2x[(4 * c + lk) * (16 * TE) + lr + 16 * t];
+        }
+        if (odd) {
+#pragma unroll
+          for (int t = 0; t < TO; t++) o.bx[OO + t] = v.BO2x[(4 * c + lk) * (16 * TO) + lr + 16 * t];
+        }
+      }
+    }
+  };
+  auto prep = [&](int tau, long st, bool surf, const Slot &r, Op &o) {
+    const long p = st * 16 + lr;
+    const bool pok = p < npts;
+    if (tau < TB) {
+#pragma unroll
+      for (int q = 0; q < kCh; q++) {
+        o.ar[q] = r.x[q].x;
+        o.ai[q] = r.x[q].y;
+      }
     } else if (tau < T0) {
-      const int c = tau - TE - TO;
+      const bool s1 = second(tau), even = !s1, odd = !TWO || s1;
+      const int c = chunk(tau);
       const v4d er = aE.r[c], ei = aE.i[c], orr = aO.r[c], oi = aO.i[c];
 #pragma unroll
       for (int q = 0; q < kCh; q++) {
         const double pr = er[q] + orr[q], pi = ei[q] + oi[q];    // psi_k
         const double qr = er[q] - orr[q], qi = ei[q] - oi[q];    // psi_k'
-        // w V = 0 past nt; psi = 0 at k >= nh (P columns zero there)
-        ob += (pok ? pr * pr + pi * pi : 0.0) * r.w[q];
-        ob += (pok ? qr * qr + qi * qi : 0.0) * r.w[kCh + q];
-        if (surf && kk(c, q) < v.nh) {
-          *(QD_G v2d *)(v.psi + (long)kk(c, q) * v.M + p) = v2d{pr, pi};
-          *(QD_G v2d *)(v.psi + (long)kp(c, q) * v.M + p) = v2d{qr, qi};
+        if (!s1) {
+          // w V = 0 past nt; psi = 0 at k >= nh (P columns zero there)
+          ob += (pok ? pr * pr + pi * pi : 0.0) * r.w[q];
+          ob += (pok ? qr * qr + qi * qi : 0.0) * r.w[kCh + q];
+          if (surf && kk(c, q) < v.nh) {
+            *(QD_G v2d *)(v.psi + (long)kk(c, q) * v.M + p) = v2d{pr, pi};
+            *(QD_G v2d *)(v.psi + (long)kp(c, q) * v.M + p) = v2d{qr, qi};
+          }
         }
         if (!FIN) {
           const v2d t1 = r.x[q], t2 = r.y[q];
           const double fr = fma(-pi, t1.y, pr * t1.x), fi = fma(pr, t1.y, pi * t1.x);   // phi_k
           const double gr = fma(-qi, t2.y, qr * t2.x), gi = fma(qr, t2.y, qi * t2.x);   // phi_k'
-          o.ar[q] = fr + gr;
-          o.ai[q] = fi + gi;
-          o.br[q] = fr - gr;
-          o.bi[q] = fi - gi;
-          if (!MIXED) {
-#pragma unroll
-            for (int t = 0; t < TE; t++) o.bv[q][t] = v.BE2[(16 * c + 4 * q + lk) * WsE + lr + 16 * t];
-#pragma unroll
-            for (int t = 0; t < TO; t++) o.bv[q][TE + t] = v.BO2[(16 * c + 4 * q + lk) * WsO + lr + 16 * t];
-          } else {   // table row 16 c + 4 q + lk holds node kk(c, q)
-#pragma unroll
-            for (int t = 0; t < TE; t++) o.bh[q][t] = v.BE2h[(16 * c + 4 * q + lk) * WsE + lr + 16 * t];
-#pragma unroll
-            for (int t = 0; t < TO; t++) o.bh[q][TE + t] = v.BO2h[(16 * c + 4 * q + lk) * WsO + lr + 16 * t];
+          if (even) {
+            o.ar[q] = fr + gr;
+            o.ai[q] = fi + gi;
+          }
+          if (odd && TWO) {   // the odd sweep's operands take the even ones' registers
+            o.ar[q] = fr - gr;
+            o.ai[q] = fi - gi;
+          } else if (odd) {
+            o.br[q] = fr - gr;
+            o.bi[q] = fi - gi;
           }
         }
       }
-      if (MIXED && !FIN) {
-#pragma unroll
-        for (int t = 0; t < TE; t++) o.bx[t] = v.BE2x[(4 * c + lk) * (16 * TE) + lr + 16 * t];
-#pragma unroll
-        for (int t = 0; t < TO; t++) o.bx[TE + t] = v.BO2x[(4 * c + lk) * (16 * TO) + lr + 16 * t];
-      }
     }
+    if (!TWO) frag(tau, o);
   };
-  auto mma = [&](int tau, long st, const Op &o) {
-    if (tau < TE + TO) {
+  auto mma = [&](int tau, long st, Op &o) {
+    if (TWO) frag(tau, o);
+    if (tau < TB) {
       PairAcc<TH> &acc = tau < TE ? aE : aO;
       if (tau == 0) {
         aE.reset();
@@ -252,50 +305,60 @@ __device__ __forceinline__ void split_wave(const SplitView &v, long strip0, int 
           }
       }
     } else if (!FIN && tau < T0) {
-      const int c = tau - TE - TO;
+      const bool s1 = second(tau), even = !s1, odd = !TWO || s1;
+      const int c = chunk(tau);
       if (c == 0) {
-        bE.reset();
-        bO.reset();
+        if (even) bE.reset();
+        if (odd) bO.reset();
       }
       if constexpr (MIXED) {
-        split_mac_mixed<TE, NB>(o.bh, o.bx, 0, o.ar, o.ai, bE);
-        split_mac_mixed<TO, NB>(o.bh, o.bx, TE, o.br, o.bi, bO);
+        if (even) split_mac_mixed<TE, NB>(o.bh, o.bx, 0, o.ar, o.ai, bE);
+        if (odd) split_mac_mixed<TO, NB>(o.bh, o.bx, OO, TWO ? o.ar : o.br, TWO ? o.ai : o.bi, bO);
       } else {
 #pragma unroll
         for (int q = 0; q < kCh; q++) {
+          if (even) {
 #pragma unroll
-          for (int t = 0; t < TE; t++) {
-            bE.r[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(o.bv[q][t], o.ar[q], bE.r[t], 0, 0, 0);
-            bE.i[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(o.bv[q][t], o.ai[q], bE.i[t], 0, 0, 0);
+            for (int t = 0; t < TE; t++) {
+              bE.r[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(o.bv[q][t], o.ar[q], bE.r[t], 0, 0, 0);
+              bE.i[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(o.bv[q][t], o.ai[q], bE.i[t], 0, 0, 0);
+            }
           }
+          if (odd) {
 #pragma unroll
-          for (int t = 0; t < TO; t++) {
-            bO.r[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(o.bv[q][TE + t], o.br[q], bO.r[t], 0, 0, 0);
-            bO.i[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(o.bv[q][TE + t], o.bi[q], bO.i[t], 0, 0, 0);
+            for (int t = 0; t < TO; t++) {
+              const double xr = TWO ? o.ar[q] : o.br[q], xi = TWO ? o.ai[q] : o.bi[q];
+              bO.r[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(o.bv[q][OO + t], xr, bO.r[t], 0, 0, 0);
+              bO.i[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(o.bv[q][OO + t], xi, bO.i[t], 0, 0, 0);
+            }
           }
         }
       }
-      if (c == TH - 1) {   // streaming 16-byte stores of both parities
+      if (c == TH - 1) {   // streaming 16-byte stores of the sweep's parities
         const long p = st * 16 + lr;
         const bool pok = p < npts;
+        if (even) {
 #pragma unroll
-        for (int t = 0; t < TE; t++)
+          for (int t = 0; t < TE; t++)
 #pragma unroll
-          for (int it = 0; it < 4; it++) {
-            const int m = acc_col<MIXED>(t, lk, it);
-            if (pok && m < v.ne)
-              __builtin_nontemporal_store(v2d{bE.re(t, it), bE.im(t, it)},
-                                          (v2d *)(v.fbr + (long)(2 * m) * lda + 2 * p));
-          }
+            for (int it = 0; it < 4; it++) {
+              const int m = acc_col<MIXED>(t, lk, it);
+              if (pok && m < v.ne)
+                __builtin_nontemporal_store(v2d{bE.re(t, it), bE.im(t, it)},
+                                            (v2d *)(v.fbr + (long)(2 * m) * lda + 2 * p));
+            }
+        }
+        if (odd) {
 #pragma unroll
-        for (int t = 0; t < TO; t++)
+          for (int t = 0; t < TO; t++)
 #pragma unroll
-          for (int it = 0; it < 4; it++) {
-            const int m = acc_col<MIXED>(t, lk, it);
-            if (pok && m < v.no)
-              __builtin_nontemporal_store(v2d{bO.re(t, it), bO.im(t, it)},
-                                          (v2d *)(v.fbr + (long)(2 * m + 1) * lda + 2 * p));
-          }
+            for (int it = 0; it < 4; it++) {
+              const int m = acc_col<MIXED>(t, lk, it);
+              if (pok && m < v.no)
+                __builtin_nontemporal_store(v2d{bO.re(t, it), bO.im(t, it)},
+                                            (v2d *)(v.fbr + (long)(2 * m + 1) * lda + 2 * p));
+            }
+        }
       }
     }
   };
@@ -334,6 +397,7 @@ template <int TH, int TE, int TO, bool MIXED>
 __device__ __forceinline__ void split_forward_wave(const SplitView &v, long strip0, int nstrips, int sstride) {
   constexpr int kCh = 4;
   constexpr int WsE = lds_stride(TE), WsO = lds_stride(TO);
+  constexpr bool TWO = TH >= 3;   // even and odd products in separate sweeps, as the fused mode
   constexpr int NB = TE + TO;
   const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
   const long npts = v.M;
@@ -347,86 +411,99 @@ __device__ __forceinline__ void split_forward_wave(const SplitView &v, long stri
     const long p = st * 16 + lr;
     const bool pok = p < npts;
     const uint32_t pc = (uint32_t)(pok ? p : npts - 1);
-    PairAcc<TE> bE;
-    PairAcc<TO> bO;
-    bE.reset();
-    bO.reset();
 #pragma unroll
-    for (int c = 0; c < TH; c++) {
-      v2d x1[kCh], x2[kCh], t1[kCh], t2[kCh];
+    for (int sw = 0; sw < (TWO ? 2 : 1); sw++) {
+      const bool even = !TWO || sw == 0, odd = !TWO || sw == 1;
+      PairAcc<TE> bE;
+      PairAcc<TO> bO;
+      bE.reset();
+      bO.reset();
 #pragma unroll
-      for (int q = 0; q < kCh; q++) {
-        const uint32_t k1 = (uint32_t)acc_col<MIXED>(c, lk, q), k2 = (uint32_t)(2 * v.nh - 1) - k1;
-        x1[q] = buffer_load_v2d(rP, (k1 * lda + 2u * pc) * 8u);
-        x2[q] = buffer_load_v2d(rP, (k2 * lda + 2u * pc) * 8u);
-        t1[q] = buffer_load_v2d(rT, (k1 * lda + 2u * pc) * 8u);
-        t2[q] = buffer_load_v2d(rT, (k2 * lda + 2u * pc) * 8u);
-      }
-      double ar[kCh], ai[kCh], br[kCh], bi[kCh];
-#pragma unroll
-      for (int q = 0; q < kCh; q++) {
-        // rows k >= nh are zero in the fused mode (E and O vanish there)
-        const bool kv = acc_col<MIXED>(c, lk, q) < v.nh;
-        const double pr = kv ? x1[q].x : 0.0, pi = kv ? x1[q].y : 0.0;
-        const double qr = kv ? x2[q].x : 0.0, qi = kv ? x2[q].y : 0.0;
-        const double fr = fma(-pi, t1[q].y, pr * t1[q].x), fi = fma(pr, t1[q].y, pi * t1[q].x);
-        const double gr = fma(-qi, t2[q].y, qr * t2[q].x), gi = fma(qr, t2[q].y, qi * t2[q].x);
-        ar[q] = fr + gr;
-        ai[q] = fi + gi;
-        br[q] = fr - gr;
-        bi[q] = fi - gi;
-      }
-      if constexpr (MIXED) {
-        float bh[kCh][NB];
-        v4u bx[NB];
+      for (int c = 0; c < TH; c++) {
+        v2d x1[kCh], x2[kCh], t1[kCh], t2[kCh];
 #pragma unroll
         for (int q = 0; q < kCh; q++) {
-#pragma unroll
-          for (int t = 0; t < TE; t++) bh[q][t] = v.BE2h[(16 * c + 4 * q + lk) * WsE + lr + 16 * t];
-#pragma unroll
-          for (int t = 0; t < TO; t++) bh[q][TE + t] = v.BO2h[(16 * c + 4 * q + lk) * WsO + lr + 16 * t];
+          const uint32_t k1 = (uint32_t)acc_col<MIXED>(c, lk, q), k2 = (uint32_t)(2 * v.nh - 1) - k1;
+          x1[q] = buffer_load_v2d(rP, (k1 * lda + 2u * pc) * 8u);
+          x2[q] = buffer_load_v2d(rP, (k2 * lda + 2u * pc) * 8u);
+          t1[q] = buffer_load_v2d(rT, (k1 * lda + 2u * pc) * 8u);
+          t2[q] = buffer_load_v2d(rT, (k2 * lda + 2u * pc) * 8u);
         }
-#pragma unroll
-        for (int t = 0; t < TE; t++) bx[t] = v.BE2x[(4 * c + lk) * (16 * TE) + lr + 16 * t];
-#pragma unroll
-        for (int t = 0; t < TO; t++) bx[TE + t] = v.BO2x[(4 * c + lk) * (16 * TO) + lr + 16 * t];
-        split_mac_mixed<TE, NB>(bh, bx, 0, ar, ai, bE);
-        split_mac_mixed<TO, NB>(bh, bx, TE, br, bi, bO);
-      } else {
+        double ar[kCh], ai[kCh], br[kCh], bi[kCh];
 #pragma unroll
         for (int q = 0; q < kCh; q++) {
+          // rows k >= nh are zero in the fused mode (E and O vanish there)
+          const bool kv = acc_col<MIXED>(c, lk, q) < v.nh;
+          const double pr = kv ? x1[q].x : 0.0, pi = kv ? x1[q].y : 0.0;
+          const double qr = kv ? x2[q].x : 0.0, qi = kv ? x2[q].y : 0.0;
+          const double fr = fma(-pi, t1[q].y, pr * t1[q].x), fi = fma(pr, t1[q].y, pi * t1[q].x);
+          const double gr = fma(-qi, t2[q].y, qr * t2[q].x), gi = fma(qr, t2[q].y, qi * t2[q].x);
+          ar[q] = fr + gr;
+          ai[q] = fi + gi;
+          br[q] = fr - gr;
+          bi[q] = fi - gi;
+        }
+        if constexpr (MIXED) {
+          float bh[kCh][NB];
+          v4u bx[NB];
 #pragma unroll
-          for (int t = 0; t < TE; t++) {
-            const double bv = v.BE2[(16 * c + 4 * q + lk) * WsE + lr + 16 * t];
-            bE.r[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, ar[q], bE.r[t], 0, 0, 0);
-            bE.i[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, ai[q], bE.i[t], 0, 0, 0);
+          for (int q = 0; q < kCh; q++) {
+#pragma unroll
+            for (int t = 0; t < TE; t++) bh[q][t] = v.BE2h[(16 * c + 4 * q + lk) * WsE + lr + 16 * t];
+#pragma unroll
+            for (int t = 0; t < TO; t++) bh[q][TE + t] = v.BO2h[(16 * c + 4 * q + lk) * WsO + lr + 16 * t];
           }
 #pragma unroll
-          for (int t = 0; t < TO; t++) {
-            const double bv = v.BO2[(16 * c + 4 * q + lk) * WsO + lr + 16 * t];
-            bO.r[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, br[q], bO.r[t], 0, 0, 0);
-            bO.i[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, bi[q], bO.i[t], 0, 0, 0);
+          for (int t = 0; t < TE; t++) bx[t] = v.BE2x[(4 * c + lk) * (16 * TE) + lr + 16 * t];
+#pragma unroll
+          for (int t = 0; t < TO; t++) bx[TE + t] = v.BO2x[(4 * c + lk) * (16 * TO) + lr + 16 * t];
+          if (even) split_mac_mixed<TE, NB>(bh, bx, 0, ar, ai, bE);
+          if (odd) split_mac_mixed<TO, NB>(bh, bx, TE, br, bi, bO);
+        } else {
+#pragma unroll
+          for (int q = 0; q < kCh; q++) {
+            if (even) {
+#pragma unroll
+              for (int t = 0; t < TE; t++) {
+                const double bv = v.BE2[(16 * c + 4 * q + lk) * WsE + lr + 16 * t];
+                bE.r[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, ar[q], bE.r[t], 0, 0, 0);
+                bE.i[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, ai[q], bE.i[t], 0, 0, 0);
+              }
+            }
+            if (odd) {
+#pragma unroll
+              for (int t = 0; t < TO; t++) {
+                const double bv = v.BO2[(16 * c + 4 * q + lk) * WsO + lr + 16 * t];
+                bO.r[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, br[q], bO.r[t], 0, 0, 0);
+                bO.i[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, bi[q], bO.i[t], 0, 0, 0);
+              }
+            }
           }
         }
+      }
+      if (even) {
+#pragma unroll
+        for (int t = 0; t < TE; t++)
+#pragma unroll
+          for (int it = 0; it < 4; it++) {
+            const int m = acc_col<MIXED>(t, lk, it);
+            if (pok && m < v.ne)
+              __builtin_nontemporal_store(v2d{bE.re(t, it), bE.im(t, it)},
+                                          (v2d *)(v.fbr + (long)(2 * m) * lda + 2 * p));
+          }
+      }
+      if (odd) {
+#pragma unroll
+        for (int t = 0; t < TO; t++)
+#pragma unroll
+          for (int it = 0; it < 4; it++) {
+            const int m = acc_col<MIXED>(t, lk, it);
+            if (pok && m < v.no)
+              __builtin_nontemporal_store(v2d{bO.re(t, it), bO.im(t, it)},
+                                          (v2d *)(v.fbr + (long)(2 * m + 1) * lda + 2 * p));
+          }
       }
     }
-#pragma unroll
-    for (int t = 0; t < TE; t++)
-#pragma unroll
-      for (int it = 0; it < 4; it++) {
-        const int m = acc_col<MIXED>(t, lk, it);
-        if (pok && m < v.ne)
-          __builtin_nontemporal_store(v2d{bE.re(t, it), bE.im(t, it)}, (v2d *)(v.fbr + (long)(2 * m) * lda + 2 * p));
-      }
-#pragma unroll
-    for (int t = 0; t < TO; t++)
-#pragma unroll
-      for (int it = 0; it < 4; it++) {
-        const int m = acc_col<MIXED>(t, lk, it);
-        if (pok && m < v.no)
-          __builtin_nontemporal_store(v2d{bO.re(t, it), bO.im(t, it)},
-                                      (v2d *)(v.fbr + (long)(2 * m + 1) * lda + 2 * p));
-      }
   }
 }
 
@@ -530,7 +607,7 @@ __global__ __launch_bounds__(4 * 64) void k_leg_split(StepArgs a) {
     }
   } else {
     double ob = 0.0;
-    split_wave<TH, TE, TO, 3, FIN, MIXED>(v, strip0, S, WAVES, ob, stage);
+    split_wave<TH, TE, TO, (TH >= 3 ? 2 : 3), FIN, MIXED>(v, strip0, S, WAVES, ob, stage);
     ob = wave_sum64(ob);
     if (lane == 0) red[wave] = ob;
     __syncthreads();
@@ -540,15 +617,18 @@ __global__ __launch_bounds__(4 * 64) void k_leg_split(StepArgs a) {
   if (u == 0 && threadIdx.x == 0) advance_counters(jb);
 }
 
+// Instantiated tile counts (theta tiles per half, even and odd l tiles) and
+// whether the mixed form is built for them: at 3 theta tiles per half the
+// mixed form's registers spill, and those jobs run the separate passes.
 #define QD_SPLIT_CASES(X) \
-  X(1, 1, 1) X(2, 1, 1) X(2, 2, 1) X(2, 2, 2)
+  X(1, 1, 1, true) X(2, 1, 1, true) X(2, 2, 1, true) X(2, 2, 2, true) X(3, 3, 3, false)
 
-bool leg_split_supported(int th, int te, int to) {
+bool leg_split_supported(int th, int te, int to, bool mixed) {
   switch (th * 64 + te * 8 + to) {
-#define QD_SPLIT_OK(H, E, O) case H * 64 + E * 8 + O:
+#define QD_SPLIT_OK(H, E, O, M) \
+  case H * 64 + E * 8 + O: return !mixed || M;
     QD_SPLIT_CASES(QD_SPLIT_OK)
 #undef QD_SPLIT_OK
-    return true;
     default: return false;
   }
 }
@@ -562,34 +642,44 @@ size_t leg_split_lds_bytes(int th, int te, int to, int mixed) {
   return x * 16 + f * sizeof(float);
 }
 
+template <int H, int E, int O, bool M>
+static void launch_split_case(const StepArgs &a, int total_units, int mixed, size_t lds, int finish, hipStream_t s) {
+  const dim3 grid(total_units), block(256);
+  if constexpr (M) {
+    if (mixed) {
+      if (finish) hipLaunchKernelGGL((k_leg_split<H, E, O, true, true>), grid, block, lds, s, a);
+      else hipLaunchKernelGGL((k_leg_split<H, E, O, false, true>), grid, block, lds, s, a);
+      return;
+    }
+  }
+  if (finish) hipLaunchKernelGGL((k_leg_split<H, E, O, true, false>), grid, block, lds, s, a);
+  else hipLaunchKernelGGL((k_leg_split<H, E, O, false, false>), grid, block, lds, s, a);
+}
+
 void launch_leg_split(const StepArgs &a, int total_units, int th, int te, int to, int mixed, size_t lds, int finish,
                       hipStream_t s) {
-  const dim3 grid(total_units), block(256);
   switch (th * 64 + te * 8 + to) {
-#define QD_SPLIT_LAUNCH(H, E, O)                                                         \
-  case H * 64 + E * 8 + O:                                                               \
-    if (mixed) {                                                                         \
-      if (finish) hipLaunchKernelGGL((k_leg_split<H, E, O, true, true>), grid, block, lds, s, a);   \
-      else hipLaunchKernelGGL((k_leg_split<H, E, O, false, true>), grid, block, lds, s, a);         \
-    } else {                                                                             \
-      if (finish) hipLaunchKernelGGL((k_leg_split<H, E, O, true, false>), grid, block, lds, s, a);  \
-      else hipLaunchKernelGGL((k_leg_split<H, E, O, false, false>), grid, block, lds, s, a);        \
-    }                                                                                    \
-    break;
+#define QD_SPLIT_LAUNCH(H, E, O, M) \
+  case H * 64 + E * 8 + O: launch_split_case<H, E, O, M>(a, total_units, mixed, lds, finish, s); break;
     QD_SPLIT_CASES(QD_SPLIT_LAUNCH)
 #undef QD_SPLIT_LAUNCH
     default: break;   // excluded by leg_split_supported
   }
 }
 
-void init_attrs_leg_split() {
-  const int lim = 160 * 1024;
+template <int H, int E, int O, bool M>
+static void attrs_split_case(int lim) {
   const hipFuncAttribute at = hipFuncAttributeMaxDynamicSharedMemorySize;
-#define QD_SPLIT_ATTR(H, E, O)                                                            \
-  (void)hipFuncSetAttribute((const void *)k_leg_split<H, E, O, false, false>, at, lim);   \
-  (void)hipFuncSetAttribute((const void *)k_leg_split<H, E, O, true, false>, at, lim);    \
-  (void)hipFuncSetAttribute((const void *)k_leg_split<H, E, O, false, true>, at, lim);    \
-  (void)hipFuncSetAttribute((const void *)k_leg_split<H, E, O, true, true>, at, lim);
+  (void)hipFuncSetAttribute((const void *)k_leg_split<H, E, O, false, false>, at, lim);
+  (void)hipFuncSetAttribute((const void *)k_leg_split<H, E, O, true, false>, at, lim);
+  if constexpr (M) {
+    (void)hipFuncSetAttribute((const void *)k_leg_split<H, E, O, false, true>, at, lim);
+    (void)hipFuncSetAttribute((const void *)k_leg_split<H, E, O, true, true>, at, lim);
+  }
+}
+
+void init_attrs_leg_split() {
+#define QD_SPLIT_ATTR(H, E, O, M) attrs_split_case<H, E, O, M>(160 * 1024);
   QD_SPLIT_CASES(QD_SPLIT_ATTR)
 #undef QD_SPLIT_ATTR
 }

@@ -326,8 +326,9 @@ int legendre_step_form(const qd_inputs *in, const JobShape &sh) {
   const int nt = sh.nt, nl_max = sh.lmax - sh.omin + 1;
   const int theta_tiles = round_up(nt, 16) / 16, l_tiles = round_up(nl_max, 16) / 16;
   const int th = (nt / 2 + 15) / 16, te = ((nl_max + 1) / 2 + 15) / 16, to = (nl_max / 2 + 15) / 16;
-  if (leg_step_enabled() && leg_split_enabled() && nt % 2 == 0 && nl_max >= 2 && leg_split_supported(th, te, to) &&
-      leg_split_lds_bytes(th, te, to, sh.mixed) <= kLdsLimit && legendre_symmetric(in, sh))
+  if (leg_step_enabled() && leg_split_enabled() && nt % 2 == 0 && nl_max >= 2 &&
+      leg_split_supported(th, te, to, sh.mixed) && leg_split_lds_bytes(th, te, to, sh.mixed) <= kLdsLimit &&
+      legendre_symmetric(in, sh))
     return 2;
   if (leg_step_supported(theta_tiles, l_tiles, sh.mixed) && leg_step_lds_bytes(theta_tiles, l_tiles) <= kLdsLimit)
     return 1;

@@ -251,7 +251,7 @@ bool leg_step_supported(int theta_tiles, int l_tiles, bool mixed);
 // Parity-split form of the fused kernel (kernels_leg_split.hip): supported
 // tile counts (half theta grid, even and odd l), LDS bytes, launch (finish as
 // launch_leg_step); mixed: the FP32 hi / bfloat16 compensation form.
-bool leg_split_supported(int th, int te, int to);
+bool leg_split_supported(int th, int te, int to, bool mixed);
 size_t leg_split_lds_bytes(int th, int te, int to, int mixed);
 void launch_leg_split(const StepArgs &a, int total_units, int th, int te, int to, int mixed, size_t lds, int finish,
                       hipStream_t stream);

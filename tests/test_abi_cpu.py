@@ -166,9 +166,9 @@ def test_legendre_form_choice():
     step with parity-split products where the theta grid and the tables are
     symmetric under x -> -x (Gauss-Legendre nodes, normalised associated
     Legendre functions), the direct fused step where they are not, the
-    separate passes where the fused kernels have no instantiation (96 theta
-    points: 6 column tiles).  The mixed form has the split fused step only:
-    on an asymmetric grid it runs the separate passes."""
+    separate passes where the fused kernels have no instantiation.  The
+    mixed form has the split fused step only (on an asymmetric grid it runs
+    the separate passes), and not at 3 theta tiles per half (96 points)."""
     job = inputs.build_job(inputs.config_spec("c3"))
     assert api.legendre_form(abi.Inputs(job, total_steps=1)) == 2
 
@@ -194,4 +194,8 @@ def test_legendre_form_choice():
     assert api.legendre_form(abi.Inputs(job, total_steps=1, precision=abi.QD_PREC_MIXED)) == 2
     assert api.legendre_form(altered(x=x, precision=abi.QD_PREC_MIXED)) == 0
     big = inputs.build_job(inputs.config_spec("c2"))
-    assert api.legendre_form(abi.Inputs(big, total_steps=1)) == 0
+    assert api.legendre_form(abi.Inputs(big, total_steps=1)) == 2
+    assert api.legendre_form(abi.Inputs(big, total_steps=1, precision=abi.QD_PREC_MIXED)) == 0
+    # 128 theta points (4 tiles per half): no instantiation of either fused form
+    wide = inputs.build_job(inputs.JobSpec(n1=32, n2=16, ntheta=128, R_range=(2.0, 12.0), r_range=(0.5, 5.0)))
+    assert api.legendre_form(abi.Inputs(wide, total_steps=1)) == 0
