@@ -134,10 +134,15 @@ def cpu_baseline(job, abi, n_steps=2):
                 break
     except OSError:
         pass
+    import shutil
+    host = {"nproc": os.cpu_count(), "cpus_available": len(os.sched_getaffinity(0)), "model": cpu_model,
+            # the reference's driver is Matlab code that needs a CUDA MEX; Octave is probed and recorded
+            "octave": shutil.which("octave") or "not installed"}
     return {"value": pts * n_steps / dt, "unit": "grid-point-steps/s", "cores": threads, "kind": "port",
             "sample": f"{n_steps} steps of the same workload after 1 warm-up step, FP64 oracle "
                       f"(oracle/oracle.cpp, OpenMP, {threads} threads), {dt:.2f} s; {cpu_model}",
-            "single_thread": {"value": pts / dt1, "sample": f"1 step on 1 thread, {dt1:.2f} s"}}
+            "single_thread": {"value": pts / dt1, "sample": f"1 step on 1 thread, {dt1:.2f} s"},
+            "host": host}
 
 
 def mixed_crp_error(api, abi, inputs):
