@@ -89,6 +89,7 @@ struct SplitView {
   int nt, nh, M, n1, j0, j1;// surface rows [j0, j1) (fused), all rows (FIN)
   int ne, no;               // even / odd l columns of this block
   const double *BE1, *BO1;  // LDS: P_E [16 TE][WsH], P_O [16 TO][WsH]  (backward, rows m, columns k)
+  v2d *odd;                 // LDS, two-sweep form: this wave's odd-sweep operands [TH][4][64 lanes]
   const double *BE2, *BO2;  // LDS: wP_E [16 TH][WsE], wP_O [16 TH][WsO] (forward, rows k, columns m)
 };
 
@@ -165,7 +166,7 @@ __device__ __forceinline__ void split_wave(const SplitView &v, long strip0, int 
       for (int q = 0; q < kCh; q++) {
         // partner rows below 0 (k past nh on small grids) wrap to huge
         // offsets: the range check returns zero (their psi is zero anyway)
-        if (!FIN) {
+        if (!FIN && !second(tau)) {
           r.x[q] = buffer_load_v2d(rT, ((uint32_t)kk(c, q) * lda + 2u * pc) * 8u);
           r.y[q] = buffer_load_v2d(rT, ((uint32_t)kp(c, q) * lda + 2u * pc) * 8u);
         }
@@ -248,35 +249,38 @@ __device__ __forceinline__ void split_wave(const SplitView &v, long strip0, int 
         o.ar[q] = r.x[q].x;
         o.ai[q] = r.x[q].y;
       }
+    } else if (tau < T0 && second(tau)) {
+      // odd sweep: the operands phi_k - phi_k' that the even sweep left in LDS
+      const int c = chunk(tau);
+#pragma unroll
+      for (int q = 0; q < kCh; q++) {
+        const v2d d = v.odd[(c * kCh + q) * 64 + lane];
+        o.ar[q] = d.x;
+        o.ai[q] = d.y;
+      }
     } else if (tau < T0) {
-      const bool s1 = second(tau), even = !s1, odd = !TWO || s1;
       const int c = chunk(tau);
       const v4d er = aE.r[c], ei = aE.i[c], orr = aO.r[c], oi = aO.i[c];
 #pragma unroll
       for (int q = 0; q < kCh; q++) {
         const double pr = er[q] + orr[q], pi = ei[q] + oi[q];    // psi_k
         const double qr = er[q] - orr[q], qi = ei[q] - oi[q];    // psi_k'
-        if (!s1) {
-          // w V = 0 past nt; psi = 0 at k >= nh (P columns zero there)
-          ob += (pok ? pr * pr + pi * pi : 0.0) * r.w[q];
-          ob += (pok ? qr * qr + qi * qi : 0.0) * r.w[kCh + q];
-          if (surf && kk(c, q) < v.nh) {
-            *(QD_G v2d *)(v.psi + (long)kk(c, q) * v.M + p) = v2d{pr, pi};
-            *(QD_G v2d *)(v.psi + (long)kp(c, q) * v.M + p) = v2d{qr, qi};
-          }
+        // w V = 0 past nt; psi = 0 at k >= nh (P columns zero there)
+        ob += (pok ? pr * pr + pi * pi : 0.0) * r.w[q];
+        ob += (pok ? qr * qr + qi * qi : 0.0) * r.w[kCh + q];
+        if (surf && kk(c, q) < v.nh) {
+          *(QD_G v2d *)(v.psi + (long)kk(c, q) * v.M + p) = v2d{pr, pi};
+          *(QD_G v2d *)(v.psi + (long)kp(c, q) * v.M + p) = v2d{qr, qi};
         }
         if (!FIN) {
           const v2d t1 = r.x[q], t2 = r.y[q];
           const double fr = fma(-pi, t1.y, pr * t1.x), fi = fma(pr, t1.y, pi * t1.x);   // phi_k
           const double gr = fma(-qi, t2.y, qr * t2.x), gi = fma(qr, t2.y, qi * t2.x);   // phi_k'
-          if (even) {
-            o.ar[q] = fr + gr;
-            o.ai[q] = fi + gi;
-          }
-          if (odd && TWO) {   // the odd sweep's operands take the even ones' registers
-            o.ar[q] = fr - gr;
-            o.ai[q] = fi - gi;
-          } else if (odd) {
+          o.ar[q] = fr + gr;
+          o.ai[q] = fi + gi;
+          if (TWO) {   // the odd sweep's operands, kept for it in LDS
+            v.odd[(c * kCh + q) * 64 + lane] = v2d{fr - gr, fi - gi};
+          } else {
             o.br[q] = fr - gr;
             o.bi[q] = fi - gi;
           }
@@ -533,6 +537,9 @@ __global__ __launch_bounds__(4 * 64) void k_leg_split(StepArgs a) {
   double *BO2 = BE2 + 16 * TH * WsE;         // [16 TH][WsO]
   double *BE1 = BO2 + 16 * TH * WsO;         // [16 TE][WsH]
   double *BO1 = BE1 + 16 * TE * WsH;         // [16 TO][WsH]
+  // two-sweep form (3 or more theta tiles per half, FP64): each wave's
+  // odd-sweep operands [TH][4][64] after the tables
+  v2d *ODD = (v2d *)(BO1 + 16 * TO * WsH) + (long)wave * TH * 4 * 64;
   v4u *BE2x = (v4u *)lds_d;                  // [4 TH][16 TE]
   v4u *BO2x = BE2x + 4 * TH * 16 * TE;       // [4 TH][16 TO]
   v4u *BE1x = BO2x + 4 * TH * 16 * TO;       // [4 TE][16 TH]
@@ -590,6 +597,7 @@ __global__ __launch_bounds__(4 * 64) void k_leg_split(StepArgs a) {
   v.BO1 = BO1;
   v.BE2 = BE2;
   v.BO2 = BO2;
+  v.odd = ODD;
   v.BE1h = BE1h;
   v.BO1h = BO1h;
   v.BE2h = BE2h;
@@ -636,7 +644,8 @@ bool leg_split_supported(int th, int te, int to, bool mixed) {
 size_t leg_split_lds_bytes(int th, int te, int to, int mixed) {
   const size_t f = (size_t)16 * ((size_t)th * lds_stride(te) + (size_t)th * lds_stride(to) +
                                  (size_t)te * lds_stride(th) + (size_t)to * lds_stride(th));
-  if (!mixed) return f * sizeof(double);
+  // two-sweep form: 4 waves x th chunks x 4 k-steps x 64 lanes of 16-byte operands
+  if (!mixed) return f * sizeof(double) + (th >= 3 ? (size_t)4 * th * 4 * 64 * 16 : 0);
   // compensation entries: 4 rows of 16-byte entries per 16 k, all columns
   const size_t x = (size_t)4 * 16 * ((size_t)th * te + (size_t)th * to + (size_t)te * th + (size_t)to * th);
   return x * 16 + f * sizeof(float);
