@@ -482,18 +482,25 @@ def test_parity_wide_theta_tiles(shape):
 
 @pytest.mark.parametrize("waves", ["4", "8"])
 @pytest.mark.parametrize("shape", [(30, 18, 24), (50, 18, 96)])
-def test_parity_partial_last_strip(monkeypatch, waves, shape):
+@pytest.mark.parametrize("precision", [abi.QD_PREC_FP64, abi.QD_PREC_MIXED])
+def test_parity_partial_last_strip(monkeypatch, waves, shape, precision):
     """n1 n2 not a multiple of 16 (540 and 900 points): the last strip of 16
-    points is partial, so the GEMMs clamp its loads to the last point and mask
-    its stores (paired-row kernels at 24 theta points, one-row kernels at 96),
-    in 4- and 8-wave workgroups."""
+    points is partial, so the Legendre kernels clamp its loads to the last
+    point and mask its stores.  FP64: the fused parity-split step at 24 theta
+    points (one forward sweep) and at 96 (two sweeps); mixed: the fused step
+    at 24, the one-row separate passes at 96, in 4- and 8-wave workgroups.
+    Mixed against the oracle at the mixed form's bars (1e-5 relative in psi,
+    1e-6 in the observables)."""
     monkeypatch.setenv("QD_LEG_WAVES", waves)
     n1, n2, nt = shape
     spec = inputs.JobSpec(n1=n1, n2=n2, ntheta=nt, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
                           J=3, parity=1, k_max=-1, omega0=1, j0=1, E_coll=0.02)
-    a, b = run_both(spec, 3, renormalise=True)
+    a, b = run_both(spec, 3, precision=precision, renormalise=True)
     assert (n1 * n2) % 16 != 0
-    check_parity(a, b)
+    if precision == abi.QD_PREC_FP64:
+        check_parity(a, b)
+    else:
+        check_parity(a, b, psi_tol=1e-5, obs_tol=1e-6)
 
 
 def test_autocorrelation_across_resume_matches_oracle():
