@@ -26,6 +26,11 @@
 // the forward MFMAs consume them (row 16 c + 4 q + g holds node
 // 16 c + 4 g + q), so the hand-off stays in the lane.
 //
+// At 3 theta tiles per half (96 theta points, FP64) the forward products run
+// in two sweeps over the theta chunks, even l then odd l, so that one set of
+// forward accumulators is live at a time; the even sweep leaves the odd
+// product's operands in LDS (split_wave).
+//
 // Modes, as the direct fused kernel: fused (backward product of the pending
 // step, potential energy, flux stencil rows, phase, forward product), FIN
 // (backward product alone, the DVR state stored, finishing a step sequence),
@@ -513,9 +518,10 @@ __device__ __forceinline__ void split_forward_wave(const SplitView &v, long stri
 
 // Workgroup (block b, group g): 4 waves, one per SIMD, the strips
 // g S 4 + wave + 4 s of block b (the fused kernel's decomposition).  LDS: the
-// forward tables wP_E, wP_O, then (fused, FIN) the backward tables P_E, P_O;
-// mixed: the four compensation tables (16-byte entries, rows of 16 k), then
-// the four FP32 hi tables.
+// forward tables wP_E, wP_O, then (fused, FIN) the backward tables P_E, P_O,
+// then (two-sweep form) each wave's odd-sweep operands; mixed: the four
+// compensation tables (16-byte entries, rows of 16 k), then the four FP32 hi
+// tables.
 template <int TH, int TE, int TO, bool FIN, bool MIXED>
 __global__ __launch_bounds__(4 * 64) void k_leg_split(StepArgs a) {
   constexpr int WAVES = 4, TB = WAVES * 64;
