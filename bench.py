@@ -405,6 +405,11 @@ def main():
     t_model = step_model_seconds(job, form)
     roof["step_model_us"] = t_model * 1e6
     roof["step_model_frac"] = t_model / (el64 / args.steps)
+    # the same step against the five-pass model (separate Legendre passes,
+    # 40G + 128F per block): the bytes a step moves without the fused step
+    t_5pass = step_model_seconds(job, 0)
+    roof["step_model_5pass_us"] = t_5pass * 1e6
+    roof["step_model_5pass_frac"] = t_5pass / (el64 / args.steps)
     per_kernel = {}
     for k, us_k in prof64.items():
         if us_k <= 0.0:         # kinds fused into a neighbouring kernel
