@@ -132,6 +132,63 @@ __device__ __forceinline__ void stages_ip(double2 *A, const double2 *tw) {
   }
 }
 
+// Orders a wave's LDS accesses across lanes (a wave's LDS instructions run
+// in order; the fences keep the compiler from moving accesses across).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Wave-local form of stages_ip for a row-major tile A[p * N + e] whose
+// pencils belong to single waves: wave w owns pencils w + 4 j (j < PPW) and
+// runs their butterflies on its lanes (at most 64 per pencil and stage), so
+// the stages need no workgroup barrier, only the wave's own ordering.  Same
+// plan (PlanStd), twiddles and slots as stages_ip.  Entry: the wave's pencils
+// written and ordered (wave_lds_sync); exit likewise.
+template <bool INV, int N, int NS, int PPW, bool SW, int NSEND = N>
+__device__ __forceinline__ void stages_wave(double2 *A, const double2 *tw) {
+  if constexpr (NS < NSEND) {
+    constexpr int R = PlanStd::radix(N, NS);
+    constexpr int NB = N / R;
+    static_assert(NB <= 64, "one butterfly per lane and pencil");
+    constexpr int TWS = N / (NS * R);
+    constexpr int TWO = tw_stage_offset<PlanStd>(N, NS);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane < NB) {
+      double2 X[PPW][R];
+#pragma unroll
+      for (int j = 0; j < PPW; j++) {
+        const int p = wave + 4 * j;
+        double2 v[R];
+#pragma unroll
+        for (int q = 0; q < R; q++) v[q] = A[swz<SW>(p * N + lane + q * NB)];
+        if constexpr (NS > 1) {
+          const int k = lane % NS;
+#pragma unroll
+          for (int q = 1; q < R; q++) {
+            double2 w = SW ? tw[TWO + (q - 1) * NS + k] : tw[q * k * TWS];
+            if (INV) w.y = -w.y;
+            v[q] = cmul(v[q], w);
+          }
+        }
+        dft_small<INV, R>(v, X[j]);
+      }
+      // every lane's reads of the stage precede its writes in the wave's
+      // instruction stream, so the stage needs no ordering in between
+      const int d = (lane / NS) * NS * R + lane % NS;
+#pragma unroll
+      for (int j = 0; j < PPW; j++) {
+        const int p = wave + 4 * j;
+#pragma unroll
+        for (int q = 0; q < R; q++) A[swz<SW>(p * N + d + q * NS)] = X[j][q];
+      }
+    }
+    wave_lds_sync();
+    stages_wave<INV, N, NS * R, PPW, SW, NSEND>(A, tw);
+  }
+}
+
 // Stride of the last stage of PlanStd for length n.
 __host__ __device__ constexpr int last_stage_ns(int n) {
   int ns = 1;
@@ -244,6 +301,10 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
   constexpr int RL = N1 / NSL;
   constexpr int TL = P * NSL, MBL = (TL + kBlock - 1) / kBlock;
   constexpr bool OWN = MBL * RL == PPT;
+  // WV: with the last stage's stride 64 every point of a thread lies in a
+  // pencil of its own wave (pencils wave + 4 m), so the FFT stages run wave
+  // by wave (stages_wave) without workgroup barriers
+  constexpr bool WV = OWN && NSL == 64 && P % 4 == 0;
   constexpr int XPT = PPT;
   auto ept = [](int k) {
     if constexpr (OWN) {
@@ -357,8 +418,17 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
 #pragma unroll
         for (int k = 0; k < XPT; k++)
           if (eok(k)) A[swz<SW>(ept(k))] = x[b][k];
-        __syncthreads();
-        if constexpr (OWN) {
+        if constexpr (WV) wave_lds_sync();
+        else __syncthreads();
+        if constexpr (WV) {
+          stages_wave<false, N1, 1, P / 4, SW, NSL>(A, tw);
+          double2 y[XPT];
+          last_stage_own<false, N1, NSL, P, XPT, SW>(A, tw, y);
+#pragma unroll
+          for (int k = 0; k < XPT; k++)
+            if (eok(k)) store_out(base + b * LM + ept(k), y[k], stream);
+          wave_lds_sync();   // the last stage's reads before the next block's writes
+        } else if constexpr (OWN) {
           stages_ip<false, N1, 1, P, N1, 1, false, PlanStd, NSL, SW>(A, tw);
           double2 y[XPT];
           last_stage_own<false, N1, NSL, P, XPT, SW>(A, tw, y);
@@ -410,8 +480,13 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
 #pragma unroll
         for (int k = 0; k < XPT; k++)
           if (eok(k)) A[swz<SW>(ept(k))] = x[b][k];
-        __syncthreads();
-        if constexpr (OWN) {
+        if constexpr (WV) wave_lds_sync();
+        else __syncthreads();
+        if constexpr (WV) {
+          stages_wave<true, N1, 1, P / 4, SW, NSL>(A, tw);
+          last_stage_own<true, N1, NSL, P, XPT, SW>(A, tw, x[b]);
+          wave_lds_sync();   // the last stage's reads before the next block's writes
+        } else if constexpr (OWN) {
           stages_ip<true, N1, 1, P, N1, 1, false, PlanStd, NSL, SW>(A, tw);
           last_stage_own<true, N1, NSL, P, XPT, SW>(A, tw, x[b]);
         } else {
