@@ -142,29 +142,34 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 // Wave-local form of stages_ip for a row-major tile A[p * N + e] whose
 // pencils belong to single waves: wave w owns pencils w + 4 j (j < PPW) and
-// runs their butterflies on its lanes (at most 64 per pencil and stage), so
-// the stages need no workgroup barrier, only the wave's own ordering.  Same
-// plan (PlanStd), twiddles and slots as stages_ip.  Entry: the wave's pencils
-// written and ordered (wave_lds_sync); exit likewise.
+// runs their butterflies on its lanes, 64 / NB pencils side by side when a
+// stage has NB <= 32 butterflies per pencil, so the stages need no workgroup
+// barrier, only the wave's own ordering.  Same plan (PlanStd), twiddles and
+// slots as stages_ip.  Entry: the wave's pencils written and ordered
+// (wave_lds_sync); exit likewise.
 template <bool INV, int N, int NS, int PPW, bool SW, int NSEND = N>
 __device__ __forceinline__ void stages_wave(double2 *A, const double2 *tw) {
   if constexpr (NS < NSEND) {
     constexpr int R = PlanStd::radix(N, NS);
     constexpr int NB = N / R;
     static_assert(NB <= 64, "one butterfly per lane and pencil");
+    constexpr int SIDE = 64 / NB < PPW ? 64 / NB : PPW;   // pencils side by side on the lanes
+    constexpr int PASSES = (PPW + SIDE - 1) / SIDE;
     constexpr int TWS = N / (NS * R);
     constexpr int TWO = tw_stage_offset<PlanStd>(N, NS);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane < NB) {
-      double2 X[PPW][R];
+    const int idx = lane % NB, js = lane / NB;   // butterfly and pencil slot of this lane
+    if (js < SIDE) {
+      double2 X[PASSES][R];
 #pragma unroll
-      for (int j = 0; j < PPW; j++) {
-        const int p = wave + 4 * j;
+      for (int m = 0; m < PASSES; m++) {
+        const int j = m * SIDE + js;
+        const int p = wave + 4 * (j < PPW ? j : 0);
         double2 v[R];
 #pragma unroll
-        for (int q = 0; q < R; q++) v[q] = A[swz<SW>(p * N + lane + q * NB)];
+        for (int q = 0; q < R; q++) v[q] = A[swz<SW>(p * N + idx + q * NB)];
         if constexpr (NS > 1) {
-          const int k = lane % NS;
+          const int k = idx % NS;
 #pragma unroll
           for (int q = 1; q < R; q++) {
             double2 w = SW ? tw[TWO + (q - 1) * NS + k] : tw[q * k * TWS];
@@ -172,16 +177,19 @@ __device__ __forceinline__ void stages_wave(double2 *A, const double2 *tw) {
             v[q] = cmul(v[q], w);
           }
         }
-        dft_small<INV, R>(v, X[j]);
+        dft_small<INV, R>(v, X[m]);
       }
       // every lane's reads of the stage precede its writes in the wave's
       // instruction stream, so the stage needs no ordering in between
-      const int d = (lane / NS) * NS * R + lane % NS;
+      const int d = (idx / NS) * NS * R + idx % NS;
 #pragma unroll
-      for (int j = 0; j < PPW; j++) {
-        const int p = wave + 4 * j;
+      for (int m = 0; m < PASSES; m++) {
+        const int j = m * SIDE + js;
+        if (j < PPW) {
+          const int p = wave + 4 * j;
 #pragma unroll
-        for (int q = 0; q < R; q++) A[swz<SW>(p * N + d + q * NS)] = X[j][q];
+          for (int q = 0; q < R; q++) A[swz<SW>(p * N + d + q * NS)] = X[m][q];
+        }
       }
     }
     wave_lds_sync();
