@@ -16,16 +16,19 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
-// bfloat16 bits of a finite float, rounded to nearest even.
-__device__ __forceinline__ unsigned bf16_bits(float f) {
-  const unsigned u = __float_as_uint(f);
-  return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+// bfloat16 operand of the compensation MFMA: slots 0..3 the hi parts, 4..7
+// the lo parts, each an FP32 value rounded to nearest even
+// (v_cvt_pk_bf16_f32, two values per instruction).  The empty asm operand
+// keeps the compiler from folding the conversion with the FP64 -> FP32
+// rounding that produced the value into a direct FP64 -> bfloat16
+// conversion, which it would emulate with compares and selects.
+__device__ __forceinline__ __bf16 bf16_of(float x) {
+  asm("" : "+v"(x));
+  return (__bf16)x;
 }
-
-// bfloat16 operand of the compensation MFMA: slots 0..3 the hi parts, 4..7 the lo parts
-__device__ __forceinline__ v8bf bf16_operand(const unsigned (&hb)[4], const unsigned (&lb)[4]) {
-  return __builtin_bit_cast(v8bf, v4u{hb[0] | (hb[1] << 16), hb[2] | (hb[3] << 16), lb[0] | (lb[1] << 16),
-                                      lb[2] | (lb[3] << 16)});
+__device__ __forceinline__ v8bf bf16_operand(const float (&h)[4], const float (&l)[4]) {
+  return v8bf{bf16_of(h[0]), bf16_of(h[1]), bf16_of(h[2]), bf16_of(h[3]),
+              bf16_of(l[0]), bf16_of(l[1]), bf16_of(l[2]), bf16_of(l[3])};
 }
 
 __host__ __device__ constexpr int lds_stride(int nt_tiles) { return nt_tiles * 16 + ((nt_tiles & 1) ? 0 : 16); }
@@ -257,15 +260,15 @@ __device__ __forceinline__ void gemm_wave(const GemmWave &gw, long strip0, int n
 #pragma unroll
       for (int q0 = 0; q0 < kCh; q0 += 4) {
         if (c * kCh + q0 < gw.ksteps) {
-          unsigned hb[4], lb[4];
+          float hb[4], lb[4];
           v4f hh[NT];
 #pragma unroll
           for (int t = 0; t < NT; t++) hh[t] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
           for (int q = q0; q < q0 + 4; q++) {
             const float ah = (float)ac[q];
-            hb[q - q0] = bf16_bits(ah);
-            lb[q - q0] = bf16_bits((float)(ac[q] - (double)ah));
+            hb[q - q0] = ah;
+            lb[q - q0] = (float)(ac[q] - (double)ah);
             if (c * kCh + q < gw.ksteps) {
               const float *brow = gw.Bh + (4 * (c * kCh + q) + lk) * gw.Ws + lr;
 #pragma unroll
@@ -401,17 +404,17 @@ __device__ __forceinline__ void paired_mac(const GemmWave &gw, int c, int lk, in
     }
   } else {
     static_assert(kCh == 4, "the compensation MFMA takes 4 k-steps per lane");
-    unsigned hrb[4], lrb[4], hib[4], lib[4];
+    float hrb[4], lrb[4], hib[4], lib[4];
     v4f hr[NT], hi[NT];
 #pragma unroll
     for (int t = 0; t < NT; t++) hr[t] = hi[t] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int q = 0; q < kCh; q++) {
       const float arh = (float)ar[q], aih = (float)ai[q];
-      hrb[q] = bf16_bits(arh);
-      lrb[q] = bf16_bits((float)(ar[q] - (double)arh));
-      hib[q] = bf16_bits(aih);
-      lib[q] = bf16_bits((float)(ai[q] - (double)aih));
+      hrb[q] = arh;
+      lrb[q] = (float)(ar[q] - (double)arh);
+      hib[q] = aih;
+      lib[q] = (float)(ai[q] - (double)aih);
       if (c * kCh + q < gw.ksteps) {
         const float *brow = gw.Bh + (4 * (c * kCh + q) + lk) * gw.Ws + lr;
 #pragma unroll

@@ -47,17 +47,17 @@ namespace qd {
 template <int NT, int NB>
 __device__ __forceinline__ void split_mac_mixed(const float (&bh)[4][NB], const v4u (&bx)[NB], int t0,
                                                 const double (&ar)[4], const double (&ai)[4], PairAcc<NT> &acc) {
-  unsigned hrb[4], lrb[4], hib[4], lib[4];
+  float hrb[4], lrb[4], hib[4], lib[4];
   v4f hr[NT], hi[NT];
 #pragma unroll
   for (int t = 0; t < NT; t++) hr[t] = hi[t] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int q = 0; q < 4; q++) {
     const float arh = (float)ar[q], aih = (float)ai[q];
-    hrb[q] = bf16_bits(arh);
-    lrb[q] = bf16_bits((float)(ar[q] - (double)arh));
-    hib[q] = bf16_bits(aih);
-    lib[q] = bf16_bits((float)(ai[q] - (double)aih));
+    hrb[q] = arh;
+    lrb[q] = (float)(ar[q] - (double)arh);
+    hib[q] = aih;
+    lib[q] = (float)(ai[q] - (double)aih);
 #pragma unroll
     for (int t = 0; t < NT; t++) {
       hr[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(bh[q][t0 + t], arh, hr[t], 0, 0, 0);
