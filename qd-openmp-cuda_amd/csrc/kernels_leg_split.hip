@@ -80,6 +80,14 @@ __device__ __forceinline__ void split_mac_mixed(const float (&bh)[4][NB], const 
   }
 }
 
+// Store of a forward product's output (FBR coefficients, read once by the
+// next pass): streaming when the job's arrays are larger than the Infinity
+// Cache, plain otherwise so that the next pass finds them there.
+__device__ __forceinline__ void fbr_store(int stream, v2d val, v2d *p) {
+  if (stream) __builtin_nontemporal_store(val, p);
+  else *p = val;
+}
+
 // Device view of one block of a split job.
 struct SplitView {
   // mixed form: FP32 hi tables and bfloat16 compensation tables in LDS
@@ -92,6 +100,7 @@ struct SplitView {
   const QD_G double *vph;   // phase table as doubles, rows of 2M
   const QD_G double *wpot;  // w_k V rows [bwd_np][M]
   int nt, nh, M, n1, j0, j1;// surface rows [j0, j1) (fused), all rows (FIN)
+  int stream;               // FBR output by streaming stores (arrays larger than the Infinity Cache)
   int ne, no;               // even / odd l columns of this block
   const double *BE1, *BO1;  // LDS: P_E [16 TE][WsH], P_O [16 TO][WsH]  (backward, rows m, columns k)
   v2d *odd;                 // LDS, two-sweep form: this wave's odd-sweep operands [TH][4][64 lanes]
@@ -353,7 +362,7 @@ __device__ __forceinline__ void split_wave(const SplitView &v, long strip0, int 
             for (int it = 0; it < 4; it++) {
               const int m = acc_col<MIXED>(t, lk, it);
               if (pok && m < v.ne)
-                __builtin_nontemporal_store(v2d{bE.re(t, it), bE.im(t, it)},
+                fbr_store(v.stream, v2d{bE.re(t, it), bE.im(t, it)},
                                             (v2d *)(v.fbr + (long)(2 * m) * lda + 2 * p));
             }
         }
@@ -364,7 +373,7 @@ __device__ __forceinline__ void split_wave(const SplitView &v, long strip0, int 
             for (int it = 0; it < 4; it++) {
               const int m = acc_col<MIXED>(t, lk, it);
               if (pok && m < v.no)
-                __builtin_nontemporal_store(v2d{bO.re(t, it), bO.im(t, it)},
+                fbr_store(v.stream, v2d{bO.re(t, it), bO.im(t, it)},
                                             (v2d *)(v.fbr + (long)(2 * m + 1) * lda + 2 * p));
             }
         }
@@ -497,7 +506,7 @@ __device__ __forceinline__ void split_forward_wave(const SplitView &v, long stri
           for (int it = 0; it < 4; it++) {
             const int m = acc_col<MIXED>(t, lk, it);
             if (pok && m < v.ne)
-              __builtin_nontemporal_store(v2d{bE.re(t, it), bE.im(t, it)},
+              fbr_store(v.stream, v2d{bE.re(t, it), bE.im(t, it)},
                                           (v2d *)(v.fbr + (long)(2 * m) * lda + 2 * p));
           }
       }
@@ -508,7 +517,7 @@ __device__ __forceinline__ void split_forward_wave(const SplitView &v, long stri
           for (int it = 0; it < 4; it++) {
             const int m = acc_col<MIXED>(t, lk, it);
             if (pok && m < v.no)
-              __builtin_nontemporal_store(v2d{bO.re(t, it), bO.im(t, it)},
+              fbr_store(v.stream, v2d{bO.re(t, it), bO.im(t, it)},
                                           (v2d *)(v.fbr + (long)(2 * m + 1) * lda + 2 * p));
           }
       }
@@ -595,6 +604,7 @@ __global__ __launch_bounds__(4 * 64) void k_leg_split(StepArgs a) {
   v.nh = jb.nt / 2;
   v.M = jb.M;
   v.n1 = jb.n1;
+  v.stream = jb.stream_out;
   v.j0 = FIN ? 0 : jb.flux_on ? jb.flux_ns - jb.flux_ng / 2 : 0;
   v.j1 = FIN ? jb.n2 : jb.flux_on ? v.j0 + jb.flux_ng : 0;
   v.ne = (nl + 1) / 2;
