@@ -116,8 +116,8 @@ def cpu_baseline(job, abi, n_steps=2):
     from oracle import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     pts = job["r1"]["n"] * job["r2"]["n"] * job["theta"]["n"] * len(job["omega_states"]["omegas"])
-    inp = abi.Inputs(job, total_steps=1, want_acf=False)
-    oracle.evolve(inp, threads)           # warm-up step (page-in, thread pool)
+    inp = abi.Inputs(job, total_steps=5, want_acf=False)
+    oracle.evolve(inp, threads)           # 5 warm-up steps (page-in, thread pool)
     inp = abi.Inputs(job, total_steps=n_steps, want_acf=False)
     t0 = time.perf_counter()
     oracle.evolve(inp, threads)
@@ -139,7 +139,7 @@ def cpu_baseline(job, abi, n_steps=2):
             # the reference's driver is Matlab code that needs a CUDA MEX; Octave is probed and recorded
             "octave": shutil.which("octave") or "not installed"}
     return {"value": pts * n_steps / dt, "unit": "grid-point-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{n_steps} steps of the same workload after 1 warm-up step, FP64 oracle "
+            "sample": f"{n_steps} steps of the same workload after 5 warm-up steps, FP64 oracle "
                       f"(oracle/oracle.cpp, OpenMP, {threads} threads), {dt:.2f} s; {cpu_model}",
             "single_thread": {"value": pts / dt1, "sample": f"1 step on 1 thread, {dt1:.2f} s"},
             "host": host}
