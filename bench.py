@@ -14,7 +14,7 @@ hour, p50/p99 latency from submit to result), FP64 and mixed.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-mixed]
                     [--no-serve] [--serve-jobs N] [--serve-steps S] [--serve-energies E]
-                    [--no-configs] [--bandwidth MIB] [--c4]
+                    [--no-configs] [--bandwidth MIB] [--c4] [--c1-batch]
 """
 import argparse
 import dataclasses
@@ -223,20 +223,7 @@ def other_configs(api, abi, inputs, cpu_threads, with_cpu):
         rec["fp64"]["model_frac"] = step_model_seconds(job, form) / (rec["fp64"]["ms_per_step"] * 1e-3)
         rec["fp64"]["legendre_form"] = form
         if name == "c1":
-            # the latency-bound small job batched: 16 copies stepped together by
-            # the server (one launch per kernel for all 16)
-            from qd_openmp_cuda_amd import serve
-            copies, steps16 = 16, 400
-            prepared = [(0.0, abi.Inputs(job, total_steps=steps16, want_acf=False, final_copy=False,
-                                         calculate_crp=False)) for _ in range(copies)]
-            r = serve.run_stream(prepared, time_scale=0.0)
-            # per job-step: the wall time over the 16 x 400 job-steps; against the
-            # same per-step model as a single job's model_frac
-            per_job_step = r["wall_s"] / (copies * steps16)
-            rec["fp64_served_x16"] = {"jobs": copies, "steps": steps16, "wall_s": r["wall_s"],
-                                      "grid_point_steps_per_s": r["grid_point_steps_per_s"],
-                                      "ms_per_job_step": per_job_step * 1e3,
-                                      "model_frac": step_model_seconds(job, form) / per_job_step}
+            rec["fp64_served_x16"] = c1_batched(abi, job, form)
         if with_cpu:
             from oracle import oracle
             if cpu_warm:
@@ -255,6 +242,22 @@ def other_configs(api, abi, inputs, cpu_threads, with_cpu):
               file=sys.stderr, flush=True)
         out[name] = rec
     return out
+
+
+def c1_batched(abi, job, form, copies=16, steps=400):
+    """The latency-bound 128x64x48 job batched: `copies` copies stepped
+    together by the server (one launch per kernel for all of them), FP64.
+    Per job-step: the wall time (admission included) over copies x steps
+    job-steps, against the same per-step model as a single job's
+    model_frac."""
+    from qd_openmp_cuda_amd import serve
+    prepared = [(0.0, abi.Inputs(job, total_steps=steps, want_acf=False, final_copy=False, calculate_crp=False))
+                for _ in range(copies)]
+    r = serve.run_stream(prepared, time_scale=0.0)
+    per_job_step = r["wall_s"] / (copies * steps)
+    return {"jobs": copies, "steps": steps, "wall_s": r["wall_s"],
+            "grid_point_steps_per_s": r["grid_point_steps_per_s"], "ms_per_job_step": per_job_step * 1e3,
+            "model_frac": step_model_seconds(job, form) / per_job_step}
 
 
 def served_crp_error(crp64, crpmx, etas):
@@ -311,6 +314,8 @@ def main():
     ap.add_argument("--bandwidth", type=int, default=0,
                     help="only run the HBM ceiling probes over this many MiB and exit")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--c1-batch", action="store_true",
+                    help="only run 16 copies of the 128x64x48 job through the server (profiling) and exit")
     ap.add_argument("--c4", action="store_true",
                     help="only run the mixed-precision CRP comparison (256x192x96, J=0, 200 energies) and exit")
     ap.add_argument("--serve-jobs", type=int, default=64)
@@ -344,6 +349,10 @@ def main():
 
     if args.c4:
         print(json.dumps(mixed_crp_error(api, abi, inputs)))
+        return
+    if args.c1_batch:
+        job1 = inputs.build_job(inputs.config_spec("c1"))
+        print(json.dumps(c1_batched(abi, job1, api.legendre_form(abi.Inputs(job1, total_steps=1)))))
         return
 
     if args.serve_only:
