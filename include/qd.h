@@ -217,7 +217,8 @@ enum {
   QD_K_FFT_R_INV = 5,/* inverse FFT along R, 1/(n1 n2) scale, second half step     */
   QD_K_FBR2 = 6,     /* second half step: fused into QD_K_FFT_R_INV                */
   QD_K_LEG_BWD = 7,  /* backward Legendre transform (with the potential energy)    */
-  QD_K_OBS = 8,      /* flux surface values (when CRP is requested)                */
+  QD_K_OBS = 8,      /* flux surface values (when CRP is requested; in the fused step
+                        sequence carried by QD_K_FFT_R_FWD)                         */
   QD_K_REDUCE = 9,   /* final reduction of the per-step observables (where the fused
                         Legendre step runs, part of QD_K_FFT_R_FWD)                */
   QD_K_LEG_STEP = 10,/* fused Legendre step: backward transform of the previous step,

@@ -15,6 +15,7 @@
 
 #include "fft_common.h"
 #include "step_reduce.h"
+#include "flux_surface.h"
 
 namespace qd {
 
@@ -342,7 +343,8 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
   __shared__ double Vs[VLDS ? NB * NB : 1];
   __shared__ double2 cms[VLDS ? NB : 1];
   if (!INV && reduce_carried(a, red)) return;   // leading workgroups: the previous step's reduction
-  const int bid = blockIdx.x - (INV ? 0 : a.red_units);
+  if (!INV && flux_carried(a)) return;          // then its flux surface values
+  const int bid = blockIdx.x - (INV ? 0 : a.red_units + a.flux_units);
   const int job = locate_job(a.unit_base, a.n_jobs, bid);
   const JobDev &jb = a.jobs[job];
   const int stream = jb.stream_out;   // read before any store (stores may alias the descriptor)
@@ -760,7 +762,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_r_ct(StepArgs a) {
 // ---------------------------------------------------------------------------
 template <bool INV, int N1, bool ACF>
 void launch_R_nb(const StepArgs &a, int units, int nb_max, hipStream_t s) {
-  const dim3 g(units + (INV ? 0 : a.red_units)), t(kBlock);
+  const dim3 g(units + (INV ? 0 : a.red_units + a.flux_units)), t(kBlock);
   if (nb_max <= 1) hipLaunchKernelGGL((k_fft_R_ct<INV, 1, N1, ACF>), g, t, 0, s, a);
   else if (nb_max <= 2) hipLaunchKernelGGL((k_fft_R_ct<INV, 2, N1, ACF>), g, t, 0, s, a);
   else if (nb_max <= 3) hipLaunchKernelGGL((k_fft_R_ct<INV, 3, N1, ACF>), g, t, 0, s, a);

@@ -21,6 +21,7 @@
 
 #include "fft_common.h"
 #include "step_reduce.h"
+#include "flux_surface.h"
 #include "host_internal.h"
 
 namespace qd {
@@ -88,7 +89,8 @@ __global__ __launch_bounds__(kBlock) void k_fft_R(StepArgs a) {
   extern __shared__ double2 lds[];
   __shared__ double red[4 * NF * NB];
   if (!INV && reduce_carried(a, red)) return;   // leading workgroups: the previous step's reduction
-  const int bid = blockIdx.x - (INV ? 0 : a.red_units);
+  if (!INV && flux_carried(a)) return;          // then its flux surface values
+  const int bid = blockIdx.x - (INV ? 0 : a.red_units + a.flux_units);
   const int job = locate_job(a.unit_base, a.n_jobs, bid);
   const JobDev &jb = a.jobs[job];
   const int u = bid - a.unit_base[job];
@@ -156,7 +158,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_R(StepArgs a) {
 
 void launch_fft_R_rt(const StepArgs &a, int total_units, int inverse, int acf, int nb_max, size_t lds,
                      hipStream_t stream) {
-  const dim3 g(total_units + (inverse ? 0 : a.red_units)), t(kBlock);
+  const dim3 g(total_units + (inverse ? 0 : a.red_units + a.flux_units)), t(kBlock);
 #define QD_FFTR_CASE(NB)                                                                    \
   if (nb_max <= NB) {                                                                       \
     if (!inverse) hipLaunchKernelGGL((k_fft_R<false, NB, false>), g, t, lds, stream, a);    \

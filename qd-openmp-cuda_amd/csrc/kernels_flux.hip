@@ -13,37 +13,11 @@
 // a window of T steps; the window is then folded into A and B as one
 // (n_E x T) . (T x surface) product, so the accumulators are read and written
 // once per T steps.
-#include "qd_internal.h"
-#include "qd_complex.h"
+#include "flux_surface.h"
 
 namespace qd {
 
-// One thread per surface point s = (b nt + k) n1 + i of every flux job.
-__global__ __launch_bounds__(kBlock) void k_flux_surface(StepArgs a) {
-  const int u = blockIdx.x;
-  const int q = locate_job(a.unit_base, a.n_jobs, u);
-  const QD_G JobDev &jb = a.jobs[q];
-  const int s = (u - a.unit_base[q]) * kBlock + threadIdx.x;
-  // in the fused step sequence: only jobs whose last step is finishing now
-  if (!jb.flux_on || s >= jb.flux_S || (a.gate && jb.counters[3] == 0)) return;
-  const int n1 = jb.n1, nt = jb.nt, M = jb.M, ng = jb.flux_ng, half = ng / 2;
-  const int i = s % n1, k = (s / n1) % nt, b = s / (n1 * nt);
-  const QD_G double2 *p = jb.psi + ((long)b * nt + k) * M + (long)(jb.flux_ns - half) * n1 + i;
-  const QD_G double2 *rph = jb.flux_rph + (long)k * n1 + i;
-  const long rstride = (long)nt * n1;
-  double2 d = make_double2(0.0, 0.0), v = make_double2(0.0, 0.0);
-  for (int g = 0; g < ng; g++) {
-    const double2 x = cmul(p[(long)g * n1], rph[g * rstride]);
-    const double c = jb.flux_coef[g];
-    d.x = fma(c, x.x, d.x);
-    d.y = fma(c, x.y, d.y);
-    if (g == half) v = x;
-  }
-  const int slot = jb.counters[0] % jb.flux_T;   // the step counter before this step's increment
-  QD_G double2 *W = jb.flux_W + (long)slot * 2 * jb.flux_S;
-  W[s] = v;
-  W[jb.flux_S + s] = d;
-}
+__global__ __launch_bounds__(kBlock) void k_flux_surface(StepArgs a) { flux_unit(a, blockIdx.x); }
 
 void launch_flux(const StepArgs &a, int total_units, hipStream_t stream) {
   if (total_units <= 0) return;

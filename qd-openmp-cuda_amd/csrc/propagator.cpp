@@ -931,6 +931,8 @@ StepArgs Engine::args(int kind, int gate) const {
   a.gate = gate;
   a.red_base = nullptr;
   a.red_units = 0;
+  a.flux_base = nullptr;
+  a.flux_units = 0;
   return a;
 }
 
@@ -953,6 +955,8 @@ StepArgs Engine::args_one(int q, int kind) const {
   a.gate = 0;
   a.red_base = nullptr;
   a.red_units = 0;
+  a.flux_base = nullptr;
+  a.flux_units = 0;
   return a;
 }
 
@@ -973,12 +977,8 @@ void Engine::enqueue(hipEvent_t *ev) {
     begin(QD_K_LEG_STEP);
     launch_step_kernel(args(KK_LEG_STEP), total_[KK_LEG_STEP], 0, s);
     end(QD_K_LEG_STEP);
-    if (any_flux_) {
-      begin(QD_K_OBS);
-      launch_flux(args(KK_FLUX, 1), total_[KK_FLUX], s);
-      end(QD_K_OBS);
-    }
-    // the reduction of that step rides on the forward R pass below
+    // the reduction and the flux surface values of that step ride on the
+    // forward R pass below
   } else {
     begin(QD_K_LEG_FWD);
     launch_leg_fwd(args(KK_LEG_FWD), total_[KK_LEG_FWD], fwd_tiles_, fwd_kp_, mixed_, fwd_waves_,
@@ -994,6 +994,10 @@ void Engine::enqueue(hipEvent_t *ev) {
     if (fused_) {
       ar.red_base = d_bases_ + (size_t)KK_REDUCE * (jobs_.size() + 1);
       ar.red_units = total_[KK_REDUCE];
+      if (any_flux_) {
+        ar.flux_base = d_bases_ + (size_t)KK_FLUX * (jobs_.size() + 1);
+        ar.flux_units = total_[KK_FLUX];
+      }
     }
     if (ct_n1_) launch_fft_R_ct(ar, total_[KK_FFT_R], 0, 0, ct_n1_, nb_max_, s);
     else launch_fft_R_rt(ar, total_[KK_FFT_R], 0, 0, nb_max_, lds_fftR_, s);
@@ -1045,7 +1049,7 @@ void Engine::finish(int q) {
 unsigned Engine::timed_kinds() const {
   unsigned m = (1u << QD_K_FFT_R_FWD) | (1u << QD_K_FFT_r) | (1u << QD_K_FFT_R_INV);
   m |= fused_ ? (1u << QD_K_LEG_STEP) : ((1u << QD_K_LEG_FWD) | (1u << QD_K_LEG_BWD) | (1u << QD_K_REDUCE));
-  if (any_flux_) m |= 1u << QD_K_OBS;
+  if (any_flux_ && !fused_) m |= 1u << QD_K_OBS;   // fused: carried by the forward R pass
   return m;
 }
 

@@ -165,6 +165,10 @@ struct StepArgs {
   // reduction's prefix table red_base (0: none)
   const QD_G int *red_base;
   int red_units;
+  // ... and its next flux_units workgroups the (gated) flux surface values of
+  // that step, with the surface kind's prefix table flux_base (0: none)
+  const QD_G int *flux_base;
+  int flux_units;
 };
 
 // Locate the job owning workgroup `u` (binary search over the prefix table).
