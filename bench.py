@@ -185,19 +185,22 @@ def mixed_crp_error(api, abi, inputs):
     return out
 
 
-def step_kinds(form):
+def step_kinds(form, passes=3):
     """Kernel kinds of one step: the fused Legendre step (forms 1, 2) or the
-    separate forward and backward passes (form 0), the three FFT passes."""
+    separate forward and backward passes (form 0), then the FFT passes: three
+    (fft_R_fwd, fft_r, fft_R_inv), or two in the two-pass form (fft_r with
+    the first rot.C.rot half step and the r kinetic factor, fft_R_inv with
+    the R kinetic factor and the second half step; 32 F_b each as well)."""
     leg = ("leg_step",) if form else ("leg_fwd", "leg_bwd")
-    return leg + ("fft_R_fwd", "fft_r", "fft_R_inv")
+    return leg + (("fft_r", "fft_R_inv") if passes == 2 else ("fft_R_fwd", "fft_r", "fft_R_inv"))
 
 
-def step_model_seconds(job, form):
+def step_model_seconds(job, form, passes=3):
     """Model time of one step: sum over its kernels of max(bytes / HBM peak,
     flops / FP64 MFMA peak), with the per-kernel figures of kernel_model."""
     model, _ = kernel_model(job, form)
     return sum(max(model[k]["bytes"] / (HBM_PEAK_GBS * 1e9), model[k]["flops"] / (FP64_MFMA_PEAK_TFLOPS * 1e12))
-               for k in step_kinds(form))
+               for k in step_kinds(form, passes))
 
 
 def other_configs(api, abi, inputs, cpu_threads, with_cpu):
@@ -220,10 +223,12 @@ def other_configs(api, abi, inputs, cpu_threads, with_cpu):
             rec[prec_name] = {"steps": gpu_steps, "ms_per_step": el / gpu_steps * 1e3,
                               "grid_point_steps_per_s": pts * gpu_steps / el}
         form = api.legendre_form(abi.Inputs(job, total_steps=1))
-        rec["fp64"]["model_frac"] = step_model_seconds(job, form) / (rec["fp64"]["ms_per_step"] * 1e-3)
+        passes = api.fft_passes(abi.Inputs(job, total_steps=1))
+        rec["fp64"]["model_frac"] = step_model_seconds(job, form, passes) / (rec["fp64"]["ms_per_step"] * 1e-3)
         rec["fp64"]["legendre_form"] = form
+        rec["fp64"]["fft_passes"] = passes
         if name == "c1":
-            rec["fp64_served_x16"] = c1_batched(abi, job, form)
+            rec["fp64_served_x16"] = c1_batched(abi, job, form, passes)
         if with_cpu:
             from oracle import oracle
             if cpu_warm:
@@ -244,7 +249,7 @@ def other_configs(api, abi, inputs, cpu_threads, with_cpu):
     return out
 
 
-def c1_batched(abi, job, form, copies=16, steps=400):
+def c1_batched(abi, job, form, passes, copies=16, steps=400):
     """The latency-bound 128x64x48 job batched: `copies` copies stepped
     together by the server (one launch per kernel for all of them), FP64.
     Per job-step: the wall time (admission included) over copies x steps
@@ -257,7 +262,7 @@ def c1_batched(abi, job, form, copies=16, steps=400):
     per_job_step = r["wall_s"] / (copies * steps)
     return {"jobs": copies, "steps": steps, "wall_s": r["wall_s"],
             "grid_point_steps_per_s": r["grid_point_steps_per_s"], "ms_per_job_step": per_job_step * 1e3,
-            "model_frac": step_model_seconds(job, form) / per_job_step}
+            "model_frac": step_model_seconds(job, form, passes) / per_job_step}
 
 
 def served_crp_error(crp64, crpmx, etas):
@@ -352,7 +357,8 @@ def main():
         return
     if args.c1_batch:
         job1 = inputs.build_job(inputs.config_spec("c1"))
-        print(json.dumps(c1_batched(abi, job1, api.legendre_form(abi.Inputs(job1, total_steps=1)))))
+        inp1 = abi.Inputs(job1, total_steps=1)
+        print(json.dumps(c1_batched(abi, job1, api.legendre_form(inp1), api.fft_passes(inp1))))
         return
 
     if args.serve_only:
@@ -375,6 +381,7 @@ def main():
                                    R0=9.0, seed=None)
     job = inputs.build_job(spec)
     form = api.legendre_form(abi.Inputs(job, total_steps=1))
+    passes = api.fft_passes(abi.Inputs(job, total_steps=1))
     model, pts = kernel_model(job, form)
     print(f"# workload {args.workload}: {spec.n1}x{spec.n2}x{spec.ntheta}, J={spec.J}, K_max={spec.k_max}, "
           f"{len(job['omega_states']['omegas'])} blocks; steps={args.steps} warmup={args.warmup}",
@@ -407,11 +414,12 @@ def main():
     roof["traffic"] = read_traffic(dom)
     roof["kernel"] = dom
     roof["legendre_form"] = LEGENDRE_FORMS[form]
+    roof["fft_passes"] = passes
     roof["kernel_us"] = us
     roof["algorithmic_bytes"] = m["bytes"]
     roof["algorithmic_flops"] = m["flops"]
     # whole step against the same per-kernel model: sum of max(bytes/BW, flops/peak)
-    t_model = step_model_seconds(job, form)
+    t_model = step_model_seconds(job, form, passes)
     roof["step_model_us"] = t_model * 1e6
     roof["step_model_frac"] = t_model / (el64 / args.steps)
     # the same step against the five-pass model (separate Legendre passes,

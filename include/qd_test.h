@@ -59,6 +59,16 @@ qd_status qd_probe_bandwidth(int64_t bytes, int32_t repeats, int32_t kind, doubl
  * reference. */
 qd_status qd_probe_legendre_form(const qd_inputs *in, int32_t *form);
 
+/* How many read-write passes over the FBR array the FFT work of a job built
+ * from `in` would take (host only, like qd_probe_legendre_form): 3 (R pass
+ * with the first rotational . Coriolis . rotational half step, r pass with
+ * the kinetic phase, R pass with the second half step) or 2 (r pass with the
+ * first half step and the r kinetic factor, R pass with the R kinetic factor
+ * and the second half step; compile-time lengths, up to 5 omega blocks;
+ * QD_FFT_TWO_PASS=0 in the environment turns it off).  No counterpart in the
+ * reference. */
+qd_status qd_probe_fft_passes(const qd_inputs *in, int32_t *passes);
+
 #ifdef __cplusplus
 }
 #endif

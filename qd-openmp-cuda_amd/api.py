@@ -38,6 +38,7 @@ _DECLS = {
     "qd_probe_copy": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
     "qd_probe_bandwidth": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
     "qd_probe_legendre_form": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    "qd_probe_fft_passes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "qd_server_create": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_void_p)]),
     "qd_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32)]),
     "qd_server_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
@@ -228,6 +229,15 @@ def legendre_form(inp) -> int:
     form = C.c_int32(-1)
     _check(lib().qd_probe_legendre_form(C.addressof(inp.struct), C.byref(form)))
     return form.value
+
+
+def fft_passes(inp) -> int:
+    """Read-write passes over the FBR array of a job's FFT work (host only):
+    3, or 2 in the two-pass form (r pass with the first half step, R pass
+    with the R kinetic factor)."""
+    n = C.c_int32(-1)
+    _check(lib().qd_probe_fft_passes(C.addressof(inp.struct), C.byref(n)))
+    return n.value
 
 
 def probe_gemm(A: np.ndarray, B: np.ndarray) -> np.ndarray:

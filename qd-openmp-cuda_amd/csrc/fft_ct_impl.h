@@ -33,6 +33,12 @@ struct PlanEnds {
     return (ns == 1 || ns * first(n) == n) ? first(n) : next_radix(n / (ns * first(n)));
   }
 };
+// PlanEnds with radix 4 at both ends (n a multiple of 16), for the r pass
+// that holds every block's points in registers: 4 per butterfly and block.
+struct PlanEnds4 {
+  static constexpr int first(int) { return 4; }
+  static constexpr int radix(int n, int ns) { return (ns == 1 || ns * 4 == n) ? 4 : next_radix(n / (ns * 4)); }
+};
 
 // R-pass LDS layout (SW).  16-byte LDS stores are served 8 lanes at a time
 // over 128 bytes (32 banks), loads 16 lanes at a time over 256 bytes; a
@@ -254,6 +260,23 @@ __host__ __device__ constexpr int ct_rows(int n1, int nb, bool inv) {
 // Columns per workgroup of the r pass: 16 or 8, so the boundary stages of
 // its radix plan have 128..256 butterflies for the 256 threads.
 __host__ __device__ constexpr int ct_cols(int n2) { return (n2 <= 64 || n2 == 128) ? 16 : 8; }
+// Columns per workgroup of the two-pass form's r pass (k_fft_r_mix).  The
+// tiles of every block present stay in registers through the Coriolis mix,
+// 4 points per radix-4 butterfly and ceil(Q n2 / 4 / kBlock) butterflies a
+// thread, so Q is the larger of 16 and 8 with a tile of at most 2048 points
+// and at most 20 points a thread over the instantiated block count (1, 2, 3
+// or 5), one butterfly a thread with more than one block (two spill); 0 (the
+// three-pass form runs) above 5 blocks or where neither fits.  Narrower
+// tiles would read and write rows of less than 128 bytes.
+__host__ __device__ constexpr int mix_cols(int n2, int nb) {
+  if (nb > 5 || n2 % 16 != 0) return 0;
+  const int b = nb <= 3 ? (nb < 1 ? 1 : nb) : 5;
+  for (int q = 16; q >= 8; q /= 2) {
+    const int mb = (q * n2 / 4 + kBlock - 1) / kBlock;
+    if (q * n2 <= 2048 && b * 4 * mb <= 20 && (b == 1 || mb == 1)) return q;
+  }
+  return 0;
+}
 
 // First stage of the Coriolis mix for the XPT points of a thread and NB
 // blocks, in place: t_a(k) = phi(k, a) sum_b V[a][b] x_b(k).  Every entry
@@ -289,10 +312,13 @@ __device__ __forceinline__ void mix_first_stage(double2 (&x)[NB][XPT], VPtr V, i
 
 // ---------------------------------------------------------------------------
 // R pass with the Coriolis mix (forward: mix then DFT; inverse: DFT, scale,
-// energies, mix)
+// energies, mix).  KIN (inverse only, the two-pass form): each block's
+// forward DFT, the R kinetic factor kin1 with the R part of the kinetic
+// energy, then the inverse DFT.
 // ---------------------------------------------------------------------------
-template <bool INV, int NB, int N1, bool ACF>
+template <bool INV, int NB, int N1, bool ACF, bool KIN = false>
 __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
+  static_assert(INV || !KIN, "the R kinetic factor rides on the inverse pass");
   constexpr int NF = ACF ? 5 : 3;
   constexpr int P = ct_rows(N1, NB, INV);
   constexpr int PN = P * N1;
@@ -334,6 +360,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
   __shared__ double2 tw[NTWS];   // natural order, or stage order (SW)
   __shared__ double2 phs[N1 * NB];   // corPhi(l, i, a) of this slab, reused by the P rows
   __shared__ double red[4 * NF * NB];
+  __shared__ double redk[KIN ? 4 * NB : 1];   // R kinetic sums per wave and block
   // inverse pass with more than 5 blocks: the slab's eigenvectors and C_l
   // entries in LDS (too many for SGPRs; from global memory they would be
   // reloaded after every store of the mix).  The forward mix has no stores
@@ -486,12 +513,40 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
     // blocks pass through LDS in turn
 #pragma unroll
     for (int b = 0; b < NB; b++) {
+      double ks = 0.0;   // KIN: R kinetic energy of this thread's points of block b
       if (b < nb) {
 #pragma unroll
         for (int k = 0; k < XPT; k++)
           if (eok(k)) A[swz<SW>(ept(k))] = x[b][k];
         if constexpr (WV) wave_lds_sync();
         else __syncthreads();
+        if constexpr (KIN) {
+          // forward DFT; the thread's points (last-stage outputs under the
+          // OWN map) take kin1 and go back to their own slots, so the
+          // inverse DFT below needs only the usual entry ordering
+          double2 y[XPT];
+          if constexpr (WV) {
+            stages_wave<false, N1, 1, P / 4, SW, NSL>(A, tw);
+            last_stage_own<false, N1, NSL, P, XPT, SW>(A, tw, y);
+          } else if constexpr (OWN) {
+            stages_ip<false, N1, 1, P, N1, 1, false, PlanStd, NSL, SW>(A, tw);
+            last_stage_own<false, N1, NSL, P, XPT, SW>(A, tw, y);
+          } else {
+            stages_ip<false, N1, 1, P, N1, 1, false, PlanStd, N1, SW>(A, tw);
+#pragma unroll
+            for (int k = 0; k < XPT; k++) y[k] = eok(k) ? A[swz<SW>(ept(k))] : make_double2(0.0, 0.0);
+          }
+#pragma unroll
+          for (int k = 0; k < XPT; k++) {
+            if (eok(k)) {
+              const int i = ept(k) % N1;
+              ks += cnorm(y[k]) * jb.kinE1[i];
+              A[swz<SW>(ept(k))] = cmul(y[k], jb.kin1[i]);
+            }
+          }
+          if constexpr (WV) wave_lds_sync();
+          else __syncthreads();
+        }
         if constexpr (WV) {
           stages_wave<true, N1, 1, P / 4, SW, NSL>(A, tw);
           last_stage_own<true, N1, NSL, P, XPT, SW>(A, tw, x[b]);
@@ -511,6 +566,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
 #pragma unroll
         for (int k = 0; k < XPT; k++) x[b][k] = make_double2(0.0, 0.0);
       }
+      if constexpr (KIN) wave_sum_to(ks, redk + wave * NB + b);
       if (b > 0) coriolis_energy(b - 1);
     }
     coriolis_energy(NB - 1);
@@ -628,6 +684,26 @@ __global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
       const int q = threadIdx.x;   // q = NF b + f, stored at field f of block b ([b][5][units])
       const double sum = (q < NV) ? (red[q] + red[NV + q]) + (red[2 * NV + q] + red[3 * NV + q]) : 0.0;
       pcor[(long)(5 * (q / NF) + q % NF) * ustride + u] = sum;
+    }
+    if constexpr (KIN) {
+      // kinetic energy of block b in this workgroup's rows, as the 2-D
+      // momentum-space sum sum |F psi|^2 (E1 + E2) that the reduction scales.
+      // R part: the sums above over n2 (the r pass leaves its output
+      // multiplied by n2, so they are n2^2 sum |F_R psi|^2 E1, and by
+      // Parseval along r the 2-D sum is n2 times that; the r factor is
+      // unitary along r and changes neither).  r part: n1 times the r pass's
+      // sums (Parseval along R), those of the slab's column groups g,
+      // g + groups, .. taken here in that order.
+      if ((int)threadIdx.x < nO) {
+        const int b = threadIdx.x;
+        const double e1 = (redk[b] + redk[NB + b]) + (redk[2 * NB + b] + redk[3 * NB + b]);
+        const int gcols = jb.n1 / jb.mix_cols;
+        const QD_G double *k2 =
+            jb.part_ke2 + (long)b * jb.units[KK_FFT_r] + (long)(l - jb.omin) * gcols;
+        double e2 = 0.0;
+        for (int ga = g; ga < gcols; ga += groups) e2 += k2[ga];
+        jb.part_ke[(long)b * ustride + u] = e1 / (double)jb.n2 + (double)N1 * e2;
+      }
     }
     // fused Legendre step: this step's backward transform is now pending
     if (u == 0 && threadIdx.x == 0 && jb.leg_step) jb.counters[3] = 1;
@@ -758,6 +834,188 @@ __global__ __launch_bounds__(kBlock) void k_fft_r_ct(StepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Two-pass form, r pass (where mix_cols(n2, blocks) > 0).  The step's kinetic
+// factor F^-1 exp(-i dt (E1 + E2)) F, F = F_r F_R the 2-D transform, is the
+// product of the R factor F_R^-1 kin1 F_R and the r factor F_r^-1 kin2 F_r
+// (they act on different indices).  This pass applies the first rotational .
+// Coriolis . rotational half step (the forward R pass's per-point mix) and
+// the r factor; the inverse R pass (KIN) applies the R factor and the second
+// half step: two read-write passes over the FBR array per step instead of
+// three.  The kinetic energy of the state after the first half step
+// (cudaOpenMP2.cu:358-360) splits the same way: its r part is summed here
+// (Parseval along R), its R part in the R pass.
+// Workgroup (l, g) takes columns i = g Q .. g Q + Q - 1 of slab l in every
+// block present; each block's points stay in registers from the load to its
+// first DFT stage, as the inputs of the thread's radix-4 butterflies.  The
+// output is left multiplied by n2 (unscaled transform pair): the R pass
+// scales by 1 / (n1 n2).
+// ---------------------------------------------------------------------------
+template <int N2, int NB>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_fft_r_mix(StepArgs a) {
+  constexpr int Q = mix_cols(N2, NB);
+  static_assert(Q > 0 && kBlock % Q == 0, "no r-pass tiling with the mix for these sizes");
+  constexpr int QN = Q * N2;
+  constexpr int R1 = PlanEnds4::first(N2), NB1 = N2 / R1, NSL = NB1;
+  static_assert(N2 % (R1 * R1) == 0, "PlanEnds4 needs n2 to be a multiple of 16");
+  constexpr int T1 = Q * NB1, MB = (T1 + kBlock - 1) / kBlock;
+  __shared__ double2 A[QN];   // column p element j at A[p + j Q]
+  __shared__ double2 tw[N2];
+  __shared__ double red[4 * NB];
+  if (reduce_carried(a, red)) return;   // leading workgroups: the previous step's reduction
+  if (flux_carried(a)) return;          // then its flux surface values
+  const int bid = blockIdx.x - a.red_units - a.flux_units;
+  const int job = locate_job(a.unit_base, a.n_jobs, bid);
+  const JobDev &jb = a.jobs[job];
+  const int stream = jb.stream_out;   // read before any store (stores may alias the descriptor)
+  const int u = bid - a.unit_base[job];
+  const int n1 = jb.n1, nO = jb.nO;
+  const long LM = (long)jb.L * jb.M;
+  const int groups = n1 / Q;
+  const int l = jb.omin + u / groups, g = u % groups;
+  const int nb = min(jb.kmax, l) - jb.omin + 1;
+  const int p = threadIdx.x % Q, i = g * Q + p;   // the thread's column (Q divides kBlock)
+  QD_G double2 *base = jb.fbr + (long)l * jb.M + (long)g * Q;   // block b at + b LM; (p, j) at [j n1 + p]
+  QD_G double *pke = jb.part_ke2 + u;
+  const long ua = jb.units[KK_FFT_r];
+  auto live = [](int m) { return T1 % kBlock == 0 || (int)threadIdx.x + m * kBlock < T1; };
+  auto bfly = [&](int m) { return live(m) ? ((int)threadIdx.x + m * kBlock) / Q : 0; };
+  // every block's points of the thread's butterflies (inputs j = idx + q NB1
+  // of the first stage), in flight at once; blocks >= nb load block nb - 1
+  // (values never used), idle threads butterfly 0
+  double2 x[NB][MB][R1];
+#pragma unroll
+  for (int b = 0; b < NB; b++)
+#pragma unroll
+    for (int m = 0; m < MB; m++)
+#pragma unroll
+      for (int q = 0; q < R1; q++)
+        x[b][m][q] = base[(b < nb ? b : nb - 1) * LM + (long)(bfly(m) + q * NB1) * n1 + p];
+  constexpr int NTW = (N2 + kBlock - 1) / kBlock;
+  double2 tv[NTW];
+#pragma unroll
+  for (int it = 0; it < NTW; it++) tv[it] = jb.twr[min((int)threadIdx.x + it * kBlock, N2 - 1)];
+  double2 ph[NB];   // corPhi(l, i, a)
+#pragma unroll
+  for (int al = 0; al < NB; al++) ph[al] = jb.corPhi[((long)l * n1 + i) * nO + min(al, nO - 1)];
+#pragma unroll
+  for (int it = 0; it < NTW; it++)
+    if ((int)threadIdx.x + it * kBlock < N2) tw[threadIdx.x + it * kBlock] = tv[it];
+  // first half step, point by point (V entries scalar, as in the forward R pass)
+  {
+    const QD_C double *V = (const QD_C double *)(jb.corV + (long)l * nO * nO);
+    FbrSums<NB, 3> none;
+#pragma unroll
+    for (int m = 0; m < MB; m++) {
+      if (live(m)) {
+#pragma unroll
+        for (int q = 0; q < R1; q++) {
+          double2 xp[NB];
+#pragma unroll
+          for (int b = 0; b < NB; b++) xp[b] = x[b][m][q];
+          coriolis_point<false, NB, 3>(jb, xp, nb, l, i, bfly(m) + q * NB1, V, none, nullptr, LM,
+                                       (const double2 *)ph);
+#pragma unroll
+          for (int b = 0; b < NB; b++) x[b][m][q] = xp[b];
+        }
+      }
+    }
+  }
+  // the r factor and energy at the last stage's outputs j = idx + q NSL, the
+  // same for every block: read before the first store
+  double2 kf[MB][R1];
+  double ef[MB][R1];
+#pragma unroll
+  for (int m = 0; m < MB; m++)
+#pragma unroll
+    for (int q = 0; q < R1; q++) {
+      kf[m][q] = jb.kin2[bfly(m) + q * NSL];
+      ef[m][q] = jb.kinE2[bfly(m) + q * NSL];
+    }
+  const int wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+    double ke = 0.0;   // r kinetic energy of the thread's points of block b
+    if (b < nb) {
+      // the twiddles (first block) or the previous block's last-stage reads
+      // before these writes
+      __syncthreads();
+      // forward first stage (stride 1) from registers
+#pragma unroll
+      for (int m = 0; m < MB; m++) {
+        if (live(m)) {
+          double2 X[R1];
+          dft_small<false, R1>(x[b][m], X);
+#pragma unroll
+          for (int q = 0; q < R1; q++) A[p + (bfly(m) * R1 + q) * Q] = X[q];
+        }
+      }
+      __syncthreads();
+      stages_ip<false, N2, R1, Q, 1, Q, true, PlanEnds4, NSL>(A, tw);
+      // forward last stage into registers, the r factor and energy, the
+      // inverse first stage
+#pragma unroll
+      for (int m = 0; m < MB; m++) {
+        if (live(m)) {
+          const int idx = bfly(m);
+          double2 v[R1], y[R1];
+#pragma unroll
+          for (int q = 0; q < R1; q++) v[q] = A[p + (idx + q * NB1) * Q];
+          if constexpr (NSL > 1) {
+#pragma unroll
+            for (int q = 1; q < R1; q++) v[q] = cmul(v[q], tw[q * idx]);
+          }
+          dft_small<false, R1>(v, y);
+#pragma unroll
+          for (int q = 0; q < R1; q++) {
+            ke += cnorm(y[q]) * ef[m][q];
+            y[q] = cmul(y[q], kf[m][q]);
+          }
+          dft_small<true, R1>(y, x[b][m]);
+        }
+      }
+      __syncthreads();   // every read of the last stage precedes the in-place writes
+#pragma unroll
+      for (int m = 0; m < MB; m++) {
+        if (live(m)) {
+#pragma unroll
+          for (int q = 0; q < R1; q++) A[p + (bfly(m) * R1 + q) * Q] = x[b][m][q];
+        }
+      }
+      __syncthreads();
+      stages_ip<true, N2, R1, Q, 1, Q, true, PlanEnds4, NSL>(A, tw);
+      // inverse last stage, written to HBM
+#pragma unroll
+      for (int m = 0; m < MB; m++) {
+        if (live(m)) {
+          const int idx = bfly(m);
+          double2 v[R1], y[R1];
+#pragma unroll
+          for (int q = 0; q < R1; q++) v[q] = A[p + (idx + q * NB1) * Q];
+          if constexpr (NSL > 1) {
+#pragma unroll
+            for (int q = 1; q < R1; q++) {
+              double2 w = tw[q * idx];
+              w.y = -w.y;
+              v[q] = cmul(v[q], w);
+            }
+          }
+          dft_small<true, R1>(v, y);
+#pragma unroll
+          for (int q = 0; q < R1; q++) store_out(base + b * LM + (long)(idx + q * NSL) * n1 + p, y[q], stream);
+        }
+      }
+    }
+    wave_sum_to(ke, red + wave * NB + b);   // 0 for blocks absent at slab l
+  }
+  // r kinetic sums of the workgroup per block, [b][units]
+  __syncthreads();
+  if ((int)threadIdx.x < nO) {
+    const int b = threadIdx.x;
+    pke[b * ua] = (red[b] + red[NB + b]) + (red[2 * NB + b] + red[3 * NB + b]);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // launchers for one compile-time R length (instantiated in kernels_fft_ct*.hip)
 // ---------------------------------------------------------------------------
 template <bool INV, int N1, bool ACF>
@@ -773,6 +1031,29 @@ void launch_R_nb(const StepArgs &a, int units, int nb_max, hipStream_t s) {
   else hipLaunchKernelGGL((k_fft_R_ct<INV, 24, N1, ACF>), g, t, 0, s, a);
 }
 
+// two-pass form: the inverse R pass with the R kinetic factor (up to 5 blocks)
+template <int N1, bool ACF>
+void launch_R_kin_nb(const StepArgs &a, int units, int nb_max, hipStream_t s) {
+  const dim3 g(units), t(kBlock);
+  if (nb_max <= 1) hipLaunchKernelGGL((k_fft_R_ct<true, 1, N1, ACF, true>), g, t, 0, s, a);
+  else if (nb_max <= 2) hipLaunchKernelGGL((k_fft_R_ct<true, 2, N1, ACF, true>), g, t, 0, s, a);
+  else if (nb_max <= 3) hipLaunchKernelGGL((k_fft_R_ct<true, 3, N1, ACF, true>), g, t, 0, s, a);
+  else hipLaunchKernelGGL((k_fft_R_ct<true, 5, N1, ACF, true>), g, t, 0, s, a);
+}
+// and the r pass with the first half step, for the block counts it has a tiling for
+template <int N2, int NB>
+void launch_r_mix_one(const StepArgs &a, dim3 g, hipStream_t s) {
+  if constexpr (mix_cols(N2, NB) > 0) hipLaunchKernelGGL((k_fft_r_mix<N2, NB>), g, dim3(kBlock), 0, s, a);
+}
+template <int N2>
+void launch_r_mix_nb(const StepArgs &a, int units, int nb_max, hipStream_t s) {
+  const dim3 g(units + a.red_units + a.flux_units);
+  if (nb_max <= 1) launch_r_mix_one<N2, 1>(a, g, s);
+  else if (nb_max <= 2) launch_r_mix_one<N2, 2>(a, g, s);
+  else if (nb_max <= 3) launch_r_mix_one<N2, 3>(a, g, s);
+  else launch_r_mix_one<N2, 5>(a, g, s);
+}
+
 #define QD_FFT_CT_DEFINE(N1)                                                                   \
   void launch_fft_R_ct_##N1(const StepArgs &a, int units, int inverse, int acf, int nb_max, hipStream_t s) { \
     if (!inverse) launch_R_nb<false, N1, false>(a, units, nb_max, s);                          \
@@ -781,6 +1062,13 @@ void launch_R_nb(const StepArgs &a, int units, int nb_max, hipStream_t s) {
   }                                                                                            \
   void launch_fft_r_ct_##N1(const StepArgs &a, int units, hipStream_t s) {                     \
     hipLaunchKernelGGL(k_fft_r_ct<N1>, dim3(units), dim3(kBlock), 0, s, a);                    \
+  }                                                                                            \
+  void launch_fft_R_kin_##N1(const StepArgs &a, int units, int acf, int nb_max, hipStream_t s) { \
+    if (acf) launch_R_kin_nb<N1, true>(a, units, nb_max, s);                                   \
+    else launch_R_kin_nb<N1, false>(a, units, nb_max, s);                                      \
+  }                                                                                            \
+  void launch_fft_r_mix_##N1(const StepArgs &a, int units, int nb_max, hipStream_t s) {        \
+    launch_r_mix_nb<N1>(a, units, nb_max, s);                                                  \
   }
 
 }  // namespace qd

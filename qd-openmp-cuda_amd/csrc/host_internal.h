@@ -52,6 +52,10 @@ JobShape validate_inputs(const qd_inputs *in);
 // Form of the Legendre transforms of a validated job: 0 separate passes,
 // 1 fused step, 2 fused step with parity-split products (host only).
 int legendre_step_form(const qd_inputs *in, const JobShape &sh);
+// FFT passes of a validated job: 2 (the two-pass form: r pass with the first
+// half step, R pass with the R kinetic factor) or 3; *cols (if given)
+// receives the two-pass r pass's columns per workgroup, 0 for 3 passes.
+int fft_passes(const JobShape &sh, int *cols);
 
 // Where a job's device memory and its large host<->device copies come from.
 // The plain form allocates with hipMalloc, frees with hipFree and copies
@@ -165,6 +169,7 @@ class Engine {
   int total_[KK_COUNT] = {};
   size_t lds_fftR_ = 0, lds_fftr_ = 0;
   int ct_n1_ = 0, ct_n2_ = 0;   // lengths of the compile-time FFT kernels, 0 = runtime path
+  int fft2_ = 0;                // two-pass FFT form (every job of the engine)
   int fwd_tiles_ = 0, bwd_tiles_ = 0, fwd_kp_ = 0, bwd_kp_ = 0, nb_max_ = 1;
   int fwd_waves_ = 4, bwd_waves_ = 4;   // waves per GEMM workgroup (the same for every job)
   int mixed_ = 0, any_flux_ = 0, any_acf_ = 0;

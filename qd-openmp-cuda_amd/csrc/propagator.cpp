@@ -322,6 +322,16 @@ static bool legendre_symmetric(const qd_inputs *in, const JobShape &s) {
   return true;
 }
 
+// two-pass form where both lengths have compile-time kernels and the r pass
+// with the first half step has a tiling for nO blocks
+int fft_passes(const JobShape &sh, int *cols) {
+  const bool ct = fft_ct_supported(sh.n1) && sh.n2 % fft_ct_rows(sh.n1, sh.nO, true) == 0;
+  const int q = ct ? fft_mix_cols(sh.n2, sh.nO) : 0;
+  const bool two = q > 0 && sh.n1 % q == 0;
+  if (cols) *cols = two ? q : 0;
+  return two ? 2 : 3;
+}
+
 int legendre_step_form(const qd_inputs *in, const JobShape &sh) {
   const int nt = sh.nt, nl_max = sh.lmax - sh.omin + 1;
   const int theta_tiles = round_up(nt, 16) / 16, l_tiles = round_up(nl_max, 16) / 16;
@@ -442,13 +452,14 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st, De
   }
   if (fft_ct_supported(n2) && n1 % fft_ct_cols(n2) == 0) d.fft_cols = fft_ct_cols(n2);
   else d.fft_cols = pick_divisor(n1, cols_c, 5, 2048, n2);
+  d.fft2 = fft_passes(sh, &d.mix_cols) == 2 ? 1 : 0;
   long slabs = 0;
   for (int b = 0; b < nO; b++) slabs += L - (sh.omin + b);
   d.units[KK_LEG_FWD] = nO * d.gemm_fwd_groups;
   d.units[KK_LEG_BWD] = nO * d.gemm_bwd_groups;
-  d.units[KK_FFT_R] = (L - sh.omin) * (n2 / d.fft_rows);
+  d.units[KK_FFT_R] = d.fft2 ? 0 : (L - sh.omin) * (n2 / d.fft_rows);
   d.units[KK_FFT_RI] = (L - sh.omin) * (n2 / d.fft_rows_inv);
-  d.units[KK_FFT_r] = (int)(slabs * (n1 / d.fft_cols));
+  d.units[KK_FFT_r] = d.fft2 ? (L - sh.omin) * (n1 / d.mix_cols) : (int)(slabs * (n1 / d.fft_cols));
   d.units[KK_REDUCE] = nO * kRedFields;
   d.units[KK_FLUX] = 0;
   d.units[KK_LEG_STEP] = d.leg_step ? nO * d.step_groups : 0;
@@ -642,7 +653,8 @@ DeviceJob::DeviceJob(const qd_inputs *in, const JobShape &sh, hipStream_t st, De
   d.twR = upload(twiddles(n1), st);
   d.twr = upload(twiddles(n2), st);
   d.part_cor = (double *)alloc(sizeof(double) * (size_t)d.units[KK_FFT_RI] * nO * 5);
-  d.part_ke = (double *)alloc(sizeof(double) * (size_t)d.units[KK_FFT_r]);
+  d.part_ke = (double *)alloc(sizeof(double) * (size_t)(d.fft2 ? nO * d.units[KK_FFT_RI] : d.units[KK_FFT_r]));
+  d.part_ke2 = d.fft2 ? (double *)alloc(sizeof(double) * (size_t)nO * d.units[KK_FFT_r]) : nullptr;
   // potential-energy partials of the backward pass or of the fused step
   d.part_obs = (double *)alloc(sizeof(double) * (size_t)nO * std::max(d.gemm_bwd_groups, d.step_groups));
   if (sh.acf) {
@@ -870,6 +882,7 @@ void Engine::set_jobs(const std::vector<DeviceJob *> &jobs) {
   any_acf_ = 0;
   fused_ = jobs[0]->dev.leg_step;
   split_ = jobs[0]->dev.leg_split;
+  fft2_ = jobs[0]->dev.fft2;
   step_waves_ = jobs[0]->dev.step_waves;
   step_ttiles_ = step_ltiles_ = 0;
   lds_step_ = 0;
@@ -890,6 +903,8 @@ void Engine::set_jobs(const std::vector<DeviceJob *> &jobs) {
     int *single = bases.data() + (size_t)KK_COUNT * (n + 1) + (size_t)2 * KK_COUNT * q;
     for (int k = 0; k < KK_COUNT; k++) single[2 * k + 1] = d.units[k];
     QD_REQUIRE(d.mixed == mixed_, "all jobs of one engine must share a precision");
+    QD_REQUIRE(d.fft2 == fft2_ && d.mix_cols == jobs[0]->dev.mix_cols,
+               "all jobs of one engine must share the FFT form and the r-pass tiling");
     QD_REQUIRE(d.gemm_fwd_waves == fwd_waves_ && d.gemm_bwd_waves == bwd_waves_,
                "all jobs of one engine must share the GEMM workgroup size");
     // the strips of a job (16 points when paired, 16 rows otherwise) follow
@@ -917,6 +932,9 @@ void Engine::set_jobs(const std::vector<DeviceJob *> &jobs) {
     if (jobs[q]->dev.fft_rows != fft_ct_rows(ct_n1_, nb_max_, false) ||
         jobs[q]->dev.fft_rows_inv != fft_ct_rows(ct_n1_, nb_max_, true))
       ct_n1_ = 0;
+  // the two-pass form has compile-time kernels only, instantiated for nb_max_
+  QD_REQUIRE(!fft2_ || (ct_n1_ && ct_n2_ && jobs[0]->dev.mix_cols == fft_mix_cols(ct_n2_, nb_max_)),
+             "two-pass FFT form without compile-time kernels for every job");
   for (int k = 0; k < KK_COUNT; k++) total_[k] = bases[(size_t)k * (n + 1) + n];
   QD_HIP(hipMemcpyAsync(d_jobs_, descs.data(), sizeof(JobDev) * n, hipMemcpyHostToDevice, stream_));
   QD_HIP(hipMemcpyAsync(d_bases_, bases.data(), sizeof(int) * bases.size(), hipMemcpyHostToDevice, stream_));
@@ -985,6 +1003,27 @@ void Engine::enqueue(hipEvent_t *ev) {
                    jobs_[0]->dev.leg_fwd_paired, s);
     end(QD_K_LEG_FWD);
   }
+  if (fft2_) {
+    // two-pass form: the r pass with the first half step, carrying the
+    // previous step's reduction and flux values like the forward R pass
+    // below (it writes only the r kinetic partial sums, which that
+    // reduction does not read); then the R pass with the R kinetic factor
+    begin(QD_K_FFT_r);
+    StepArgs ar = args(KK_FFT_r);
+    if (fused_) {
+      ar.red_base = d_bases_ + (size_t)KK_REDUCE * (jobs_.size() + 1);
+      ar.red_units = total_[KK_REDUCE];
+      if (any_flux_) {
+        ar.flux_base = d_bases_ + (size_t)KK_FLUX * (jobs_.size() + 1);
+        ar.flux_units = total_[KK_FLUX];
+      }
+    }
+    launch_fft_r_mix(ar, total_[KK_FFT_r], ct_n2_, nb_max_, s);
+    end(QD_K_FFT_r);
+    begin(QD_K_FFT_R_INV);
+    launch_fft_R_kin(args(KK_FFT_RI), total_[KK_FFT_RI], any_acf_, ct_n1_, nb_max_, s);
+    end(QD_K_FFT_R_INV);
+  } else {
   begin(QD_K_FFT_R_FWD);
   {
     // fused sequence: the previous step's (gated) reduction as the leading
@@ -1011,6 +1050,7 @@ void Engine::enqueue(hipEvent_t *ev) {
   if (ct_n1_) launch_fft_R_ct(args(KK_FFT_RI), total_[KK_FFT_RI], 1, any_acf_, ct_n1_, nb_max_, s);
   else launch_fft_R_rt(args(KK_FFT_RI), total_[KK_FFT_RI], 1, any_acf_, nb_max_, lds_fftR_, s);
   end(QD_K_FFT_R_INV);
+  }
   if (!fused_) {
     begin(QD_K_LEG_BWD);
     launch_leg_bwd(args(KK_LEG_BWD), total_[KK_LEG_BWD], bwd_tiles_, bwd_kp_, mixed_, bwd_waves_,
@@ -1047,7 +1087,7 @@ void Engine::finish(int q) {
 }
 
 unsigned Engine::timed_kinds() const {
-  unsigned m = (1u << QD_K_FFT_R_FWD) | (1u << QD_K_FFT_r) | (1u << QD_K_FFT_R_INV);
+  unsigned m = (fft2_ ? 0u : (1u << QD_K_FFT_R_FWD)) | (1u << QD_K_FFT_r) | (1u << QD_K_FFT_R_INV);
   m |= fused_ ? (1u << QD_K_LEG_STEP) : ((1u << QD_K_LEG_FWD) | (1u << QD_K_LEG_BWD) | (1u << QD_K_REDUCE));
   if (any_flux_ && !fused_) m |= 1u << QD_K_OBS;   // fused: carried by the forward R pass
   return m;

@@ -322,6 +322,15 @@ qd_status qd_probe_legendre_form(const qd_inputs *in, int32_t *form) {
   })
 }
 
+qd_status qd_probe_fft_passes(const qd_inputs *in, int32_t *passes) {
+  if (!passes) return fail(nullptr, QD_ERR_INVALID, "NULL argument");
+  QD_GUARD(nullptr, {
+    const JobShape shape = validate_inputs(in);
+    *passes = fft_passes(shape, nullptr);
+    return QD_OK;
+  })
+}
+
 qd_status qd_probe_gemm(const double *A, int32_t rows, int32_t k, const double *B, int32_t n, double *C) {
   QD_GUARD(nullptr, {
     init_kernel_attributes();

@@ -61,6 +61,13 @@ struct JobDev {
   int bwd_kp, bwd_np;        // padded K and N, backward
   int fft_rows, fft_rows_inv; // pencils per workgroup along R (forward, inverse R pass)
   int fft_cols;              // pencils per workgroup along r
+  // two-pass form of the FFT work (1 when used): an r pass that applies the
+  // first rotational . Coriolis . rotational half step and the r kinetic
+  // factor (mix_cols columns of every block per workgroup), then an R pass
+  // with the R kinetic factor and the second half step; otherwise three
+  // passes (forward R with the first half step, r with both kinetic factors,
+  // inverse R with the second half step)
+  int fft2, mix_cols;
   // fused Legendre step (KK_LEG_STEP): 1 when the job's steps run it; its
   // waves per workgroup, strips (16 points) per wave and workgroups per block
   int leg_step, step_waves, step_strips, step_groups;
@@ -116,7 +123,9 @@ struct JobDev {
   const QD_G double2 *twr;        // [n2] exp(-2 pi i e / n2)
   // reduction scratch (per-workgroup partial sums)
   QD_G double *part_cor;          // [nO][5][units FFT_R] Coriolis, rotational energy, norm, acf re/im
-  QD_G double *part_ke;           // [units FFT_r]       kinetic energy of one (block, slab, column group)
+  QD_G double *part_ke;           // [units FFT_r]       kinetic energy of one (block, slab, column group);
+                                  // two-pass form: [nO][units FFT_RI] per R workgroup and block
+  QD_G double *part_ke2;          // two-pass form: [nO][units FFT_r] r-kinetic sums of the r pass
   QD_G double *part_obs;          // [nO][gemm_bwd_groups or step_groups] w V |psi|^2 (potential energy)
   QD_G double *obs_ring;          // [ring][nO][5] per-step observables
   QD_G double2 *acf_ring;         // [ring][nO] autocorrelation contribution of each block
@@ -282,6 +291,14 @@ int fft_ct_rows(int n1, int n_blocks, bool inverse);
 int fft_ct_cols(int n2);
 void launch_fft_R_ct(const StepArgs &a, int total_units, int inverse, int acf, int n1, int nb_max, hipStream_t stream);
 void launch_fft_r_ct(const StepArgs &a, int total_units, int n2, hipStream_t stream);
+// Two-pass form (compile-time lengths, up to 5 omega blocks): columns per
+// workgroup of the r pass with the first half step (0: no such pass for these
+// sizes, or turned off with QD_FFT_TWO_PASS=0), that r pass, and the R pass
+// with the R kinetic factor and the second half step (fft_ct_rows(n1,
+// blocks, true) rows per workgroup).
+int fft_mix_cols(int n2, int n_blocks);
+void launch_fft_r_mix(const StepArgs &a, int total_units, int n2, int nb_max, hipStream_t stream);
+void launch_fft_R_kin(const StepArgs &a, int total_units, int acf, int n1, int nb_max, hipStream_t stream);
 // standalone batched 1-D FFT of `count` contiguous pencils (test probe)
 void launch_fft_probe(double2 *data, const double2 *tw, int n, int count, int inverse, hipStream_t stream);
 void launch_reduce(const StepArgs &a, int total_units, hipStream_t stream);

@@ -38,7 +38,7 @@ double now_s() {
 // shapes and kernel forms (workgroup waves, paired or one-row strips, the
 // fused Legendre step) and the precision.
 using ClassKey = std::tuple<int, int, int, int, int, int, int, int, int, int, int, int, int, int, int, int, int,
-                            int, int, int>;
+                            int, int, int, int, int>;
 
 struct ServerJob {
   int id = 0;
@@ -215,7 +215,7 @@ struct qd_server {
     const ClassKey key{d.n1,      d.n2,     d.nt,       d.fwd_np,       d.bwd_np,           d.fwd_kp,
                        d.bwd_kp,  d.mixed,  d.fft_rows, d.fft_rows_inv, d.gemm_fwd_waves,   d.gemm_bwd_waves,
                        d.leg_fwd_paired, d.leg_bwd_paired, d.leg_step, d.step_waves, d.leg_split,
-                       d.sp_th, d.sp_te, d.sp_to};
+                       d.sp_th, d.sp_te, d.sp_to, d.fft2, d.mix_cols};
     for (auto &c : classes)
       if (c->key == key) return *c;
     std::unique_ptr<ShapeClass> c(new ShapeClass());

@@ -32,7 +32,10 @@ __device__ __forceinline__ void reduce_unit(const StepArgs &a, int unit, double 
   const int b = u / kRedFields, f = u % kRedFields;
   const QD_G double *src;
   long n;
-  if (f == 0) {   // kinetic energy: FFT_r units ordered by block, slab, column group
+  if (f == 0 && jb.fft2) {   // kinetic energy (two-pass form): [b][units] of the inverse R pass
+    n = jb.units[KK_FFT_RI];
+    src = jb.part_ke + (long)b * n;
+  } else if (f == 0) {   // kinetic energy: FFT_r units ordered by block, slab, column group
     const int gcols = jb.n1 / jb.fft_cols;
     long off = 0;
     for (int q = 0; q < b; q++) off += (long)(jb.L - (jb.omin + q)) * gcols;

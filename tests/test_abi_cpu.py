@@ -199,3 +199,25 @@ def test_legendre_form_choice():
     # 128 theta points (4 tiles per half): no instantiation of either fused form
     wide = inputs.build_job(inputs.JobSpec(n1=32, n2=16, ntheta=128, R_range=(2.0, 12.0), r_range=(0.5, 5.0)))
     assert api.legendre_form(abi.Inputs(wide, total_steps=1)) == 0
+
+
+def test_fft_pass_choice(monkeypatch):
+    """The FFT work's pass count (host-side choice, no GPU): two passes where
+    both lengths have compile-time kernels and the r pass with the first half
+    step has a tiling for the blocks (up to 5), three otherwise or with
+    QD_FFT_TWO_PASS=0."""
+    def passes(spec):
+        return api.fft_passes(abi.Inputs(inputs.build_job(spec), total_steps=1))
+    assert passes(inputs.config_spec("c3")) == 2          # 192 x 128, 5 blocks
+    assert passes(inputs.config_spec("c2")) == 2          # 256 x 192, 1 block
+    assert passes(inputs.config_spec("c1")) == 2          # 128 x 64, 1 block
+    nine = inputs.JobSpec(n1=64, n2=32, ntheta=24, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
+                          J=8, parity=0, k_max=8, omega0=0, j0=1)
+    assert passes(nine) == 3                              # 9 blocks
+    wide = inputs.JobSpec(n1=64, n2=192, ntheta=24, R_range=(2.0, 12.0), r_range=(0.5, 5.0),
+                          J=4, parity=0, k_max=-1, omega0=0, j0=1)
+    assert passes(wide) == 3                              # 5 blocks of 192-point r pencils: no tiling
+    odd = inputs.JobSpec(n1=40, n2=32, ntheta=12, R_range=(2.0, 12.0), r_range=(0.5, 5.0))
+    assert passes(odd) == 3                               # R length without compile-time kernels
+    monkeypatch.setenv("QD_FFT_TWO_PASS", "0")
+    assert passes(inputs.config_spec("c3")) == 3

@@ -421,6 +421,38 @@ def test_legendre_kernel_forms_match_oracle(tmp_path, switch):
     assert e_psi <= PSI_TOL and e_obs <= OBS_TOL
 
 
+FFT_FORM_CASES = [
+    # (grid, J, parity, k_max): the c3 block structure (5 blocks, r pass
+    # with 16 columns of 64 points), 3 blocks on 128-point r pencils (8
+    # columns), one block on 192-point r pencils (8 columns, two butterflies
+    # a thread), 2 blocks with a 96-point R pass
+    ((96, 64, 32), 20, 0, 4),
+    ((64, 128, 24), 2, 0, -1),
+    ((64, 192, 24), 0, 0, -1),
+    ((96, 96, 24), 3, 1, -1),
+]
+
+
+@pytest.mark.parametrize("passes", [2, 3])
+@pytest.mark.parametrize("case", range(len(FFT_FORM_CASES)))
+def test_parity_fft_pass_forms(monkeypatch, passes, case):
+    """The FFT work in two passes (the default where it has kernels: r pass
+    with the first rot.C.rot half step and the r kinetic factor, R pass with
+    the R kinetic factor and the second half step) and in three
+    (QD_FFT_TWO_PASS=0: R pass with the first half step, r pass with both
+    kinetic factors, R pass with the second half step), each against the
+    oracle; the two-pass kinetic energy is summed from its R and r parts."""
+    (n1, n2, nt), J, par, kmax = FFT_FORM_CASES[case]
+    if passes == 3:
+        monkeypatch.setenv("QD_FFT_TWO_PASS", "0")
+    spec = inputs.JobSpec(n1=n1, n2=n2, ntheta=nt, R_range=(2.0, 12.0), r_range=(0.5, 5.0), J=J, parity=par,
+                          k_max=kmax, omega0=par, j0=max(1, par), E_coll=0.02)
+    job = inputs.build_job(spec)
+    assert api.fft_passes(abi.Inputs(job, total_steps=1)) == passes
+    a, b = run_both(spec, 4)
+    check_parity(a, b)
+
+
 def test_parity_asymmetric_theta_grid():
     """A theta grid whose nodes are not symmetric under x -> -x (one node moved
     by 1e-6; the tables as given) takes the fused step with direct products
