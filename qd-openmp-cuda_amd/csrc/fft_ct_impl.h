@@ -34,10 +34,14 @@ struct PlanEnds {
   }
 };
 // PlanEnds with radix 4 at both ends (n a multiple of 16), for the r pass
-// that holds every block's points in registers: 4 per butterfly and block.
+// that holds every block's points in registers: 4 per butterfly and block;
+// in between radix 8 where it divides what remains (one LDS round trip for
+// 4 8 4 = 128), else the standard plan.
 struct PlanEnds4 {
   static constexpr int first(int) { return 4; }
-  static constexpr int radix(int n, int ns) { return (ns == 1 || ns * 4 == n) ? 4 : next_radix(n / (ns * 4)); }
+  static constexpr int radix(int n, int ns) {
+    return (ns == 1 || ns * 4 == n) ? 4 : next_radix(n / (ns * 4));
+  }
 };
 
 // R-pass LDS layout (SW).  16-byte LDS stores are served 8 lanes at a time
@@ -260,20 +264,28 @@ __host__ __device__ constexpr int ct_rows(int n1, int nb, bool inv) {
 // Columns per workgroup of the r pass: 16 or 8, so the boundary stages of
 // its radix plan have 128..256 butterflies for the 256 threads.
 __host__ __device__ constexpr int ct_cols(int n2) { return (n2 <= 64 || n2 == 128) ? 16 : 8; }
-// Columns per workgroup of the two-pass form's r pass (k_fft_r_mix).  The
-// tiles of every block present stay in registers through the Coriolis mix,
-// 4 points per radix-4 butterfly and ceil(Q n2 / 4 / kBlock) butterflies a
-// thread, so Q is the larger of 16 and 8 with a tile of at most 2048 points
-// and at most 20 points a thread over the instantiated block count (1, 2, 3
-// or 5), one butterfly a thread with more than one block (two spill); 0 (the
+// Radix of the end stages of the two-pass form's r pass: PlanEnds's 8 for
+// one block on pencils of 128 or more points that 64 divides (on 192 x 256
+// with 96 theta points: 44.5 -> 41.2 us; on 64 points, where half the
+// threads would idle at the ends, 10.1 -> 11.2), else 4 (PlanEnds4): with
+// more blocks all their points stay in registers through the Coriolis mix.
+__host__ __device__ constexpr int mix_radix(int n2, int nb) {
+  return nb <= 1 && n2 % 64 == 0 && n2 >= 128 ? 8 : 4;
+}
+// Columns per workgroup of that r pass (k_fft_r_mix): mix_radix points per
+// butterfly and ceil(Q n2 / radix / kBlock) butterflies a thread, so Q is the
+// larger of 16 and 8 with a tile of at most 2048 points and at most 20
+// points a thread over the instantiated block count (1, 2, 3 or 5), one
+// butterfly a thread with more than one block (two spill); 0 (the
 // three-pass form runs) above 5 blocks or where neither fits.  Narrower
 // tiles would read and write rows of less than 128 bytes.
 __host__ __device__ constexpr int mix_cols(int n2, int nb) {
   if (nb > 5 || n2 % 16 != 0) return 0;
   const int b = nb <= 3 ? (nb < 1 ? 1 : nb) : 5;
+  const int r1 = mix_radix(n2, b);
   for (int q = 16; q >= 8; q /= 2) {
-    const int mb = (q * n2 / 4 + kBlock - 1) / kBlock;
-    if (q * n2 <= 2048 && b * 4 * mb <= 20 && (b == 1 || mb == 1)) return q;
+    const int mb = (q * n2 / r1 + kBlock - 1) / kBlock;
+    if (q * n2 <= 2048 && b * r1 * mb <= 20 && (b == 1 || mb == 1)) return q;
   }
   return 0;
 }
@@ -317,7 +329,7 @@ __device__ __forceinline__ void mix_first_stage(double2 (&x)[NB][XPT], VPtr V, i
 // energy, then the inverse DFT.
 // ---------------------------------------------------------------------------
 template <bool INV, int NB, int N1, bool ACF, bool KIN = false>
-__global__ __launch_bounds__(kBlock) void k_fft_R_ct(StepArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KIN && NB == 5 && N1 == 192 ? 4 : 1))) void k_fft_R_ct(StepArgs a) {
   static_assert(INV || !KIN, "the R kinetic factor rides on the inverse pass");
   constexpr int NF = ACF ? 5 : 3;
   constexpr int P = ct_rows(N1, NB, INV);
@@ -846,17 +858,21 @@ __global__ __launch_bounds__(kBlock) void k_fft_r_ct(StepArgs a) {
 // (Parseval along R), its R part in the R pass.
 // Workgroup (l, g) takes columns i = g Q .. g Q + Q - 1 of slab l in every
 // block present; each block's points stay in registers from the load to its
-// first DFT stage, as the inputs of the thread's radix-4 butterflies.  The
-// output is left multiplied by n2 (unscaled transform pair): the R pass
-// scales by 1 / (n1 n2).
+// first DFT stage, as the inputs of the thread's butterflies of radix
+// mix_radix.  The output is left multiplied by n2 (unscaled transform pair):
+// the R pass scales by 1 / (n1 n2).  At most 3 waves per SIMD (the register
+// budget of mix_cols; 4 would spill), and the two-transform R pass of 5
+// blocks of 192 points at 4 (4 registers spill; 77 against 82 us on
+// 192 x 128 x 64 with 5 blocks).
 // ---------------------------------------------------------------------------
 template <int N2, int NB>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_fft_r_mix(StepArgs a) {
   constexpr int Q = mix_cols(N2, NB);
   static_assert(Q > 0 && kBlock % Q == 0, "no r-pass tiling with the mix for these sizes");
   constexpr int QN = Q * N2;
-  constexpr int R1 = PlanEnds4::first(N2), NB1 = N2 / R1, NSL = NB1;
-  static_assert(N2 % (R1 * R1) == 0, "PlanEnds4 needs n2 to be a multiple of 16");
+  using RP = std::conditional_t<mix_radix(N2, NB) == 8, PlanEnds, PlanEnds4>;
+  constexpr int R1 = RP::first(N2), NB1 = N2 / R1, NSL = NB1;
+  static_assert(R1 == mix_radix(N2, NB) && N2 % (R1 * R1) == 0, "end stages of radix R1 need n2 % R1^2 == 0");
   constexpr int T1 = Q * NB1, MB = (T1 + kBlock - 1) / kBlock;
   __shared__ double2 A[QN];   // column p element j at A[p + j Q]
   __shared__ double2 tw[N2];
@@ -950,7 +966,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         }
       }
       __syncthreads();
-      stages_ip<false, N2, R1, Q, 1, Q, true, PlanEnds4, NSL>(A, tw);
+      stages_ip<false, N2, R1, Q, 1, Q, true, RP, NSL>(A, tw);
       // forward last stage into registers, the r factor and energy, the
       // inverse first stage
 #pragma unroll
@@ -982,7 +998,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         }
       }
       __syncthreads();
-      stages_ip<true, N2, R1, Q, 1, Q, true, PlanEnds4, NSL>(A, tw);
+      stages_ip<true, N2, R1, Q, 1, Q, true, RP, NSL>(A, tw);
       // inverse last stage, written to HBM
 #pragma unroll
       for (int m = 0; m < MB; m++) {
