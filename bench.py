@@ -57,14 +57,17 @@ def kernel_model(job, form=2):
     FFT passes (R forward with the first rot.C.rot half step, r forward .
     kinetic . r inverse, R inverse with the second half step) reads and writes
     the FBR state, 32 F_b; the backward Legendre pass reads the FBR state and
-    writes psi, 16 F_b + 16 G.  Total 40 G + 128 F_b.  Flops: the real-B
+    writes psi, 16 F_b + 16 G.  Total 40 G + 128 F_b.  In the two-pass form
+    of the FFT work (fft_r with the first half step and the r kinetic
+    factor, fft_R_inv with the R kinetic factor and the second half step)
+    the two passes take 32 F_b each (step_kinds).  Flops: the real-B
     Legendre product, 4 n1 n2 ntheta N_l per block and direction.  Kinds fused
     into these kernels (dvr, fbr1, fbr2) carry no traffic of their own; the
     flux kernel runs only when reaction probabilities are requested.  The
     fused Legendre step (leg_step: the backward pass of one step with the
     forward pass of the next, psi kept on chip) reads the FBR state and V and
     writes the FBR state, 8 G + 32 F_b, with both products' flops; a step
-    then moves 8 G + 128 F_b.  In its parity-split form (`form` 2, the
+    then moves 8 G + 128 F_b, or 8 G + 96 F_b with two FFT passes.  In its parity-split form (`form` 2, the
     default on Gauss-Legendre grids) each product runs over half the theta
     grid, so both together take 4 n1 n2 ntheta N_l per block."""
     r1, r2, th, os_ = job["r1"], job["r2"], job["theta"], job["omega_states"]
