@@ -211,16 +211,20 @@ const char *qd_last_global_error(void);
 enum {
   QD_K_DVR = 0,      /* absorber and potential phase: fused into QD_K_LEG_FWD      */
   QD_K_LEG_FWD = 1,  /* forward Legendre transform (with the DVR phase)            */
-  QD_K_FBR1 = 2,     /* first rotational-Coriolis-rotational half step: fused into QD_K_FFT_R_FWD */
-  QD_K_FFT_R_FWD = 3,/* forward FFT along R                                        */
-  QD_K_FFT_r = 4,    /* FFT along r, kinetic phase, inverse FFT                    */
-  QD_K_FFT_R_INV = 5,/* inverse FFT along R, 1/(n1 n2) scale, second half step     */
+  QD_K_FBR1 = 2,     /* first rotational-Coriolis-rotational half step: fused into QD_K_FFT_R_FWD
+                        (into QD_K_FFT_r in the two-pass FFT form, qd_probe_fft_passes) */
+  QD_K_FFT_R_FWD = 3,/* forward FFT along R (reads 0 in the two-pass FFT form)        */
+  QD_K_FFT_r = 4,    /* FFT along r, kinetic phase, inverse FFT (two-pass form: the first
+                        half step and the r factor of the kinetic phase)            */
+  QD_K_FFT_R_INV = 5,/* inverse FFT along R, 1/(n1 n2) scale, second half step (two-pass
+                        form: forward FFT along R and the R factor before them)     */
   QD_K_FBR2 = 6,     /* second half step: fused into QD_K_FFT_R_INV                */
   QD_K_LEG_BWD = 7,  /* backward Legendre transform (with the potential energy)    */
   QD_K_OBS = 8,      /* flux surface values (when CRP is requested; in the fused step
-                        sequence carried by QD_K_FFT_R_FWD)                         */
+                        sequence carried by the first FFT pass: QD_K_FFT_R_FWD, or
+                        QD_K_FFT_r in the two-pass form)                            */
   QD_K_REDUCE = 9,   /* final reduction of the per-step observables (where the fused
-                        Legendre step runs, part of QD_K_FFT_R_FWD)                */
+                        Legendre step runs, part of the first FFT pass likewise)    */
   QD_K_LEG_STEP = 10,/* fused Legendre step: backward transform of the previous step,
                         potential energy, DVR phase and forward transform of this one;
                         where it runs, QD_K_LEG_FWD and QD_K_LEG_BWD read 0           */

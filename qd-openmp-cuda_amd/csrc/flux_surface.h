@@ -1,5 +1,5 @@
 // flux_surface.h - the flux surface work of one step (kernels_flux.hip),
-// also run as leading workgroups of the forward R pass.
+// also run as leading workgroups of the step's first FFT pass.
 #pragma once
 #include "qd_internal.h"
 #include "qd_complex.h"
@@ -34,10 +34,10 @@ __device__ __forceinline__ void flux_unit(const StepArgs &a, int u) {
   W[jb.flux_S + s] = d;
 }
 
-// The surface work carried by a forward R pass of the fused step sequence:
+// The surface work carried by the first FFT pass of the fused step sequence:
 // its workgroups red_units .. red_units + flux_units - 1 (after the carried
 // reduction's) write the previous step's surface values (gated); returns
-// true for those, and the R work takes the rest.  The R pass reads and
+// true for those, and the FFT work takes the rest.  The FFT pass reads and
 // writes neither the surface rows nor the window, and the step counter and
 // pending flag the surface work reads do not change during it.
 __device__ __forceinline__ bool flux_carried(const StepArgs &a) {

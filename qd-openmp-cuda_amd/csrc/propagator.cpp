@@ -996,7 +996,7 @@ void Engine::enqueue(hipEvent_t *ev) {
     launch_step_kernel(args(KK_LEG_STEP), total_[KK_LEG_STEP], 0, s);
     end(QD_K_LEG_STEP);
     // the reduction and the flux surface values of that step ride on the
-    // forward R pass below
+    // first FFT pass below
   } else {
     begin(QD_K_LEG_FWD);
     launch_leg_fwd(args(KK_LEG_FWD), total_[KK_LEG_FWD], fwd_tiles_, fwd_kp_, mixed_, fwd_waves_,
@@ -1089,7 +1089,7 @@ void Engine::finish(int q) {
 unsigned Engine::timed_kinds() const {
   unsigned m = (fft2_ ? 0u : (1u << QD_K_FFT_R_FWD)) | (1u << QD_K_FFT_r) | (1u << QD_K_FFT_R_INV);
   m |= fused_ ? (1u << QD_K_LEG_STEP) : ((1u << QD_K_LEG_FWD) | (1u << QD_K_LEG_BWD) | (1u << QD_K_REDUCE));
-  if (any_flux_ && !fused_) m |= 1u << QD_K_OBS;   // fused: carried by the forward R pass
+  if (any_flux_ && !fused_) m |= 1u << QD_K_OBS;   // fused: carried by the first FFT pass
   return m;
 }
 

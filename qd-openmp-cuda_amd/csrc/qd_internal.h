@@ -169,7 +169,8 @@ struct StepArgs {
   // they finish the previous step of each job whose backward transform was
   // pending (counters[3]) and skip the others; 0 after a separate backward pass
   int gate;
-  // forward R pass of the fused step sequence: its first red_units
+  // first FFT pass of the fused step sequence (the forward R pass, or the
+  // r pass with the first half step in the two-pass form): its first red_units
   // workgroups run the (gated) reduction of the previous step, with the
   // reduction's prefix table red_base (0: none)
   const QD_G int *red_base;

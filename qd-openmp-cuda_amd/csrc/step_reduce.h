@@ -1,6 +1,7 @@
 // step_reduce.h - one workgroup of the per-step observable reduction, shared
-// by the reduction kernel and the forward R passes of the fused step
-// sequence (which carry the previous step's reduction as leading workgroups).
+// by the reduction kernel and the first FFT pass of the fused step sequence
+// (the forward R pass, or the r pass with the first half step in the two-pass
+// form), which carries the previous step's reduction as leading workgroups.
 #pragma once
 
 #include "qd_internal.h"
@@ -87,9 +88,10 @@ __device__ __forceinline__ void reduce_unit(const StepArgs &a, int unit, double 
   }
 }
 
-// The reduction carried by a forward R pass of the fused step sequence: its
+// The reduction carried by the first FFT pass of the fused step sequence: its
 // first a.red_units workgroups reduce (gated) the previous step; returns true
-// for those, and the R work takes the rest (unit = blockIdx.x - red_units).
+// for those, and the FFT work takes the rest (unit = blockIdx.x - red_units,
+// less the carried flux work's units).
 __device__ __forceinline__ bool reduce_carried(const StepArgs &a, double *red) {
   if ((int)blockIdx.x >= a.red_units) return false;
   StepArgs r = a;
