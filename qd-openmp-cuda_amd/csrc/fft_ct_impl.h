@@ -405,12 +405,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KIN && N
   // every global load of the slab tile in flight at once, issued before the
   // table staging below so that the round trips overlap; loads are
   // unconditional from in-bounds addresses (block nb-1, last point), and
-  // values of blocks >= nb and points >= PN are never used
+  // values of blocks >= nb and points >= PN are never used.  The inverse
+  // pass, which takes the blocks one at a time through its transforms before
+  // the mix: the first PRE blocks only, and block b + PRE at the start of
+  // block b, so the slab's loads spread over the workgroup's transform work
+  // (192 x 128 x 64, 5 blocks: two-transform pass 77.5 -> 72.0 us, inverse
+  // 52.0 -> 49.3; 9 blocks of 256 x 192 x 96: inverse 363.8 -> 348.8; three
+  // blocks ahead: 80.9 with spills)
+  constexpr int PRE = INV ? 2 : NB;
   double2 x[NB][XPT];
-#pragma unroll
-  for (int b = 0; b < NB; b++)
+  auto load_block = [&](int b) {
 #pragma unroll
     for (int k = 0; k < XPT; k++) x[b][k] = base[(b < nb ? b : nb - 1) * LM + (eok(k) ? ept(k) : 0)];
+  };
+#pragma unroll
+  for (int b = 0; b < PRE && b < NB; b++) load_block(b);
   {
     // the twiddles and the slab's phases: every load first (clamped in-range
     // indices), then the LDS writes, so the staging costs one round trip
@@ -526,6 +535,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KIN && N
 #pragma unroll
     for (int b = 0; b < NB; b++) {
       double ks = 0.0;   // KIN: R kinetic energy of this thread's points of block b
+      if constexpr (INV) {
+        if (b + PRE < NB) load_block(b + PRE);
+      }
       if (b < nb) {
 #pragma unroll
         for (int k = 0; k < XPT; k++)
