@@ -96,6 +96,28 @@ def test_serve_same_shape_different_gemm_workgroups():
         _same(alone, mixed)
 
 
+def test_serve_same_grid_both_fft_forms():
+    # 64 x 32 x 24 with 2 and 4 blocks (two FFT passes: the r pass carries the
+    # first half step) and 7 blocks (three passes: more than 5 blocks do not
+    # fit the r pass's registers) in one server: three shape classes, each job
+    # bitwise equal to its run alone
+    specs = [_spec(64, 32, 24, 1, 1, -1, 14), _spec(64, 32, 24, 3, 1, -1, 14), _spec(64, 32, 24, 6, 0, -1, 14)]
+    _say("case serve: same grid, 2, 4 and 7 blocks, two- and three-pass FFT forms")
+    pairs = [_pair(s) for s in specs]
+    assert [len(a.omegas) for a, _ in pairs] == [2, 4, 7]
+    assert [api.fft_passes(a) for a, _ in pairs] == [2, 2, 3]
+    with api.Server() as srv:
+        ids = [srv.submit(m) for _, m in pairs]
+        for _ in range(100):
+            nr, nq = srv.step(8)
+            if nr == 0 and nq == 0:
+                break
+        st = [srv.status(i) for i in ids]
+        assert all(s["state"] == api.Server.DONE for s in st), [srv.error(i) for i in ids]
+    for alone, mixed in pairs:
+        _same(alone, mixed)
+
+
 def test_serve_admits_a_few_jobs_per_step():
     # six jobs submitted at once, budget for all: each step admits at most
     # four, so the first step leaves two queued; results are those alone
