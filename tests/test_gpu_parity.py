@@ -430,6 +430,8 @@ FFT_FORM_CASES = [
     ((64, 128, 24), 2, 0, -1),
     ((64, 192, 24), 0, 0, -1),
     ((96, 96, 24), 3, 1, -1),
+    # 4 blocks in the 5-block kernels (one block absent everywhere)
+    ((64, 64, 24), 4, 1, -1),
     # R pencils of 128 / 192 / 256 points: the R pass with the kinetic
     # factor on wave-local tiles (5, 3 and 1 blocks)
     ((128, 64, 24), 4, 0, -1),
