@@ -874,8 +874,9 @@ __global__ __launch_bounds__(kBlock) void k_fft_r_ct(StepArgs a) {
 // mix_radix.  The output is left multiplied by n2 (unscaled transform pair):
 // the R pass scales by 1 / (n1 n2).  At most 3 waves per SIMD (the register
 // budget of mix_cols; 4 would spill), and the two-transform R pass of 5
-// blocks of 192 points at 4 (4 registers spill; 77 against 82 us on
-// 192 x 128 x 64 with 5 blocks).
+// blocks of 192 points at 4 (4 registers spill; on 192 x 128 x 64 with 5
+// blocks 77 against 82 us at 3, measured before its blocks were loaded two
+// ahead).
 // ---------------------------------------------------------------------------
 template <int N2, int NB>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_fft_r_mix(StepArgs a) {
