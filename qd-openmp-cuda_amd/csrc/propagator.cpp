@@ -218,8 +218,12 @@ JobShape validate_inputs(const qd_inputs *in) {
       rsum[r] = a;
     }
   };
+  // one thread per MB, at most 8 (and the host's count): a 6 MB job of
+  // 128 x 64 x 48 takes 0.8 ms on one thread, which a burst of them paid
+  // before the server's first step
   const long bytes = 16L * G * s.nO;
-  const int nthr = bytes < (64L << 20) ? 1 : std::min(8, std::max(1, (int)std::thread::hardware_concurrency()));
+  const int nthr = (int)std::min<long>(std::min(8, std::max(1, (int)std::thread::hardware_concurrency())),
+                                       std::max(1L, bytes >> 20));
   if (nthr == 1) {
     sum_rows(0, rows);
   } else {
